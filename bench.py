@@ -1,0 +1,138 @@
+"""Benchmark: plain F3R solve time to a 1e-8 fp64 relative residual on one
+MI355X, and the fp16 SpMV HBM bandwidth of the solver's matrix layout.
+
+Workload (one step = one full solve from x0 = 0): the 27-point
+nonsymmetric stencil on a 128^3 grid (2,097,152 rows, 55,742,968 nonzeros),
+diagonal 26, off-diagonals -1 + 0.5*sign(dx), diagonally scaled as the
+reference driver does; b uniform in [-1, 1) from a seed; SDAINV (tol 0.1,
+acc 1.0); m = (100; 8, 4, 2), c = 64, restart 20.  All inputs are resident in
+HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--no-cpu]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solve time to 1e-8 fp64 rel. residual, learned vs plain F3R; fp16 SpMV HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(A, b, outer_gpu, scale):
+    """The oracle's F3R (the CPU restatement of the reference, fp16 emulated
+    through F16C conversions) on the same matrix, timed on the host's cores
+    over a bounded sample: one outer step, scaled to the GPU's outer count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = O.lib().orc_num_threads()
+    _, r, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=scale,
+                          max_outer=1)
+    per_outer = r.solve_s
+    return {"value": per_outer * outer_gpu, "unit": "s", "cores": int(r.nthreads),
+            "kind": "port",
+            "sample": f"1 outer FGMRES step of the same 128^3 solve ({per_outer:.2f} s, "
+                      f"{r.precond_calls} fp16 Richardson calls) scaled by the GPU's {outer_gpu} outer steps; "
+                      f"{threads} OpenMP threads; setup excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the solve is not sharded across GPUs "
+                                   "(see DESIGN.md for how it would shard); run with --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import f3r_amd as F
+
+    m = args.grid
+    t0 = time.time()
+    A = F.CSR.stencil27(m, beta=0.5)
+    F.algorithm.scaling(A)
+    b = F.uniform(A.n, -1.0, 1.0, seed=2024)
+    log(f"[bench] matrix {m}^3: n={A.n} nnz={A.nnz} ({time.time() - t0:.1f} s)")
+    ctx = F.Context(0)
+    t0 = time.time()
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    S = F.F3R(ctx, A, P, use_graph=not args.no_graph)
+    setup_s = time.time() - t0
+    log(f"[bench] setup {setup_s:.1f} s")
+    bv = F.vector.from_numpy(ctx, b)
+    xv = F.vector(ctx, "f64", A.n)
+    for w in range(args.warmup):
+        xv.fill(0.0)
+        r = S.solve(bv, xv)
+        log(f"[bench] warmup {w}: outer={r.res_iter} solved={r.is_solved} {r.solve_s * 1e3:.1f} ms")
+    times = []
+    res = None
+    for k in range(args.steps):
+        xv.fill(0.0)
+        ctx.sync()
+        res = S.solve(bv, xv)   # synchronises before it returns
+        times.append(res.solve_s)
+        log(f"[bench] step {k}: outer={res.res_iter} solved={res.is_solved} {res.solve_s * 1e3:.2f} ms")
+    # true residual of the last solve, fp64 on the device
+    A64 = F.SELL(ctx, A, "f64", C=64, sigma=1, K=4)
+    rv = F.vector(ctx, "f64", A.n)
+    A64.compute_residual(bv, xv, rv)
+    true_rel = F.nrm(rv) / F.nrm(bv)
+    # dominant kernel: the fp16 SpMV of the solver's layout, timed with HIP
+    # events on the solver's stream
+    ms16, by16 = S.time_kernel(0, reps=50)
+    kern = {}
+    for w, name in ((1, "spmv_f16val_f32vec"), (2, "spmv_f32"), (3, "spmv_f64")):
+        ms, by = S.time_kernel(w, reps=20)
+        kern[name] = {"us": round(ms * 1e3, 2), "GB/s": round(by / ms / 1e6, 1)}
+    ach = by16 / (ms16 * 1e-3) / 1e9
+    mean_s = float(np.mean(times))
+    out = {
+        "metric": METRIC,
+        "value": round(mean_s, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mean_s * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f16/f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain F3R, 27-point nonsymmetric {m}^3 (beta=0.5 along x), SDAINV tol 0.1, "
+                               f"m=(100;8,4,2), c=64, restart 20, eps 1e-8",
+                   "n": A.n, "nnz": A.nnz, "outer_iters": res.res_iter, "solved": bool(res.is_solved),
+                   "true_rel_res": float(true_rel), "precond_calls": res.precond_calls,
+                   "setup_s": round(setup_s, 2), "graph": not args.no_graph},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "fp16 SELL SpMV y = A16 x (fp16 x, y; fp32 sums)",
+                     "us_per_launch": round(ms16 * 1e3, 2), "algorithmic_bytes": by16},
+        "kernels": kern,
+    }
+    if not args.no_cpu:
+        scale = float(2.0 ** np.floor(np.log2(np.sqrt(A.n) / 4.0)))
+        log("[bench] cpu baseline (oracle, one outer step) ...")
+        out["cpu_baseline"] = cpu_baseline(A, b, res.res_iter, scale)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,419 @@
+"""f3r_amd -- MI355X-native F3R nested mixed-precision solver.
+
+Python mirror of the reference's host interface (suzuki-hpc/F3R,
+library/umineko-sparse): CSR / SELL matrices, SDAINV / ILU(0)
+preconditioners and the four-level F3R solve, over the C ABI of
+libf3r_amd.so (include/f3r_amd.h).  All compute runs in the HIP library;
+this module only moves arrays and handles.
+"""
+import ctypes
+from collections import namedtuple
+
+import numpy as np
+
+from . import _capi
+from ._capi import F16, F32, F64, I32, F3RError, check, lib
+
+__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "algorithm", "solve_res_t",
+           "F3RError", "F64", "F32", "F16", "device_count"]
+
+_NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32}
+_DT = {"f64": F64, "f32": F32, "f16": F16, "i32": I32, "double": F64, "float": F32, "half": F16}
+
+
+def _dtype(d):
+    if isinstance(d, int):
+        return d
+    if isinstance(d, str):
+        return _DT[d]
+    return {np.dtype(np.float64): F64, np.dtype(np.float32): F32, np.dtype(np.float16): F16,
+            np.dtype(np.int32): I32}[np.dtype(d)]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    check(lib.f3r_device_count(ctypes.byref(c)), "device_count")
+    return c.value
+
+
+class Context:
+    """One GPU and one HIP stream (replaces exec<device> on the default stream)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib.f3r_ctx_create(device, ctypes.byref(h)), "ctx_create")
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        check(lib.f3r_ctx_sync(self.handle), "ctx_sync")
+
+    def close(self):
+        if self.handle:
+            lib.f3r_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class vector:
+    """kmm::vector<T, device> (core/tensor.hpp:204-316)."""
+
+    def __init__(self, ctx, dtype, n, _handle=None, _base=None):
+        self.ctx = ctx
+        self.dtype = _dtype(dtype)
+        self.n = int(n)
+        self._base = _base
+        if _handle is None:
+            h = ctypes.c_void_p()
+            check(lib.f3r_vec_create(ctx.handle, self.dtype, self.n, ctypes.byref(h)), "vec_create")
+            self.handle = h
+        else:
+            self.handle = _handle
+
+    @classmethod
+    def from_numpy(cls, ctx, arr, dtype=None):
+        arr = np.asarray(arr)
+        dt = _dtype(dtype) if dtype is not None else _dtype(arr.dtype)
+        v = cls(ctx, dt, arr.size)
+        v.upload(arr)
+        return v
+
+    def view(self, offset, n):
+        h = ctypes.c_void_p()
+        check(lib.f3r_vec_view(self.handle, offset, n, ctypes.byref(h)), "vec_view")
+        return vector(self.ctx, self.dtype, n, _handle=h, _base=self)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(np.asarray(arr).reshape(-1), dtype=_NP[self.dtype])
+        if a.size != self.n:
+            raise ValueError("upload: size mismatch")
+        check(lib.f3r_vec_upload(self.handle, _p(a)), "vec_upload")
+        return self
+
+    def download(self):
+        out = np.empty(self.n, _NP[self.dtype])
+        check(lib.f3r_vec_download(self.handle, _p(out)), "vec_download")
+        return out
+
+    def fill(self, v):
+        check(lib.f3r_vec_fill(self.handle, float(v)), "vec_fill")
+        return self
+
+    def copy(self, src):
+        """self = T(src) (vector::copy, tensor.hpp:272-277)"""
+        check(lib.f3r_vec_copy(src.handle, self.handle), "vec_copy")
+        return self
+
+    def axpy(self, alpha, x):
+        """self += alpha * x"""
+        check(lib.f3r_vec_axpy(float(alpha), x.handle, self.handle), "vec_axpy")
+        return self
+
+    def scale(self, alpha):
+        check(lib.f3r_vec_scale(self.handle, float(alpha)), "vec_scale")
+        return self
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_vec_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def dot(a, b):
+    """DOT (core/tensor.hpp:509-540)"""
+    out = ctypes.c_double()
+    check(lib.f3r_vec_dot(a.handle, b.handle, ctypes.byref(out)), "vec_dot")
+    return out.value
+
+
+def nrm(a):
+    """NRM (core/tensor.hpp:542-571)"""
+    out = ctypes.c_double()
+    check(lib.f3r_vec_nrm2(a.handle, ctypes.byref(out)), "vec_nrm2")
+    return out.value
+
+
+class CSR:
+    """CSR<double, host> (sparse/matrix/csr.hpp:12-50): int32 indices, fp64 values."""
+
+    def __init__(self, n, rptr, idx, val, ncols=None):
+        self.n = int(n)
+        self.ncols = int(ncols if ncols is not None else n)
+        self.rptr = np.ascontiguousarray(rptr, np.int32)
+        self.idx = np.ascontiguousarray(idx, np.int32)
+        self.val = np.ascontiguousarray(val, np.float64)
+        self._dev = {}
+
+    @property
+    def nnz(self):
+        return int(self.idx.size)
+
+    def nrows(self):
+        return self.n
+
+    def handle(self, ctx=None):
+        """C-ABI matrix handle on ctx (None: host-only)."""
+        key = id(ctx) if ctx is not None else None
+        if key not in self._dev:
+            h = ctypes.c_void_p()
+            check(lib.f3r_mat_create_csr(ctx.handle if ctx is not None else None, self.n, self.ncols,
+                                         self.nnz, _p(self.rptr), _p(self.idx), _p(self.val),
+                                         ctypes.byref(h)), "mat_create_csr")
+            self._dev[key] = (h, ctx)
+        return self._dev[key][0]
+
+    def invalidate(self):
+        for h, _ in self._dev.values():
+            lib.f3r_mat_destroy(h)
+        self._dev = {}
+
+    def __del__(self):
+        try:
+            self.invalidate()
+        except Exception:
+            pass
+
+    # ---- synthetic inputs (host) ----
+    @classmethod
+    def stencil27(cls, nx, ny=None, nz=None, beta=0.0):
+        ny, nz = ny or nx, nz or nx
+        nnz = ctypes.c_int64()
+        check(lib.f3r_gen_stencil27_nnz(nx, ny, nz, ctypes.byref(nnz)), "gen_stencil27_nnz")
+        n = nx * ny * nz
+        rptr = np.empty(n + 1, np.int32)
+        idx = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float64)
+        check(lib.f3r_gen_stencil27(nx, ny, nz, beta, _p(rptr), _p(idx), _p(val)), "gen_stencil27")
+        return cls(n, rptr, idx, val)
+
+    @classmethod
+    def convdiff7(cls, nx, ny=None, nz=None, sigma=1.5, corr=8.0, peclet=10.0, seed=0):
+        ny, nz = ny or nx, nz or nx
+        nnz = ctypes.c_int64()
+        check(lib.f3r_gen_convdiff7_nnz(nx, ny, nz, ctypes.byref(nnz)), "gen_convdiff7_nnz")
+        n = nx * ny * nz
+        rptr = np.empty(n + 1, np.int32)
+        idx = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float64)
+        check(lib.f3r_gen_convdiff7(nx, ny, nz, sigma, corr, peclet, seed, _p(rptr), _p(idx), _p(val)),
+              "gen_convdiff7")
+        return cls(n, rptr, idx, val)
+
+    @classmethod
+    def dominant(cls, npts, nx, ny=None, nz=None, margin=1.0):
+        ny, nz = ny or nx, nz or nx
+        nnz = ctypes.c_int64()
+        check(lib.f3r_gen_dominant_nnz(npts, nx, ny, nz, ctypes.byref(nnz)), "gen_dominant_nnz")
+        n = nx * ny * nz
+        rptr = np.empty(n + 1, np.int32)
+        idx = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float64)
+        check(lib.f3r_gen_dominant(npts, nx, ny, nz, margin, _p(rptr), _p(idx), _p(val)), "gen_dominant")
+        return cls(n, rptr, idx, val)
+
+
+def uniform(n, lo=-1.0, hi=1.0, seed=0):
+    out = np.empty(n, np.float64)
+    check(lib.f3r_gen_uniform(n, lo, hi, seed, _p(out)), "gen_uniform")
+    return out
+
+
+class algorithm:
+    """sparse/matrix/algorithm.hpp"""
+
+    @staticmethod
+    def scaling(A):
+        """a_ij /= sqrt|a_ii| sqrt|a_jj| in place (algorithm.hpp:45-59)"""
+        check(lib.f3r_csr_scaling(A.n, _p(A.rptr), _p(A.idx), _p(A.val)), "csr_scaling")
+        A.invalidate()
+        return A
+
+
+def sell_layout(A, C=32, sigma=1, K=1):
+    """Host SELL-C-sigma layout (cptr, cwidth, idx, val, perm) of a CSR."""
+    ns = ctypes.c_int64()
+    check(lib.f3r_sell_nslots(A.n, _p(A.rptr), C, sigma, K, ctypes.byref(ns)), "sell_nslots")
+    nch = (A.n + C - 1) // C
+    cptr = np.empty(nch + 1, np.int32)
+    cw = np.empty(max(nch, 1), np.int32)
+    sidx = np.empty(ns.value, np.int32)
+    sval = np.empty(ns.value, np.float64)
+    perm = np.empty(A.n, np.int32)
+    check(lib.f3r_sell_layout(A.n, _p(A.rptr), _p(A.idx), _p(A.val), C, sigma, K, _p(cptr), _p(cw),
+                              _p(sidx), _p(sval), _p(perm)), "sell_layout")
+    return cptr, cw[:nch], sidx, sval, perm
+
+
+class SELL:
+    """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166)."""
+
+    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, _handle=None):
+        self.ctx = ctx
+        self.A = A
+        self.dtype = _dtype(dtype)
+        self.C, self.sigma, self.K = C, sigma, K
+        if _handle is None:
+            h = ctypes.c_void_p()
+            check(lib.f3r_mat_to_sell(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, ctypes.byref(h)),
+                  "mat_to_sell")
+            self.handle = h
+        else:
+            self.handle = _handle
+
+    def cast(self, dtype):
+        """cross-precision copy sharing the index arrays (sell.hpp:51-53)"""
+        h = ctypes.c_void_p()
+        check(lib.f3r_mat_cast(self.handle, _dtype(dtype), ctypes.byref(h)), "mat_cast")
+        return SELL(self.ctx, self.A, dtype, self.C, self.sigma, self.K, _handle=h)
+
+    def nslots(self):
+        ns = ctypes.c_int64()
+        check(lib.f3r_mat_info(self.handle, None, None, ctypes.byref(ns), None, None), "mat_info")
+        return ns.value
+
+    def operate(self, x, y):
+        check(lib.f3r_spmv(self.handle, x.handle, y.handle), "spmv")
+
+    def compute_residual(self, b, x, r):
+        check(lib.f3r_residual(self.handle, b.handle, x.handle, r.handle), "residual")
+
+    def time_spmv(self, x, y, reps=20):
+        ms = ctypes.c_double()
+        check(lib.f3r_spmv_time(self.handle, x.handle, y.handle, reps, ctypes.byref(ms)), "spmv_time")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_mat_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class _Precond:
+    def factors(self):
+        out = []
+        for w in (0, 1):
+            nnz = ctypes.c_int64()
+            check(lib.f3r_prec_factor_nnz(self.handle, w, ctypes.byref(nnz)), "factor_nnz")
+            r = np.empty(self.n + 1, np.int32)
+            i = np.empty(nnz.value, np.int32)
+            v = np.empty(nnz.value, np.float64)
+            check(lib.f3r_prec_factor_get(self.handle, w, _p(r), _p(i), _p(v)), "factor_get")
+            out.append((r, i, v))
+        return out
+
+    def operate(self, x, y):
+        check(lib.f3r_prec_apply(self.handle, x.handle, y.handle), "prec_apply")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_prec_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class SDAINV(_Precond):
+    """SDAINV<T, device>(A, {tol, acc}) (sparse/preconditioner/ainv.hpp:129-147).
+    factors() -> (Z, W^T) as CSR triples.  ctx=None builds the host factors only."""
+
+    def __init__(self, A, tol=0.1, acc=1.0, dtype="f16", ctx=None):
+        self.n = A.n
+        self.kind = 0
+        self.dtype = _dtype(dtype)
+        h = ctypes.c_void_p()
+        check(lib.f3r_sdainv_create(ctx.handle if ctx else None, A.handle(ctx), tol, acc, self.dtype,
+                                    ctypes.byref(h)), "sdainv_create")
+        self.handle = h
+
+
+class ILU0(_Precond):
+    """ILU(0) on `blocks` diagonal blocks + split (ilu.hpp:128-164,
+    algorithm.hpp:8-43).  factors() -> (L, U)."""
+
+    def __init__(self, A, blocks=1, acc=1.0, dtype="f16", ctx=None):
+        self.n = A.n
+        self.kind = 1
+        self.dtype = _dtype(dtype)
+        h = ctypes.c_void_p()
+        check(lib.f3r_ilu0_create(ctx.handle if ctx else None, A.handle(ctx), blocks, acc, self.dtype,
+                                  ctypes.byref(h)), "ilu0_create")
+        self.handle = h
+
+
+solve_res_t = namedtuple("solve_res_t", "is_solved res_iter res_nrm2 nrm_b solve_s precond_calls outer_cycles")
+
+VARIANTS = {"half": 0, "float": 1, "double": 2}
+
+
+class F3R:
+    """Restarted<double>(A, FGMRES<double>(A, FGMRES<float>(A32, FGMRES<float>(A16,
+    Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU."""
+
+    def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
+                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True):
+        self.ctx = ctx
+        self.A = A
+        self.M = M
+        p = _capi.Params()
+        check(lib.f3r_params_default(ctypes.byref(p)), "params_default")
+        p.m1, p.m2, p.m3, p.m4 = m
+        p.c, p.restart, p.eps = c, restart, eps
+        p.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        p.range_scale = range_scale
+        p.C, p.sigma, p.K = C, sigma, K
+        p.use_graph = 1 if use_graph else 0
+        p.flags = 1 if history else 0
+        self.params = p
+        h = ctypes.c_void_p()
+        check(lib.f3r_solver_create(ctx.handle, A.handle(ctx), M.handle, ctypes.byref(p), ctypes.byref(h)),
+              "solver_create")
+        self.handle = h
+
+    def solve(self, b, x):
+        r = _capi.Result()
+        check(lib.f3r_solve(self.handle, b.handle, x.handle, ctypes.byref(r)), "solve")
+        return solve_res_t(bool(r.is_solved), r.res_iter, r.res_nrm2, r.nrm_b, r.solve_s, r.precond_calls,
+                           r.outer_cycles)
+
+    def history(self):
+        ln = ctypes.c_int32()
+        check(lib.f3r_solver_history(self.handle, None, 0, ctypes.byref(ln)), "history")
+        out = np.empty(ln.value, np.float64)
+        check(lib.f3r_solver_history(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     ln.value, ctypes.byref(ln)), "history")
+        return out
+
+    def level_apply(self, level, vin, vout):
+        check(lib.f3r_solver_level_apply(self.handle, level, vin.handle, vout.handle), "level_apply")
+
+    def time_kernel(self, which, reps=20):
+        """mean ms per launch and algorithmic bytes of kernel `which`
+        (0 fp16 SpMV, 1 fp16-value/fp32-vector SpMV, 2 fp32 SpMV, 3 fp64 SpMV)"""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        check(lib.f3r_solver_time_kernel(self.handle, which, reps, ctypes.byref(ms), ctypes.byref(by)),
+              "time_kernel")
+        return ms.value, by.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_solver_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -1,0 +1,150 @@
+// Internal declarations shared by the HIP kernels, the host set-up code and
+// the C ABI of libf3r_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "f3r_amd.h"
+
+namespace f3r {
+
+// ------------------------------------------------------------- errors -----
+void set_error(const std::string &msg);
+struct Error {
+  int status;
+  std::string msg;
+};
+
+#define F3R_HIP(call)                                                               \
+  do {                                                                              \
+    hipError_t _e = (call);                                                         \
+    if (_e != hipSuccess)                                                           \
+      throw ::f3r::Error{F3R_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)}; \
+  } while (0)
+#define F3R_CHECK(cond, status, msg)                                                \
+  do {                                                                              \
+    if (!(cond)) throw ::f3r::Error{status, msg};                                   \
+  } while (0)
+
+inline size_t dtype_size(int dt) {
+  switch (dt) {
+    case F3R_F64: return 8;
+    case F3R_F32: return 4;
+    case F3R_F16: return 2;
+    case F3R_I32: return 4;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------ device memory -----
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  void alloc(size_t b);
+  ~DevBuf();
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+using DevPtr = std::shared_ptr<DevBuf>;
+inline DevPtr dev_alloc(size_t bytes) { return std::make_shared<DevBuf>(bytes); }
+
+// Scratch of the single-pass grid reductions: per-block partial sums and
+// arrival counters.  Counters are reset by the last block of each launch, so
+// no memset is ever recorded (also under graph capture).
+struct Reducer {
+  static constexpr int kMaxBlocks = 1024;
+  static constexpr int kMaxCols = 128;
+  DevPtr partial;   // kMaxBlocks * kMaxCols doubles
+  DevPtr counters;  // 64 unsigned ints
+  DevPtr scalars;   // 64 doubles of results
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+  Reducer red;
+  double *pinned = nullptr;  // host-pinned landing buffer for small reads
+};
+
+struct Vec {
+  Context *ctx = nullptr;
+  int dtype = F3R_F64;
+  int64_t n = 0;
+  DevPtr buf;         // owner (shared with views)
+  int64_t offset = 0; // element offset into buf
+  void *ptr() const { return static_cast<char *>(buf->p) + offset * dtype_size(dtype); }
+};
+
+// Host CSR (fp64 values), int32 indices.
+struct HostCSR {
+  int32_t n = 0, ncols = 0;
+  std::vector<int32_t> rptr, idx;
+  std::vector<double> val;
+  int64_t nnz() const { return (int64_t)idx.size(); }
+};
+
+// SELL-C-sigma with K-slot interleave on the device.  Chunk s holds rows
+// [s*C, s*C + tc) of the sorted order (tc = C except for the last chunk);
+// slot group g of row j of the chunk occupies elements
+//   cptr[s] + (g * tc + j) * K + k,  k < K
+// (K = 1 gives the reference's column-major layout, sell.hpp:272-287).
+// Row r of the sorted order is row perm[r] of the original matrix; the
+// column indices stay in the original numbering unless `internal` is set, in
+// which case they are renumbered into the sorted order.
+struct SellShared {  // index data shared by all precisions of one matrix
+  int32_t n = 0, C = 64, sigma = 1, K = 1, nchunks = 0;
+  int64_t nslots = 0, nnz = 0;
+  bool internal = false;
+  DevPtr cptr, cwidth, idx, perm;  // perm null when identity
+  std::vector<int32_t> h_perm;     // host copy of the permutation
+};
+
+struct Mat {
+  Context *ctx = nullptr;
+  int format = 0;  // 0 = CSR (fp64), 1 = SELL
+  int dtype = F3R_F64;
+  // CSR
+  std::shared_ptr<HostCSR> host;
+  DevPtr d_rptr, d_idx, d_val;
+  // SELL
+  std::shared_ptr<SellShared> sh;
+  DevPtr s_val;
+  std::shared_ptr<std::vector<double>> sell_host_val;  // fp64 slot values, for casts
+};
+
+// ----------------------------------------------------- host set-up API ----
+void csr_scaling(int32_t n, const int32_t *rptr, const int32_t *idx, double *val);
+std::vector<int32_t> sigma_perm(int32_t n, const int32_t *rptr, int32_t sigma);
+int64_t sell_nslots(int32_t n, const int32_t *rptr, int32_t C, int32_t sigma, int32_t K);
+struct HostSell {
+  int32_t n, C, K, nchunks;
+  std::vector<int32_t> cptr, cwidth, idx, perm;
+  std::vector<double> val;
+};
+// Builds the layout; when `renumber` the columns are mapped through the
+// inverse permutation (the matrix P A P^T in sorted order).
+HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
+                    const std::vector<int32_t> *perm_override, bool renumber);
+struct HostFactors {
+  HostCSR f[2];
+};
+HostFactors sdainv_host(const HostCSR &A, double tol, double acc);
+HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc);
+// fp64 -> fp16 bits, correctly rounded (round to nearest even)
+uint16_t d2h_bits(double d);
+
+// ------------------------------------------------------ device helpers ----
+std::shared_ptr<SellShared> upload_sell_shared(Context *ctx, const HostSell &hs, bool internal);
+DevPtr upload_sell_values(Context *ctx, const HostSell &hs, int dtype);
+
+}  // namespace f3r

@@ -1,0 +1,918 @@
+// Runtime objects of libf3r_amd: contexts (one HIP stream each), device
+// vectors, CSR / SELL-C-sigma matrices, the BLAS-1 kernels with single-pass
+// grid reductions, and the SDAINV / ILU(0) preconditioner objects.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+#include "internal.hpp"
+#include "sell_kernel.hpp"
+
+namespace f3r {
+
+// ------------------------------------------------------------- errors ----
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+void DevBuf::alloc(size_t b) {
+  bytes = b;
+  if (b == 0) {
+    p = nullptr;
+    return;
+  }
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    throw Error{F3R_ERR_ALLOC, "hipMalloc(" + std::to_string(b) + "): " + hipGetErrorString(e)};
+  }
+}
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+// ------------------------------------------------------------ kernels ----
+template <class T> __global__ void k_fill(T *__restrict__ p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = v;
+}
+
+template <class S, class D> __global__ void k_cast(const S *__restrict__ s, D *__restrict__ d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    d[i] = cvt<D>(to_acc(s[i]));
+}
+
+template <class T> __global__ void k_axpy(int64_t n, double alpha, const T *__restrict__ x, T *__restrict__ y) {
+  using A = typename Acc<T>::type;
+  const A a = (A)alpha;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = cvt<T>(fma_<A>(a, to_acc(x[i]), to_acc(y[i])));
+}
+
+template <class T> __global__ void k_scal(int64_t n, double alpha, T *__restrict__ x) {
+  using A = typename Acc<T>::type;
+  const A a = (A)alpha;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    x[i] = cvt<T>(to_acc(x[i]) * a);
+}
+
+template <class A, class T1, class T2>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T1 *__restrict__ a,
+                                                const T2 *__restrict__ b, A *partial,
+                                                unsigned *counter, double *out) {
+  __shared__ A lds[kWavesPerBlock];
+  A v[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    v[0] = fma_<A>((A)to_acc(a[i]), (A)to_acc(b[i]), v[0]);
+  if (grid_sum<A, 1>(v, partial, counter, lds) && threadIdx.x == 0) *out = (double)v[0];
+}
+
+// CSR SpMV, one row per thread, fp64 values (CSR<double>::operate,
+// csr.hpp:32-50).  mode 0: y = A x, 1: y = b - A x.
+template <class X, class Y>
+__global__ void k_csr(int32_t n, const int32_t *__restrict__ rptr, const int32_t *__restrict__ idx,
+                      const double *__restrict__ val, const X *__restrict__ x, Y *__restrict__ y,
+                      const Y *__restrict__ b) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double t = 0;
+  for (int k = rptr[i]; k < rptr[i + 1]; ++k) t = __builtin_fma(val[k], (double)to_acc(x[idx[k]]), t);
+  y[i] = b ? cvt<Y>((double)to_acc(b[i]) - t) : cvt<Y>(t);
+}
+
+// ILU(0) triangular solves by levels: each launch solves one level of rows
+// in parallel, every row in the order of the reference's block-Jacobi solve
+// (trsv/bj.hpp:58-98): L forward with the unit diagonal stored last, U
+// backward from the largest column, times the inverted diagonal.
+template <class S>
+__global__ void k_trsv_l(int32_t cnt, const int32_t *__restrict__ rows, const int32_t *__restrict__ rptr,
+                         const int32_t *__restrict__ idx, const S *__restrict__ val,
+                         const S *__restrict__ in, S *__restrict__ out) {
+  using A = typename Acc<S>::type;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  A s = to_acc(in[i]);
+  for (int k = rptr[i]; k < rptr[i + 1] - 1; ++k) s = fma_<A>(-to_acc(val[k]), to_acc(out[idx[k]]), s);
+  out[i] = cvt<S>(s);
+}
+template <class S>
+__global__ void k_trsv_u(int32_t cnt, const int32_t *__restrict__ rows, const int32_t *__restrict__ rptr,
+                         const int32_t *__restrict__ idx, const S *__restrict__ val,
+                         const S *__restrict__ dinv, const S *__restrict__ in, S *__restrict__ out) {
+  using A = typename Acc<S>::type;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int r = rows[t];
+  A s = to_acc(in[r]);
+  for (int k = rptr[r + 1] - 1; k > rptr[r]; --k) s = fma_<A>(-to_acc(val[k]), to_acc(out[idx[k]]), s);
+  out[r] = cvt<S>(s * to_acc(dinv[r]));
+}
+
+static int grid_for(int64_t n, int cu) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  int64_t cap = (int64_t)cu * 16;
+  if (g > cap) g = cap;
+  return g < 1 ? 1 : (int)g;
+}
+
+// ------------------------------------------------------------- context ---
+Context *ctx_create(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    throw Error{F3R_ERR_NODEVICE, "no HIP device"};
+  F3R_CHECK(device >= 0 && device < count, F3R_ERR_ARG, "device index out of range");
+  F3R_HIP(hipSetDevice(device));
+  auto *c = new Context();
+  c->device = device;
+  hipDeviceProp_t prop;
+  F3R_HIP(hipGetDeviceProperties(&prop, device));
+  c->num_cu = prop.multiProcessorCount;
+  F3R_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->red.partial = dev_alloc(sizeof(double) * Reducer::kMaxBlocks * Reducer::kMaxCols);
+  c->red.counters = dev_alloc(sizeof(unsigned) * 64);
+  c->red.scalars = dev_alloc(sizeof(double) * 64);
+  F3R_HIP(hipMemsetAsync(c->red.counters->p, 0, sizeof(unsigned) * 64, c->stream));
+  F3R_HIP(hipHostMalloc((void **)&c->pinned, 4096, hipHostMallocDefault));
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  return c;
+}
+
+void ctx_destroy(Context *c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  c->red = Reducer();
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+double read_scalar(Context *c, const double *dev) {
+  F3R_HIP(hipMemcpyAsync(c->pinned, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  return c->pinned[0];
+}
+
+// ------------------------------------------------------------- vectors ---
+void vec_fill(Vec *v, double val) {
+  int g = grid_for(v->n, v->ctx->num_cu);
+  hipStream_t st = v->ctx->stream;
+  switch (v->dtype) {
+    case F3R_F64: hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kBlock), 0, st, (double *)v->ptr(), v->n, val); break;
+    case F3R_F32: hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kBlock), 0, st, (float *)v->ptr(), v->n, (float)val); break;
+    case F3R_F16: hipLaunchKernelGGL(k_fill<half_t>, dim3(g), dim3(kBlock), 0, st, (half_t *)v->ptr(), v->n, (half_t)(float)val); break;
+    case F3R_I32: hipLaunchKernelGGL(k_fill<int32_t>, dim3(g), dim3(kBlock), 0, st, (int32_t *)v->ptr(), v->n, (int32_t)val); break;
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+template <class S> static void cast_from(const S *s, Vec *d, hipStream_t st, int g) {
+  switch (d->dtype) {
+    case F3R_F64: hipLaunchKernelGGL((k_cast<S, double>), dim3(g), dim3(kBlock), 0, st, s, (double *)d->ptr(), d->n); break;
+    case F3R_F32: hipLaunchKernelGGL((k_cast<S, float>), dim3(g), dim3(kBlock), 0, st, s, (float *)d->ptr(), d->n); break;
+    case F3R_F16: hipLaunchKernelGGL((k_cast<S, half_t>), dim3(g), dim3(kBlock), 0, st, s, (half_t *)d->ptr(), d->n); break;
+  }
+}
+
+void vec_copy(Vec *s, Vec *d) {
+  F3R_CHECK(s->n == d->n, F3R_ERR_ARG, "copy: length mismatch");
+  F3R_CHECK(s->dtype != F3R_I32 && d->dtype != F3R_I32, F3R_ERR_UNSUPPORTED, "copy: int32 vectors");
+  hipStream_t st = d->ctx->stream;
+  if (s->dtype == d->dtype) {
+    F3R_HIP(hipMemcpyAsync(d->ptr(), s->ptr(), s->n * dtype_size(s->dtype), hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  int g = grid_for(s->n, d->ctx->num_cu);
+  switch (s->dtype) {
+    case F3R_F64: cast_from((const double *)s->ptr(), d, st, g); break;
+    case F3R_F32: cast_from((const float *)s->ptr(), d, st, g); break;
+    case F3R_F16: cast_from((const half_t *)s->ptr(), d, st, g); break;
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+void vec_axpy(double a, Vec *x, Vec *y) {
+  F3R_CHECK(x->n == y->n && x->dtype == y->dtype, F3R_ERR_ARG, "axpy: x and y differ in length or type");
+  int g = grid_for(x->n, x->ctx->num_cu);
+  hipStream_t st = x->ctx->stream;
+  switch (x->dtype) {
+    case F3R_F64: hipLaunchKernelGGL(k_axpy<double>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const double *)x->ptr(), (double *)y->ptr()); break;
+    case F3R_F32: hipLaunchKernelGGL(k_axpy<float>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const float *)x->ptr(), (float *)y->ptr()); break;
+    case F3R_F16: hipLaunchKernelGGL(k_axpy<half_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const half_t *)x->ptr(), (half_t *)y->ptr()); break;
+    default: throw Error{F3R_ERR_UNSUPPORTED, "axpy: int32"};
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+void vec_scale(Vec *x, double a) {
+  int g = grid_for(x->n, x->ctx->num_cu);
+  hipStream_t st = x->ctx->stream;
+  switch (x->dtype) {
+    case F3R_F64: hipLaunchKernelGGL(k_scal<double>, dim3(g), dim3(kBlock), 0, st, x->n, a, (double *)x->ptr()); break;
+    case F3R_F32: hipLaunchKernelGGL(k_scal<float>, dim3(g), dim3(kBlock), 0, st, x->n, a, (float *)x->ptr()); break;
+    case F3R_F16: hipLaunchKernelGGL(k_scal<half_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (half_t *)x->ptr()); break;
+    default: throw Error{F3R_ERR_UNSUPPORTED, "scale: int32"};
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+template <class A, class T1, class T2>
+static void dot_launch(Context *c, int64_t n, const T1 *a, const T2 *b, double *out) {
+  int g = reduce_grid(n, c->num_cu);
+  hipLaunchKernelGGL((k_dot<A, T1, T2>), dim3(g), dim3(kBlock), 0, c->stream, n, a, b,
+                     c->red.partial->as<A>(), c->red.counters->as<unsigned>(), out);
+}
+
+double dot_raw(Context *c, int da, const void *pa, int db, const void *pb, int64_t n) {
+  double *slot = c->red.scalars->as<double>();
+  F3R_CHECK(da != F3R_I32 && db != F3R_I32, F3R_ERR_UNSUPPORTED, "dot: int32");
+  // accumulate in the wider type; fp16 data in fp32
+#define F3R_DOT(TA, TB)                                                         \
+  if (std::is_same<TA, double>::value || std::is_same<TB, double>::value)       \
+    dot_launch<double>(c, n, (const TA *)pa, (const TB *)pb, slot);             \
+  else                                                                          \
+    dot_launch<float>(c, n, (const TA *)pa, (const TB *)pb, slot);
+  auto pick_b = [&](auto tagA) {
+    using TA = decltype(tagA);
+    switch (db) {
+      case F3R_F64: { F3R_DOT(TA, double) break; }
+      case F3R_F32: { F3R_DOT(TA, float) break; }
+      case F3R_F16: { F3R_DOT(TA, half_t) break; }
+    }
+  };
+  switch (da) {
+    case F3R_F64: pick_b(double()); break;
+    case F3R_F32: pick_b(float()); break;
+    case F3R_F16: pick_b(half_t()); break;
+  }
+#undef F3R_DOT
+  F3R_HIP(hipGetLastError());
+  return read_scalar(c, slot);
+}
+
+double vec_dot(Vec *a, Vec *b) {
+  F3R_CHECK(a->n == b->n, F3R_ERR_ARG, "dot: length mismatch");
+  return dot_raw(a->ctx, a->dtype, a->ptr(), b->dtype, b->ptr(), a->n);
+}
+
+// ------------------------------------------------------------ matrices ---
+SellArgs sell_args(const SellShared &sh, bool with_perm) {
+  SellArgs a;
+  a.n = sh.n;
+  int l = 0;
+  while ((1 << l) < sh.C) ++l;
+  a.log2C = l;
+  a.cptr = sh.cptr->as<int32_t>();
+  a.cwidth = sh.cwidth->as<int32_t>();
+  a.idx = sh.idx->as<int32_t>();
+  a.perm = (with_perm && sh.perm) ? sh.perm->as<int32_t>() : nullptr;
+  return a;
+}
+
+std::shared_ptr<SellShared> upload_sell_shared(Context *ctx, const HostSell &hs, bool internal) {
+  auto sh = std::make_shared<SellShared>();
+  sh->n = hs.n;
+  sh->C = hs.C;
+  sh->K = hs.K;
+  sh->nchunks = hs.nchunks;
+  sh->nslots = (int64_t)hs.idx.size();
+  sh->internal = internal;
+  hipStream_t st = ctx->stream;
+  sh->cptr = dev_alloc(sizeof(int32_t) * hs.cptr.size());
+  sh->cwidth = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, hs.cwidth.size()));
+  sh->idx = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, hs.idx.size()));
+  F3R_HIP(hipMemcpyAsync(sh->cptr->p, hs.cptr.data(), sizeof(int32_t) * hs.cptr.size(), hipMemcpyHostToDevice, st));
+  if (!hs.cwidth.empty())
+    F3R_HIP(hipMemcpyAsync(sh->cwidth->p, hs.cwidth.data(), sizeof(int32_t) * hs.cwidth.size(), hipMemcpyHostToDevice, st));
+  if (!hs.idx.empty())
+    F3R_HIP(hipMemcpyAsync(sh->idx->p, hs.idx.data(), sizeof(int32_t) * hs.idx.size(), hipMemcpyHostToDevice, st));
+  bool ident = true;
+  for (int32_t r = 0; r < hs.n && ident; ++r) ident = hs.perm[r] == r;
+  if (!ident) {
+    sh->perm = dev_alloc(sizeof(int32_t) * hs.n);
+    F3R_HIP(hipMemcpyAsync(sh->perm->p, hs.perm.data(), sizeof(int32_t) * hs.n, hipMemcpyHostToDevice, st));
+  }
+  sh->h_perm = hs.perm;
+  F3R_HIP(hipStreamSynchronize(st));
+  return sh;
+}
+
+DevPtr upload_values(Context *ctx, const std::vector<double> &v, int dtype) {
+  DevPtr d = dev_alloc(dtype_size(dtype) * std::max<size_t>(1, v.size()));
+  if (v.empty()) return d;
+  hipStream_t st = ctx->stream;
+  if (dtype == F3R_F64) {
+    F3R_HIP(hipMemcpyAsync(d->p, v.data(), 8 * v.size(), hipMemcpyHostToDevice, st));
+    F3R_HIP(hipStreamSynchronize(st));
+  } else if (dtype == F3R_F32) {
+    std::vector<float> f(v.size());
+    for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+    F3R_HIP(hipMemcpyAsync(d->p, f.data(), 4 * f.size(), hipMemcpyHostToDevice, st));
+    F3R_HIP(hipStreamSynchronize(st));
+  } else if (dtype == F3R_F16) {
+    std::vector<uint16_t> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = d2h_bits(v[i]);
+    F3R_HIP(hipMemcpyAsync(d->p, h.data(), 2 * h.size(), hipMemcpyHostToDevice, st));
+    F3R_HIP(hipStreamSynchronize(st));
+  } else {
+    throw Error{F3R_ERR_UNSUPPORTED, "matrix values must be fp64, fp32 or fp16"};
+  }
+  return d;
+}
+
+DevPtr upload_sell_values(Context *ctx, const HostSell &hs, int dtype) {
+  return upload_values(ctx, hs.val, dtype);
+}
+
+template <class V, class X, class Y>
+static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hipStream_t st) {
+  using ACC = typename std::conditional<std::is_same<V, double>::value, double, float>::type;
+  SellArgs a = sell_args(*A->sh, true);
+  if (b)
+    launch_sell<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
+                           EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
+  else
+    launch_sell<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
+                           EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
+}
+
+void mat_apply(const Mat *A, Vec *x, Vec *y, Vec *b) {
+  F3R_CHECK(x->n == (A->format == 0 ? A->host->ncols : A->sh->n) && y->n == x->n, F3R_ERR_ARG,
+            "spmv: vector length does not match the matrix");
+  F3R_CHECK(!b || (b->n == y->n && b->dtype == y->dtype), F3R_ERR_ARG, "residual: b must match r");
+  hipStream_t st = A->ctx->stream;
+  const void *bp = b ? b->ptr() : nullptr;
+  if (A->format == 0) {
+    int g = (A->host->n + kBlock - 1) / kBlock;
+    auto go = [&](auto xt, auto yt) {
+      using X = decltype(xt);
+      using Y = decltype(yt);
+      hipLaunchKernelGGL((k_csr<X, Y>), dim3(g), dim3(kBlock), 0, st, A->host->n, A->d_rptr->as<int32_t>(),
+                         A->d_idx->as<int32_t>(), A->d_val->as<double>(), (const X *)x->ptr(), (Y *)y->ptr(),
+                         (const Y *)bp);
+    };
+    if (x->dtype == F3R_F64 && y->dtype == F3R_F64) go(double(), double());
+    else if (x->dtype == F3R_F32 && y->dtype == F3R_F32) go(float(), float());
+    else if (x->dtype == F3R_F32 && y->dtype == F3R_F64) go(float(), double());
+    else if (x->dtype == F3R_F64 && y->dtype == F3R_F32) go(double(), float());
+    else throw Error{F3R_ERR_UNSUPPORTED, "CSR spmv: unsupported vector types"};
+  } else {
+    F3R_CHECK(!A->sh->internal, F3R_ERR_STATE, "spmv: internal-order matrix");
+    const int v = A->dtype, xd = x->dtype, yd = y->dtype;
+    if (v == F3R_F64 && xd == F3R_F64 && yd == F3R_F64) sell_apply_t<double, double, double>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F64 && xd == F3R_F32 && yd == F3R_F32) sell_apply_t<double, float, float>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F32 && xd == F3R_F32 && yd == F3R_F32) sell_apply_t<float, float, float>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F32 && xd == F3R_F32 && yd == F3R_F64) sell_apply_t<float, float, double>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F16 && xd == F3R_F32 && yd == F3R_F32) sell_apply_t<half_t, float, float>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F16) sell_apply_t<half_t, half_t, half_t>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F32) sell_apply_t<half_t, half_t, float>(A, x->ptr(), y->ptr(), bp, st);
+    else throw Error{F3R_ERR_UNSUPPORTED, "SELL spmv: unsupported (value, x, y) type combination"};
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------ preconditioners ---
+static std::vector<int32_t> levels_lower(const HostCSR &L, std::vector<int32_t> &lptr) {
+  const int32_t n = L.n;
+  std::vector<int32_t> lev(n, 0);
+  int32_t nl = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t m = -1;
+    for (int32_t k = L.rptr[i]; k < L.rptr[i + 1] - 1; ++k) m = std::max(m, lev[L.idx[k]]);
+    lev[i] = m + 1;
+    nl = std::max(nl, lev[i] + 1);
+  }
+  lptr.assign(nl + 1, 0);
+  for (int32_t i = 0; i < n; ++i) lptr[lev[i] + 1]++;
+  for (int32_t l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
+  std::vector<int32_t> rows(n), fill(lptr.begin(), lptr.end() - 1);
+  for (int32_t i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+  return rows;
+}
+static std::vector<int32_t> levels_upper(const HostCSR &U, std::vector<int32_t> &lptr) {
+  const int32_t n = U.n;
+  std::vector<int32_t> lev(n, 0);
+  int32_t nl = 0;
+  for (int32_t r = n - 1; r >= 0; --r) {
+    int32_t m = -1;
+    for (int32_t k = U.rptr[r] + 1; k < U.rptr[r + 1]; ++k) m = std::max(m, lev[U.idx[k]]);
+    lev[r] = m + 1;
+    nl = std::max(nl, lev[r] + 1);
+  }
+  lptr.assign(nl + 1, 0);
+  for (int32_t i = 0; i < n; ++i) lptr[lev[i] + 1]++;
+  for (int32_t l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
+  std::vector<int32_t> rows(n), fill(lptr.begin(), lptr.end() - 1);
+  for (int32_t i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+  return rows;
+}
+
+static DevPtr upload_i32(Context *c, const std::vector<int32_t> &v) {
+  DevPtr d = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, v.size()));
+  if (!v.empty()) {
+    F3R_HIP(hipMemcpyAsync(d->p, v.data(), 4 * v.size(), hipMemcpyHostToDevice, c->stream));
+    F3R_HIP(hipStreamSynchronize(c->stream));
+  }
+  return d;
+}
+
+static bool is_diag(const HostCSR &M) {
+  for (int32_t i = 0; i < M.n; ++i)
+    if (M.rptr[i + 1] - M.rptr[i] != 1 || M.idx[M.rptr[i]] != i) return false;
+  return true;
+}
+
+// Permuted copy P M P^T of a host CSR (row r of the result = row perm[r]),
+// columns renumbered and sorted.
+HostCSR permute_csr(const HostCSR &M, const std::vector<int32_t> &perm) {
+  const int32_t n = M.n;
+  std::vector<int32_t> inv(n);
+  for (int32_t r = 0; r < n; ++r) inv[perm[r]] = r;
+  HostCSR R;
+  R.n = R.ncols = n;
+  R.rptr.assign(n + 1, 0);
+  R.idx.resize(M.nnz());
+  R.val.resize(M.nnz());
+  std::vector<std::pair<int32_t, double>> row;
+  for (int32_t r = 0; r < n; ++r) {
+    int32_t o = perm[r];
+    row.clear();
+    for (int32_t k = M.rptr[o]; k < M.rptr[o + 1]; ++k) row.push_back({inv[M.idx[k]], M.val[k]});
+    std::sort(row.begin(), row.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    R.rptr[r + 1] = R.rptr[r] + (int32_t)row.size();
+    for (size_t q = 0; q < row.size(); ++q) {
+      R.idx[R.rptr[r] + q] = row[q].first;
+      R.val[R.rptr[r] + q] = row[q].second;
+    }
+  }
+  return R;
+}
+
+void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K) {
+  Context *c = P->ctx;
+  const int32_t n = P->host.f[0].n;
+  P->n = n;
+  P->tmp = dev_alloc(dtype_size(P->dtype) * std::max(1, n));
+  if (P->kind == 0) {
+    const HostCSR &Z = P->host.f[0], &W = P->host.f[1];
+    P->diag = is_diag(Z) && is_diag(W);
+    if (P->diag) {
+      std::vector<double> wd(n), zd(n);
+      // under a symmetric permutation a diagonal stays diagonal
+      for (int32_t r = 0; r < n; ++r) {
+        int32_t o = perm ? (*perm)[r] : r;
+        wd[r] = W.val[W.rptr[o]];
+        zd[r] = Z.val[Z.rptr[o]];
+      }
+      P->dw = upload_values(c, wd, P->dtype);
+      P->dz = upload_values(c, zd, P->dtype);
+    }
+    for (int w = 0; w < 2; ++w) {
+      const HostCSR &M = w == 0 ? W : Z;
+      HostSell hs = build_sell(M, C, 1, K, perm, perm != nullptr);
+      auto m = std::make_shared<Mat>();
+      m->ctx = c;
+      m->format = 1;
+      m->dtype = P->dtype;
+      m->sh = upload_sell_shared(c, hs, perm != nullptr);
+      m->s_val = upload_sell_values(c, hs, P->dtype);
+      (w == 0 ? P->W : P->Z) = m;
+    }
+  } else {
+    HostCSR L = perm ? permute_csr(P->host.f[0], *perm) : P->host.f[0];
+    HostCSR U = perm ? permute_csr(P->host.f[1], *perm) : P->host.f[1];
+    // a permutation would break the triangular structure: only the identity
+    F3R_CHECK(!perm, F3R_ERR_UNSUPPORTED, "ILU(0) with a sorted (sigma > 1) layout");
+    P->l_rows = upload_i32(c, levels_lower(L, P->l_lptr));
+    P->u_rows = upload_i32(c, levels_upper(U, P->u_lptr));
+    P->l_rptr = upload_i32(c, L.rptr);
+    P->l_idx = upload_i32(c, L.idx);
+    P->l_val = upload_values(c, L.val, P->dtype);
+    P->u_rptr = upload_i32(c, U.rptr);
+    P->u_idx = upload_i32(c, U.idx);
+    P->u_val = upload_values(c, U.val, P->dtype);
+    std::vector<double> inv(n);
+    for (int32_t r = 0; r < n; ++r) inv[r] = 1. / U.val[U.rptr[r]];
+    P->u_dinv = upload_values(c, inv, P->dtype);
+  }
+}
+
+template <class S> static void ilu_apply_t(Prec *P, const S *in, S *out, hipStream_t st) {
+  S *tmp = P->tmp->as<S>();
+  for (size_t l = 0; l + 1 < P->l_lptr.size(); ++l) {
+    int32_t cnt = P->l_lptr[l + 1] - P->l_lptr[l];
+    hipLaunchKernelGGL(k_trsv_l<S>, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
+                       P->l_rows->as<int32_t>() + P->l_lptr[l], P->l_rptr->as<int32_t>(),
+                       P->l_idx->as<int32_t>(), P->l_val->as<S>(), in, tmp);
+  }
+  for (size_t l = 0; l + 1 < P->u_lptr.size(); ++l) {
+    int32_t cnt = P->u_lptr[l + 1] - P->u_lptr[l];
+    hipLaunchKernelGGL(k_trsv_u<S>, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
+                       P->u_rows->as<int32_t>() + P->u_lptr[l], P->u_rptr->as<int32_t>(),
+                       P->u_idx->as<int32_t>(), P->u_val->as<S>(), P->u_dinv->as<S>(), tmp, out);
+  }
+}
+
+template <class S> static void sdainv_apply_t(Prec *P, const S *in, S *out, hipStream_t st) {
+  using ACC = typename Acc<S>::type;
+  S *tmp = P->tmp->as<S>();
+  const bool wp = !P->W->sh->internal;
+  SellArgs aw = sell_args(*P->W->sh, wp), az = sell_args(*P->Z->sh, wp);
+  launch_sell<S, S, ACC>(P->W->sh->K, aw, P->W->s_val->as<S>(), in, EpiStore<S, ACC>{tmp, aw.perm}, st);
+  launch_sell<S, S, ACC>(P->Z->sh->K, az, P->Z->s_val->as<S>(), tmp, EpiStore<S, ACC>{out, az.perm}, st);
+}
+
+void prec_apply_raw(Prec *P, const void *in, void *out, hipStream_t st) {
+  switch (P->dtype) {
+    case F3R_F64:
+      P->kind == 0 ? sdainv_apply_t<double>(P, (const double *)in, (double *)out, st)
+                   : ilu_apply_t<double>(P, (const double *)in, (double *)out, st);
+      break;
+    case F3R_F32:
+      P->kind == 0 ? sdainv_apply_t<float>(P, (const float *)in, (float *)out, st)
+                   : ilu_apply_t<float>(P, (const float *)in, (float *)out, st);
+      break;
+    case F3R_F16:
+      P->kind == 0 ? sdainv_apply_t<half_t>(P, (const half_t *)in, (half_t *)out, st)
+                   : ilu_apply_t<half_t>(P, (const half_t *)in, (half_t *)out, st);
+      break;
+  }
+  F3R_HIP(hipGetLastError());
+}
+
+HostCSR host_csr_from(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rptr,
+                      const int32_t *col, const double *val) {
+  F3R_CHECK(nrows >= 0 && ncols >= 0 && nnz >= 0 && rptr && (nnz == 0 || (col && val)),
+            F3R_ERR_ARG, "CSR: null array or negative size");
+  F3R_CHECK(nnz < INT32_MAX, F3R_ERR_RANGE, "CSR: nnz does not fit int32 indices");
+  F3R_CHECK(rptr[0] == 0 && rptr[nrows] == nnz, F3R_ERR_ARG, "CSR: rptr[0] != 0 or rptr[n] != nnz");
+  HostCSR h;
+  h.n = nrows;
+  h.ncols = ncols;
+  h.rptr.assign(rptr, rptr + nrows + 1);
+  h.idx.assign(col, col + nnz);
+  h.val.assign(val, val + nnz);
+  for (int32_t i = 0; i < nrows; ++i) {
+    F3R_CHECK(h.rptr[i] <= h.rptr[i + 1], F3R_ERR_ARG, "CSR: rptr is not monotone");
+    for (int32_t k = h.rptr[i]; k < h.rptr[i + 1]; ++k) {
+      F3R_CHECK(h.idx[k] >= 0 && h.idx[k] < ncols, F3R_ERR_ARG, "CSR: column index out of range");
+      F3R_CHECK(k == h.rptr[i] || h.idx[k - 1] < h.idx[k], F3R_ERR_ARG,
+                "CSR: columns of a row must be sorted and unique");
+    }
+  }
+  return h;
+}
+
+}  // namespace f3r
+
+// ================================================================ C ABI ===
+using namespace f3r;
+
+
+struct f3r_ctx_s { Context *c; };
+struct f3r_vec_s { Vec v; };
+struct f3r_mat_s { std::shared_ptr<Mat> m; };
+struct f3r_prec_s { std::shared_ptr<Prec> p; };
+
+namespace f3r {
+Context *ctx_of(f3r_ctx c) { return c ? c->c : nullptr; }
+Vec *vec_of(f3r_vec v) { return v ? &v->v : nullptr; }
+std::shared_ptr<Mat> mat_of(f3r_mat m) { return m ? m->m : nullptr; }
+std::shared_ptr<Prec> prec_of(f3r_prec p) { return p ? p->p : nullptr; }
+}  // namespace f3r
+
+extern "C" {
+
+const char *f3r_strerror(int s) {
+  switch (s) {
+    case F3R_OK: return "ok";
+    case F3R_ERR_ARG: return "invalid argument";
+    case F3R_ERR_HIP: return "HIP runtime error";
+    case F3R_ERR_ALLOC: return "allocation failed";
+    case F3R_ERR_UNSUPPORTED: return "unsupported type or format combination";
+    case F3R_ERR_NODEVICE: return "no usable GPU";
+    case F3R_ERR_STATE: return "invalid object state";
+    case F3R_ERR_RANGE: return "size beyond the format's index range";
+  }
+  return "unknown status";
+}
+const char *f3r_last_error(void) { return g_last_error.c_str(); }
+
+int f3r_device_count(int *count) {
+  F3R_API_BEGIN
+  F3R_CHECK(count, F3R_ERR_ARG, "null count");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  F3R_API_END
+}
+
+int f3r_ctx_create(int device, f3r_ctx *ctx) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx, F3R_ERR_ARG, "null ctx pointer");
+  *ctx = nullptr;
+  Context *c = ctx_create(device);
+  *ctx = new f3r_ctx_s{c};
+  F3R_API_END
+}
+int f3r_ctx_destroy(f3r_ctx ctx) {
+  F3R_API_BEGIN
+  if (!ctx) return F3R_OK;
+  ctx_destroy(ctx->c);
+  delete ctx;
+  F3R_API_END
+}
+int f3r_ctx_sync(f3r_ctx ctx) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx, F3R_ERR_ARG, "null ctx");
+  F3R_HIP(hipStreamSynchronize(ctx->c->stream));
+  F3R_API_END
+}
+
+int f3r_vec_create(f3r_ctx ctx, int dtype, int64_t n, f3r_vec *v) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && v && n >= 0 && dtype_size(dtype), F3R_ERR_ARG, "vec_create: bad arguments");
+  *v = nullptr;
+  auto *h = new f3r_vec_s();
+  h->v.ctx = ctx->c;
+  h->v.dtype = dtype;
+  h->v.n = n;
+  try {
+    h->v.buf = dev_alloc(dtype_size(dtype) * std::max<int64_t>(n, 1));
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *v = h;
+  F3R_API_END
+}
+int f3r_vec_view(f3r_vec base, int64_t offset, int64_t n, f3r_vec *v) {
+  F3R_API_BEGIN
+  F3R_CHECK(base && v && offset >= 0 && n >= 0 && offset + n <= base->v.n, F3R_ERR_ARG,
+            "vec_view: range outside the base vector");
+  auto *h = new f3r_vec_s();
+  h->v = base->v;
+  h->v.offset = base->v.offset + offset;
+  h->v.n = n;
+  *v = h;
+  F3R_API_END
+}
+int f3r_vec_destroy(f3r_vec v) {
+  F3R_API_BEGIN
+  if (v) {
+    (void)hipStreamSynchronize(v->v.ctx->stream);
+    delete v;
+  }
+  F3R_API_END
+}
+int f3r_vec_upload(f3r_vec v, const void *host) {
+  F3R_API_BEGIN
+  F3R_CHECK(v && (host || v->v.n == 0), F3R_ERR_ARG, "vec_upload: null");
+  F3R_HIP(hipMemcpyAsync(v->v.ptr(), host, v->v.n * dtype_size(v->v.dtype), hipMemcpyHostToDevice, v->v.ctx->stream));
+  F3R_HIP(hipStreamSynchronize(v->v.ctx->stream));
+  F3R_API_END
+}
+int f3r_vec_download(f3r_vec v, void *host) {
+  F3R_API_BEGIN
+  F3R_CHECK(v && (host || v->v.n == 0), F3R_ERR_ARG, "vec_download: null");
+  F3R_HIP(hipMemcpyAsync(host, v->v.ptr(), v->v.n * dtype_size(v->v.dtype), hipMemcpyDeviceToHost, v->v.ctx->stream));
+  F3R_HIP(hipStreamSynchronize(v->v.ctx->stream));
+  F3R_API_END
+}
+int f3r_vec_info(f3r_vec v, int *dtype, int64_t *n, void **ptr) {
+  F3R_API_BEGIN
+  F3R_CHECK(v, F3R_ERR_ARG, "null vec");
+  if (dtype) *dtype = v->v.dtype;
+  if (n) *n = v->v.n;
+  if (ptr) *ptr = v->v.ptr();
+  F3R_API_END
+}
+int f3r_vec_fill(f3r_vec v, double value) {
+  F3R_API_BEGIN
+  F3R_CHECK(v, F3R_ERR_ARG, "null vec");
+  vec_fill(&v->v, value);
+  F3R_API_END
+}
+int f3r_vec_copy(f3r_vec s, f3r_vec d) {
+  F3R_API_BEGIN
+  F3R_CHECK(s && d, F3R_ERR_ARG, "null vec");
+  vec_copy(&s->v, &d->v);
+  F3R_API_END
+}
+int f3r_vec_axpy(double a, f3r_vec x, f3r_vec y) {
+  F3R_API_BEGIN
+  F3R_CHECK(x && y, F3R_ERR_ARG, "null vec");
+  vec_axpy(a, &x->v, &y->v);
+  F3R_API_END
+}
+int f3r_vec_scale(f3r_vec x, double a) {
+  F3R_API_BEGIN
+  F3R_CHECK(x, F3R_ERR_ARG, "null vec");
+  vec_scale(&x->v, a);
+  F3R_API_END
+}
+int f3r_vec_dot(f3r_vec a, f3r_vec b, double *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(a && b && out, F3R_ERR_ARG, "null argument");
+  *out = vec_dot(&a->v, &b->v);
+  F3R_API_END
+}
+int f3r_vec_nrm2(f3r_vec a, double *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(a && out, F3R_ERR_ARG, "null argument");
+  *out = std::sqrt(vec_dot(&a->v, &a->v));
+  F3R_API_END
+}
+
+int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rptr,
+                       const int32_t *col, const double *val, f3r_mat *A) {
+  F3R_API_BEGIN
+  F3R_CHECK(A, F3R_ERR_ARG, "null argument");
+  *A = nullptr;
+  auto m = std::make_shared<Mat>();
+  m->ctx = ctx ? ctx->c : nullptr;
+  m->format = 0;
+  m->dtype = F3R_F64;
+  m->host = std::make_shared<HostCSR>(host_csr_from(nrows, ncols, nnz, rptr, col, val));
+  if (ctx) {
+    m->d_rptr = upload_i32(ctx->c, m->host->rptr);
+    m->d_idx = upload_i32(ctx->c, m->host->idx);
+    m->d_val = upload_values(ctx->c, m->host->val, F3R_F64);
+  }
+  *A = new f3r_mat_s{m};
+  F3R_API_END
+}
+
+int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma, int32_t K, f3r_mat *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && csr && out && csr->m->format == 0, F3R_ERR_ARG, "to_sell: needs a context and a CSR matrix");
+  F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "to_sell: C must be a power of two >= 16");
+  F3R_CHECK(K == 1 || K == 2 || K == 4, F3R_ERR_ARG, "to_sell: K must be 1, 2 or 4");
+  F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
+  F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
+  *out = nullptr;
+  HostSell hs = build_sell(*csr->m->host, C, sigma, K, nullptr, false);
+  auto m = std::make_shared<Mat>();
+  m->ctx = ctx->c;
+  m->format = 1;
+  m->dtype = dtype;
+  m->host = csr->m->host;
+  m->sh = upload_sell_shared(ctx->c, hs, false);
+  m->sh->nnz = csr->m->host->nnz();
+  m->s_val = upload_sell_values(ctx->c, hs, dtype);
+  m->sell_host_val = std::make_shared<std::vector<double>>(std::move(hs.val));
+  *out = new f3r_mat_s{m};
+  F3R_API_END
+}
+
+int f3r_sell_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val, int32_t C,
+                    int32_t sigma, int32_t K, int32_t *cptr, int32_t *cwidth, int32_t *sidx, double *sval,
+                    int32_t *perm) {
+  F3R_API_BEGIN
+  F3R_CHECK(n >= 0 && rptr && cptr && cwidth && perm && C >= 1 && sigma >= 1 && K >= 1, F3R_ERR_ARG,
+            "sell_layout: bad argument");
+  HostCSR h = host_csr_from(n, n, rptr[n], rptr, col, val);
+  HostSell hs = build_sell(h, C, sigma, K, nullptr, false);
+  std::memcpy(cptr, hs.cptr.data(), sizeof(int32_t) * hs.cptr.size());
+  std::memcpy(cwidth, hs.cwidth.data(), sizeof(int32_t) * hs.cwidth.size());
+  std::memcpy(perm, hs.perm.data(), sizeof(int32_t) * hs.perm.size());
+  if (!hs.idx.empty()) {
+    F3R_CHECK(sidx && sval, F3R_ERR_ARG, "sell_layout: null output");
+    std::memcpy(sidx, hs.idx.data(), sizeof(int32_t) * hs.idx.size());
+    std::memcpy(sval, hs.val.data(), sizeof(double) * hs.val.size());
+  }
+  F3R_API_END
+}
+
+int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(sell && out && sell->m->format == 1 && sell->m->sell_host_val, F3R_ERR_ARG,
+            "mat_cast: needs a SELL matrix");
+  auto m = std::make_shared<Mat>(*sell->m);
+  m->dtype = dtype;
+  m->s_val = upload_values(m->ctx, *m->sell_host_val, dtype);
+  *out = new f3r_mat_s{m};
+  F3R_API_END
+}
+
+int f3r_mat_destroy(f3r_mat A) {
+  F3R_API_BEGIN
+  if (A) {
+    if (A->m->ctx) (void)hipStreamSynchronize(A->m->ctx->stream);
+    delete A;
+  }
+  F3R_API_END
+}
+
+int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *dtype, int *format) {
+  F3R_API_BEGIN
+  F3R_CHECK(A, F3R_ERR_ARG, "null matrix");
+  const Mat &m = *A->m;
+  if (nrows) *nrows = m.host ? m.host->n : m.sh->n;
+  if (nnz) *nnz = m.host ? m.host->nnz() : m.sh->nnz;
+  if (nslots) *nslots = m.format == 1 ? m.sh->nslots : (m.host ? m.host->nnz() : 0);
+  if (dtype) *dtype = m.dtype;
+  if (format) *format = m.format;
+  F3R_API_END
+}
+
+int f3r_spmv(f3r_mat A, f3r_vec x, f3r_vec y) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && x && y, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(A->m->ctx, F3R_ERR_STATE, "spmv: host-only matrix (created without a context)");
+  mat_apply(A->m.get(), &x->v, &y->v, nullptr);
+  F3R_API_END
+}
+int f3r_residual(f3r_mat A, f3r_vec b, f3r_vec x, f3r_vec r) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && b && x && r, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(A->m->ctx, F3R_ERR_STATE, "residual: host-only matrix (created without a context)");
+  mat_apply(A->m.get(), &x->v, &r->v, &b->v);
+  F3R_API_END
+}
+int f3r_spmv_time(f3r_mat A, f3r_vec x, f3r_vec y, int32_t reps, double *ms) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && x && y && ms && reps > 0, F3R_ERR_ARG, "bad argument");
+  hipStream_t st = A->m->ctx->stream;
+  hipEvent_t e0, e1;
+  F3R_HIP(hipEventCreate(&e0));
+  F3R_HIP(hipEventCreate(&e1));
+  mat_apply(A->m.get(), &x->v, &y->v, nullptr);  // warm the caches of the index arrays
+  F3R_HIP(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) mat_apply(A->m.get(), &x->v, &y->v, nullptr);
+  F3R_HIP(hipEventRecord(e1, st));
+  F3R_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  F3R_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  F3R_API_END
+}
+
+static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc, int dtype, f3r_prec *P) {
+  F3R_API_BEGIN
+  F3R_CHECK(csr && P && csr->m->host, F3R_ERR_ARG, "precond: needs a CSR matrix");
+  F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16, F3R_ERR_ARG, "precond: dtype");
+  *P = nullptr;
+  auto p = std::make_shared<Prec>();
+  p->ctx = ctx ? ctx->c : nullptr;
+  p->kind = kind;
+  p->dtype = dtype;
+  p->tol = p1;
+  p->acc = acc;
+  if (kind == 0)
+    p->host = sdainv_host(*csr->m->host, p1, acc);
+  else
+    p->host = ilu0_host(*csr->m->host, (int32_t)p1, acc);
+  p->n = csr->m->host->n;
+  if (p->ctx) prec_build_device(p.get(), nullptr, 64, 1);
+  *P = new f3r_prec_s{p};
+  F3R_API_END
+}
+
+int f3r_sdainv_create(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtype, f3r_prec *P) {
+  return prec_create(ctx, csr, 0, tol, acc, dtype, P);
+}
+int f3r_ilu0_create(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype, f3r_prec *P) {
+  return prec_create(ctx, csr, 1, (double)blocks, acc, dtype, P);
+}
+int f3r_prec_apply(f3r_prec P, f3r_vec in, f3r_vec out) {
+  F3R_API_BEGIN
+  F3R_CHECK(P && in && out, F3R_ERR_ARG, "null argument");
+  Prec *p = P->p.get();
+  F3R_CHECK(p->ctx, F3R_ERR_STATE, "prec_apply: host-only preconditioner (created without a context)");
+  F3R_CHECK(in->v.dtype == p->dtype && out->v.dtype == p->dtype && in->v.n == p->n && out->v.n == p->n,
+            F3R_ERR_ARG, "prec_apply: vectors must have the preconditioner's type and size");
+  prec_apply_raw(p, in->v.ptr(), out->v.ptr(), p->ctx->stream);
+  F3R_API_END
+}
+int f3r_prec_factor_nnz(f3r_prec P, int which, int64_t *nnz) {
+  F3R_API_BEGIN
+  F3R_CHECK(P && nnz && (which == 0 || which == 1), F3R_ERR_ARG, "bad argument");
+  *nnz = P->p->host.f[which].nnz();
+  F3R_API_END
+}
+int f3r_prec_factor_get(f3r_prec P, int which, int32_t *rptr, int32_t *col, double *val) {
+  F3R_API_BEGIN
+  F3R_CHECK(P && rptr && (which == 0 || which == 1), F3R_ERR_ARG, "bad argument");
+  const HostCSR &F = P->p->host.f[which];
+  std::memcpy(rptr, F.rptr.data(), sizeof(int32_t) * (F.n + 1));
+  if (F.nnz()) {
+    F3R_CHECK(col && val, F3R_ERR_ARG, "null output");
+    std::memcpy(col, F.idx.data(), sizeof(int32_t) * F.nnz());
+    std::memcpy(val, F.val.data(), sizeof(double) * F.nnz());
+  }
+  F3R_API_END
+}
+int f3r_prec_destroy(f3r_prec P) {
+  F3R_API_BEGIN
+  if (P) {
+    if (P->p->ctx) (void)hipStreamSynchronize(P->p->ctx->stream);
+    delete P;
+  }
+  F3R_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+// Device-side building blocks for CDNA4 (gfx950): 64-wide wavefront
+// reductions, LDS block reductions, the single-pass grid reduction with a
+// last-block finish, vector loads of SELL slot groups and precision casts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace f3r {
+
+using half_t = _Float16;
+
+constexpr int kBlock = 256;          // threads per block of the streaming kernels
+constexpr int kWave = 64;            // CDNA wavefront
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ------------------------------------------------------------ precision ---
+template <class T> struct Acc { using type = float; };   // fp16 / fp32 data: fp32 sums
+template <> struct Acc<double> { using type = double; };
+
+__device__ __forceinline__ float to_acc(half_t v) { return (float)v; }
+__device__ __forceinline__ float to_acc(float v) { return v; }
+__device__ __forceinline__ double to_acc(double v) { return v; }
+
+template <class T> __device__ __forceinline__ T fma_(T a, T b, T c);
+template <> __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// store conversions, round to nearest even
+template <class D> __device__ __forceinline__ D cvt(float v);
+template <class D> __device__ __forceinline__ D cvt(double v);
+template <> __device__ __forceinline__ half_t cvt<half_t>(float v) { return (half_t)v; }
+template <> __device__ __forceinline__ float cvt<float>(float v) { return v; }
+template <> __device__ __forceinline__ double cvt<double>(float v) { return (double)v; }
+template <> __device__ __forceinline__ float cvt<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ double cvt<double>(double v) { return v; }
+// fp64 -> fp16 correctly rounded, without the double rounding of a detour
+// through fp32 (integer arithmetic on the fp64 bit pattern)
+__device__ __forceinline__ half_t d2h(double d) {
+  uint64_t u = __double_as_longlong(d);
+  uint16_t sign = (uint16_t)((u >> 48) & 0x8000u);
+  uint64_t mag = u & 0x7fffffffffffffffULL;
+  uint16_t bits;
+  int e = (int)(mag >> 52) - 1023;
+  if (mag >= 0x7ff0000000000000ULL) {
+    bits = sign | 0x7c00u | (mag > 0x7ff0000000000000ULL ? 0x0200u : 0u);
+  } else if ((mag >> 52) == 0 || e < -25) {
+    bits = sign;
+  } else if (e > 15) {
+    bits = sign | 0x7c00u;
+  } else {
+    uint64_t m = (mag & 0xfffffffffffffULL) | (1ULL << 52);
+    int drop = e >= -14 ? 42 : 42 + (-14 - e);
+    uint64_t q = m >> drop;
+    uint64_t rest = m & ((1ULL << drop) - 1), hw = 1ULL << (drop - 1);
+    q += (rest > hw || (rest == hw && (q & 1))) ? 1 : 0;
+    if (e >= -14) {
+      uint32_t ex = (uint32_t)(e + 15);
+      if (q >> 11) {
+        q >>= 1;
+        ++ex;
+      }
+      bits = ex >= 31 ? (uint16_t)(sign | 0x7c00u) : (uint16_t)(sign | (ex << 10) | (q & 0x3ffu));
+    } else {
+      bits = (uint16_t)(sign | q);
+    }
+  }
+  return __builtin_bit_cast(half_t, bits);
+}
+template <> __device__ __forceinline__ half_t cvt<half_t>(double v) { return d2h(v); }
+
+// -------------------------------------------------------------- loads ----
+// K consecutive elements starting at an address aligned to K elements.
+template <class T, int K> struct Pack { T v[K]; };
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <class T, int K>
+__device__ __forceinline__ Pack<T, K> load_pack(const T *__restrict__ p) {
+  Pack<T, K> r;
+  if constexpr (K * sizeof(T) == 16) {
+    u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    __builtin_memcpy(&r, &w, 16);
+  } else if constexpr (K * sizeof(T) == 8) {
+    u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p));
+    __builtin_memcpy(&r, &w, 8);
+  } else if constexpr (K * sizeof(T) == 4) {
+    uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+    __builtin_memcpy(&r, &w, 4);
+  } else if constexpr (K * sizeof(T) == 32) {
+    const u32x4 *q = reinterpret_cast<const u32x4 *>(p);
+    u32x4 w0 = __builtin_nontemporal_load(q), w1 = __builtin_nontemporal_load(q + 1);
+    __builtin_memcpy(&r, &w0, 16);
+    __builtin_memcpy(reinterpret_cast<char *>(&r) + 16, &w1, 16);
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) r.v[k] = __builtin_nontemporal_load(p + k);
+  }
+  return r;
+}
+
+// --------------------------------------------------------- reductions ----
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sums NV values per thread over the block; thread 0 receives the totals.
+template <class T, int NV>
+__device__ __forceinline__ void block_sum(T (&v)[NV], T *lds /* NV * kWavesPerBlock */) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[i * kWavesPerBlock + wid] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      T s = lds[i * kWavesPerBlock];
+#pragma unroll
+      for (int w = 1; w < kWavesPerBlock; ++w) s += lds[i * kWavesPerBlock + w];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+template <class T> __device__ __forceinline__ T load_coherent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Single-pass grid reduction of NV sums.  Every block writes its NV partial
+// sums to partial[i * gridDim.x + block]; the last block to arrive (counted
+// in *counter) adds all partials in block order -- a fixed order, so the
+// result does not depend on scheduling -- and returns true with the totals in
+// `out` of thread 0.  It resets *counter to zero for the next launch.
+template <class T, int NV>
+__device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
+                                         unsigned *__restrict__ counter, T *lds) {
+  block_sum<T, NV>(v, lds);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      __hip_atomic_store(&partial[i * gridDim.x + blockIdx.x], v[i], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  T acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    acc[i] = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+      acc[i] += load_coherent(&partial[i * gridDim.x + b]);
+  }
+  block_sum<T, NV>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = acc[i];
+    *counter = 0;
+  }
+  return true;
+}
+
+// Grid size of the reduction kernels: enough blocks to stream from HBM,
+// capped so that the last block's pass over the partials stays short.
+inline int reduce_grid(int64_t n, int num_cu) {
+  int64_t want = (n + kBlock * 8 - 1) / (kBlock * 8);
+  int64_t cap = (int64_t)num_cu * 4;
+  if (cap > 1024) cap = 1024;
+  if (want > cap) want = cap;
+  return want < 1 ? 1 : (int)want;
+}
+
+}  // namespace f3r

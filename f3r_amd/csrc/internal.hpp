@@ -1,0 +1,69 @@
+// Declarations shared between the translation units of libf3r_amd.
+#pragma once
+
+#include "common.hpp"
+#include "sell_kernel.hpp"
+
+namespace f3r {
+
+// Innermost preconditioner: SDAINV (M = Z D^-1 W^T as two SELL SpMVs, or two
+// diagonal scalings when both factors are diagonal) or ILU(0) (level-scheduled
+// triangular solves).  Values in `dtype`, sums in fp32 for fp16 / fp32 data.
+struct Prec {
+  Context *ctx = nullptr;
+  int kind = 0;  // 0 SDAINV, 1 ILU(0)
+  int dtype = F3R_F16;
+  double tol = 0.1, acc = 1.0;
+  int32_t n = 0;
+  HostFactors host;
+  DevPtr tmp;
+  // SDAINV
+  std::shared_ptr<Mat> W, Z;  // W holds W^T (rows w_i), Z holds Z D^-1
+  bool diag = false;
+  DevPtr dw, dz;              // diagonals when `diag`
+  // ILU(0)
+  std::vector<int32_t> l_lptr, u_lptr;  // level pointers into *_rows
+  DevPtr l_rows, u_rows, l_rptr, l_idx, l_val, u_rptr, u_idx, u_val, u_dinv;
+};
+
+Context *ctx_create(int device);
+void ctx_destroy(Context *c);
+double read_scalar(Context *c, const double *dev);
+SellArgs sell_args(const SellShared &sh, bool with_perm);
+DevPtr upload_values(Context *ctx, const std::vector<double> &v, int dtype);
+HostCSR permute_csr(const HostCSR &M, const std::vector<int32_t> &perm);
+void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K);
+void prec_apply_raw(Prec *P, const void *in, void *out, hipStream_t st);
+void vec_copy(Vec *s, Vec *d);
+// dot product of two device arrays (fp16 data summed in fp32), host result
+double dot_raw(Context *c, int da, const void *a, int db, const void *b, int64_t n);
+
+#define F3R_API_BEGIN try {
+#define F3R_API_END                                   \
+  }                                                   \
+  catch (const f3r::Error &e) {                       \
+    f3r::set_error(e.msg);                            \
+    return e.status;                                  \
+  }                                                   \
+  catch (const std::bad_alloc &) {                    \
+    f3r::set_error("host allocation failed");         \
+    return F3R_ERR_ALLOC;                             \
+  }                                                   \
+  catch (const std::exception &e) {                   \
+    f3r::set_error(e.what());                         \
+    return F3R_ERR_ARG;                               \
+  }                                                   \
+  return F3R_OK;
+
+}  // namespace f3r
+
+struct f3r_ctx_s;
+struct f3r_vec_s;
+struct f3r_mat_s;
+struct f3r_prec_s;
+namespace f3r {
+Context *ctx_of(f3r_ctx c);
+Vec *vec_of(f3r_vec v);
+std::shared_ptr<Mat> mat_of(f3r_mat m);
+std::shared_ptr<Prec> prec_of(f3r_prec p);
+}  // namespace f3r

@@ -1,0 +1,101 @@
+// SELL-C-sigma SpMV on CDNA4: one row per lane, chunks of C rows, K slots of
+// a row interleaved so that each lane issues one K-wide vector load of values
+// and one of column indices per slot group; a wavefront of 64 consecutive
+// rows therefore reads 64*K contiguous values and indices per step.  The row
+// sum is a fused multiply-add chain in slot order (= the CSR order of the
+// row, as SELL::operate sums it, sparse/matrix/sell.hpp:63-69).  What happens
+// with the sum is the epilogue's business: plain store, residual, or the
+// fused Krylov / Richardson updates of the solver.
+#pragma once
+
+#include "device.hpp"
+
+namespace f3r {
+
+struct SellArgs {
+  int32_t n;        // rows
+  int32_t log2C;    // chunk height C = 1 << log2C
+  const int32_t *__restrict__ cptr;    // element offset of each chunk
+  const int32_t *__restrict__ cwidth;  // slot groups of each chunk
+  const int32_t *__restrict__ idx;     // column of each slot
+  const int32_t *__restrict__ perm;    // sorted row -> original row (null = identity)
+};
+
+// Sum of row r (in sorted order) of A*x with accumulator ACC.
+template <class V, class X, class ACC, int K>
+__device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__ val,
+                                        const X *__restrict__ x, int r) {
+  const int s = r >> a.log2C;
+  const int j = r - (s << a.log2C);
+  const int C = 1 << a.log2C;
+  const int rem = a.n - (s << a.log2C);
+  const int tc = rem < C ? rem : C;
+  const int groups = a.cwidth[s];
+  int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K;
+  const int64_t step = (int64_t)tc * K;
+  ACC acc = 0;
+  for (int g = 0; g < groups; ++g, pos += step) {
+    const Pack<V, K> pv = load_pack<V, K>(val + pos);
+    const Pack<int32_t, K> pi = load_pack<int32_t, K>(a.idx + pos);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[pi.v[k]]), acc);
+  }
+  return acc;
+}
+
+// An epilogue provides
+//   struct State;                                   per-thread carry (may be empty)
+//   __device__ void row(int r, ACC acc, State &s);  called for every row r < n
+//   __device__ void finish(State &s);               called once by every thread
+// finish() may reduce over the block and the grid.
+struct NoState {};
+
+template <class V, class X, class ACC, int K, class Epi>
+__global__ __launch_bounds__(kBlock) void k_sell(SellArgs a, const V *__restrict__ val,
+                                                 const X *__restrict__ x, Epi epi) {
+  typename Epi::State st{};
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < a.n; r += gridDim.x * kBlock)
+    epi.row(r, sell_row<V, X, ACC, K>(a, val, x, r), st);
+  epi.finish(st);
+}
+
+// grid: 0 = one row per thread; otherwise the given (capped) block count
+template <class V, class X, class ACC, class Epi>
+inline void launch_sell(int K, const SellArgs &a, const V *val, const X *x, Epi epi,
+                        hipStream_t st, int grid = 0) {
+  if (a.n == 0) return;
+  const int full = (a.n + kBlock - 1) / kBlock;
+  dim3 g(grid > 0 && grid < full ? grid : full);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((k_sell<V, X, ACC, 1, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+    case 2: hipLaunchKernelGGL((k_sell<V, X, ACC, 2, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+    case 4: hipLaunchKernelGGL((k_sell<V, X, ACC, 4, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+    default: break;
+  }
+}
+
+// ------------------------------------------------------------ epilogues ---
+template <class Y, class ACC> struct EpiStore {
+  using State = NoState;
+  Y *__restrict__ y;
+  const int32_t *__restrict__ perm;
+  __device__ void row(int r, ACC acc, State &) const {
+    const int row = perm ? perm[r] : r;
+    y[row] = cvt<Y>(acc);
+  }
+  __device__ void finish(State &) const {}
+};
+
+template <class Y, class B, class ACC> struct EpiResid {
+  using State = NoState;
+  Y *__restrict__ y;
+  const B *__restrict__ b;
+  const int32_t *__restrict__ perm;
+  __device__ void row(int r, ACC acc, State &) const {
+    const int row = perm ? perm[r] : r;
+    y[row] = cvt<Y>((ACC)to_acc(b[row]) - acc);
+  }
+  __device__ void finish(State &) const {}
+};
+
+}  // namespace f3r

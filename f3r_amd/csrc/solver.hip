@@ -1,0 +1,933 @@
+// The F3R nested solve on one GPU:
+//   Restarted<double>(FGMRES<double>(A, L2, {m1, true}), {restart})
+//   L2 = FGMRES<T2>(A2, L3, {m2, false})     fp32 basis (fp64 in DOUBLE)
+//   L3 = FGMRES<T2>(A3, L4, {m3, false})     matrix values fp16 in HALF
+//   L4 = adaptive Richardson(A4, M, m4, c)   fp16 storage in HALF
+// as composed in work/src/f3r-gpu.cpp:210-266.
+//
+// Everything but the outer convergence test stays on the device: Hessenberg
+// columns, Givens rotations, residual estimates and the adaptive Richardson
+// weights are written by the last block of the reduction kernels that
+// produce them, so no level below the outer one synchronises with the host.
+// The body of level 2 (m2 level-3 solves and Arnoldi steps) is captured once
+// per pattern of probe calls in a HIP graph and replayed.
+//
+// Vectors live in the internal row order of the SELL layout (rows sorted
+// inside sigma windows, the matrix renumbered to P A P^T); b and x are
+// permuted on entry and exit.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "common.hpp"
+#include "internal.hpp"
+#include "sell_kernel.hpp"
+
+namespace f3r {
+
+// ------------------------------------------------------- device state ---
+// Adaptive Richardson weights of the innermost level (work/src/f3r-gpu.cpp:
+// 224-227, 276-281): sum_k starts at 1, ome_k = sum_k / den, den = number of
+// probes + 1.  Arithmetic in fp32 for the HALF / FLOAT variants.
+struct RichDev {
+  double sum[16];
+  double ome[16];
+  double alpha[16];
+  int nprobe;
+};
+
+template <class T, int NV> struct Cols {
+  const T *p[NV];
+};
+
+// Apply the previous rotations to column j of H, generate the rotation of
+// step j and rotate e (Blas1::full_grot / final_grot, gmres.hpp:14-38).
+template <class T>
+__device__ void givens_step(T *H, T *cs, T *sn, T *e, int j, bool e0_one) {
+  for (int i = 0; i < j; ++i) {
+    T t = H[i], h1 = H[i + 1];
+    H[i] = fma_<T>(cs[i], t, sn[i] * h1);
+    H[i + 1] = fma_<T>(-sn[i], t, cs[i] * h1);
+  }
+  T t = (T)sqrt(fma_<T>(H[j], H[j], H[j + 1] * H[j + 1]));
+  cs[j] = H[j] / t;
+  sn[j] = H[j + 1] / t;
+  H[j] = t;
+  H[j + 1] = 0;
+  T ej = (j == 0 && e0_one) ? (T)1 : e[j];
+  e[j + 1] = -sn[j] * ej;
+  e[j] = cs[j] * ej;
+}
+
+// back substitution of the j x j upper triangle of H (Blas1::trsv, gmres.hpp:41-53)
+template <class T>
+__device__ void hess_solve(const T *H, int ldh, const T *e, T *y, int j) {
+  for (int i = j - 1; i >= 0; --i) {
+    T t = e[i];
+    for (int q = j - 1; q > i; --q) t = fma_<T>(-H[q * ldh + i], y[q], t);
+    y[i] = t / H[i * ldh + i];
+  }
+}
+
+// ------------------------------------------------- Arnoldi epilogues ----
+// w = A z stored into V_{j+1}, and the dots <w, V_k>, k <= j, reduced in the
+// same pass; the last block writes H(j, 0..j).
+template <class T, int NV> struct EpiDots {
+  struct State {
+    T v[NV];
+  };
+  T *__restrict__ w;
+  Cols<T, NV> cols;
+  int ncols;
+  T *Hcol;
+  T *partial;
+  unsigned *counter;
+  __device__ void row(int r, T acc, State &s) const {
+    w[r] = acc;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k < ncols) s.v[k] = fma_<T>(acc, cols.p[k][r], s.v[k]);
+  }
+  __device__ void finish(State &s) const {
+    __shared__ T lds[NV * kWavesPerBlock];
+    if (grid_sum<T, NV>(s.v, partial, counter, lds) && threadIdx.x == 0)
+      for (int k = 0; k < ncols; ++k) Hcol[k] = s.v[k];
+  }
+};
+
+// r = b - A x into V_0 and ||r||: e_0 = ||r||, inv = 1/||r|| (FGMRES::solve +
+// the normalised start of FGMRES::kernel, gmres.hpp:84-102)
+template <class T> struct EpiResidNorm {
+  struct State {
+    T v[1];
+  };
+  T *__restrict__ v0;
+  const T *__restrict__ b;
+  T *e;
+  T *inv;
+  double *status;
+  T *partial;
+  unsigned *counter;
+  __device__ void row(int r, T acc, State &s) const {
+    T rv = b[r] - acc;
+    v0[r] = rv;
+    s.v[0] = fma_<T>(rv, rv, s.v[0]);
+  }
+  __device__ void finish(State &s) const {
+    __shared__ T lds[kWavesPerBlock];
+    if (grid_sum<T, 1>(s.v, partial, counter, lds) && threadIdx.x == 0) {
+      T h = (T)sqrt(s.v[0]);
+      e[0] = h;
+      *inv = (T)1 / h;
+      *status = fabs((double)h);
+    }
+  }
+};
+
+// V_{j+1} -= V H(j)[0..j] (MULMV order: V_0 h_0, then fused adds), ||V_{j+1}||,
+// then the last block: H(j, j+1), 1/h, Givens, |e_{j+1}| for the host.
+template <class T, int NV>
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict__ w, Cols<T, NV> cols,
+                                                       int ncols, T *Hcol, T *cs, T *sn, T *e, int j,
+                                                       int e0_one, T *inv, double *status, T *partial,
+                                                       unsigned *counter) {
+  __shared__ T lds[kWavesPerBlock];
+  T h[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
+  T v[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    T r = cols.p[0][i] * h[0];
+#pragma unroll
+    for (int k = 1; k < NV; ++k)
+      if (k < ncols) r = fma_<T>(cols.p[k][i], h[k], r);
+    T wn = w[i] - r;
+    w[i] = wn;
+    v[0] = fma_<T>(wn, wn, v[0]);
+  }
+  if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
+    T nv = (T)sqrt(v[0]);
+    Hcol[j + 1] = nv;
+    *inv = (T)1 / nv;
+    givens_step<T>(Hcol, cs, sn, e, j, e0_one != 0);
+    if (status) *status = fabs((double)e[j + 1]);
+  }
+}
+
+// the same for the outer level: columns V_0..V_j contiguous (ld = n), j <= 100
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cgs_update_strided(int64_t n, T *__restrict__ w,
+                                                               const T *__restrict__ V, int ncols,
+                                                               T *Hcol, T *cs, T *sn, T *e, int j, T *inv,
+                                                               double *status, T *partial,
+                                                               unsigned *counter) {
+  __shared__ T lds[kWavesPerBlock];
+  __shared__ T h[128];
+  for (int k = threadIdx.x; k < ncols; k += kBlock) h[k] = Hcol[k];
+  __syncthreads();
+  T v[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    T r = V[i] * h[0];
+    for (int k = 1; k < ncols; ++k) r = fma_<T>(V[(int64_t)k * n + i], h[k], r);
+    T wn = w[i] - r;
+    w[i] = wn;
+    v[0] = fma_<T>(wn, wn, v[0]);
+  }
+  if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
+    T nv = (T)sqrt(v[0]);
+    Hcol[j + 1] = nv;
+    *inv = (T)1 / nv;
+    givens_step<T>(Hcol, cs, sn, e, j, false);
+    *status = fabs((double)e[j + 1]);
+  }
+}
+
+// <w, V_k> for k < ncols, V contiguous; blockIdx.y selects 16 columns
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mdot_strided(int64_t n, const T *__restrict__ w,
+                                                         const T *__restrict__ V, int ncols, T *Hcol,
+                                                         T *partial, unsigned *counters) {
+  constexpr int NV = 16;
+  __shared__ T lds[NV * kWavesPerBlock];
+  const int k0 = blockIdx.y * NV;
+  T v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T wi = w[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k0 + k < ncols) v[k] = fma_<T>(wi, V[(int64_t)(k0 + k) * n + i], v[k]);
+  }
+  if (grid_sum<T, NV>(v, partial + (int64_t)blockIdx.y * NV * gridDim.x, counters + blockIdx.y, lds) &&
+      threadIdx.x == 0)
+    for (int k = 0; k < NV && k0 + k < ncols; ++k) Hcol[k0 + k] = v[k];
+}
+
+template <class T>
+__global__ void k_vscale(int64_t n, T *__restrict__ v, const T *__restrict__ inv) {
+  const T a = *inv;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    v[i] *= a;
+}
+
+// out = Z y (with_init) of an inner level; every block solves the small
+// triangular system itself (j <= NV)
+template <class T, class TO, int NV>
+__global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<T, NV> Z, int j, const T *H, int ldh,
+                                                     const T *e, TO *__restrict__ out) {
+  __shared__ T y[NV];
+  if (threadIdx.x == 0) hess_solve<T>(H, ldh, e, y, j);
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    T r = Z.p[0][i] * y[0];
+#pragma unroll
+    for (int k = 1; k < NV; ++k)
+      if (k < j) r = fma_<T>(Z.p[k][i], y[k], r);
+    out[i] = cvt<TO>(r);
+  }
+}
+
+template <class T>
+__global__ void k_hess_solve(const T *H, int ldh, const T *e, T *y, int j) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) hess_solve<T>(H, ldh, e, y, j);
+}
+
+// x += Z y for the outer level (Z contiguous)
+template <class T>
+__global__ void k_solution_strided(int64_t n, const T *__restrict__ Z, int j, const T *__restrict__ y,
+                                   T *__restrict__ x) {
+  __shared__ T ys[128];
+  for (int k = threadIdx.x; k < j; k += kBlock) ys[k] = y[k];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    T r = Z[i] * ys[0];
+    for (int k = 1; k < j; ++k) r = fma_<T>(Z[(int64_t)k * n + i], ys[k], r);
+    x[i] = x[i] + r;
+  }
+}
+
+template <class S, class D>
+__global__ void k_cast2(int64_t n, const S *__restrict__ s, D *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    d[i] = cvt<D>(to_acc(s[i]));
+}
+
+__global__ void k_gather(int64_t n, const double *__restrict__ s, const int32_t *__restrict__ perm,
+                         double *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    d[i] = s[perm[i]];
+}
+__global__ void k_scatter(int64_t n, const double *__restrict__ s, const int32_t *__restrict__ perm,
+                          double *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    d[perm[i]] = s[i];
+}
+
+// ---------------------------------------------- Richardson (fp16 level) ---
+// Entry of one call (work/src/f3r-gpu.cpp:231-243 in this build's
+// arithmetic): hin = fp16(in * s); with diagonal SDAINV factors also
+// hout = fp16(z * fp16(w * hin)) and, outside probe calls, hout *= ome_0.
+__global__ void k_l4_entry(int64_t n, const float *__restrict__ in, float s, half_t *__restrict__ hin,
+                           half_t *__restrict__ hout, const half_t *__restrict__ dw,
+                           const half_t *__restrict__ dz, const RichDev *st, int scale_ome) {
+  const float om = scale_ome ? (float)st->ome[0] : 1.f;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const half_t h = (half_t)(in[i] * s);
+    hin[i] = h;
+    if (dw) {
+      const half_t t = (half_t)__builtin_fmaf((float)dw[i], (float)h, 0.f);
+      half_t o = (half_t)__builtin_fmaf((float)dz[i], (float)t, 0.f);
+      if (scale_ome) o = (half_t)((float)o * om);
+      hout[i] = o;
+    }
+  }
+}
+
+// hout *= ome_0 (or alpha_0 in probe calls)
+__global__ void k_l4_scale(int64_t n, half_t *__restrict__ hout, const double *w) {
+  const float a = (float)*w;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    hout[i] = (half_t)((float)hout[i] * a);
+}
+
+// hout' = fp16(fma(w_k, tmp2, hout)); last step: out = fp32(hout') / s
+__global__ void k_l4_update(int64_t n, const half_t *__restrict__ tmp2, const half_t *__restrict__ hout,
+                            const double *w, half_t *__restrict__ hnext, float *__restrict__ out,
+                            float inv_s) {
+  const float a = (float)*w;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const half_t h = (half_t)__builtin_fmaf(a, (float)tmp2[i], (float)hout[i]);
+    if (out)
+      out[i] = (float)h * inv_s;
+    else
+      hnext[i] = h;
+  }
+}
+
+// Richardson step k fused into the fp16 residual SpMV, diagonal factors:
+// hr = fp16(hin - A16 hout), tmp2 = fp16(z * fp16(w * hr)), and outside
+// probe calls hout' = fp16(fma(ome_k, tmp2, hout)) (last step: out = hout'/s).
+struct EpiL4Diag {
+  using State = NoState;
+  const half_t *__restrict__ hin;
+  const half_t *__restrict__ hout;
+  const half_t *__restrict__ dw;
+  const half_t *__restrict__ dz;
+  const double *w;  // ome_k
+  half_t *__restrict__ hr_out;    // probe calls: keep hr and tmp2
+  half_t *__restrict__ tmp2_out;
+  half_t *__restrict__ hnext;
+  float *__restrict__ out;
+  float inv_s;
+  __device__ void row(int r, float acc, State &) const {
+    const half_t hr = (half_t)((float)hin[r] - acc);
+    const half_t t = (half_t)__builtin_fmaf((float)dw[r], (float)hr, 0.f);
+    const half_t t2 = (half_t)__builtin_fmaf((float)dz[r], (float)t, 0.f);
+    if (hr_out) {
+      hr_out[r] = hr;
+      tmp2_out[r] = t2;
+      return;
+    }
+    const half_t h = (half_t)__builtin_fmaf((float)*w, (float)t2, (float)hout[r]);
+    if (out)
+      out[r] = (float)h * inv_s;
+    else
+      hnext[r] = h;
+  }
+  __device__ void finish(State &) const {}
+};
+
+// probe: alpha_k = <t, r> / <t, t> (fp32 sums of fp16 data); the last block
+// updates sum_k, ome_k (work/src/f3r-gpu.cpp:234-240, 248-254)
+__global__ __launch_bounds__(kBlock) void k_probe(int64_t n, const half_t *__restrict__ t,
+                                                  const half_t *__restrict__ r, RichDev *st, int k,
+                                                  float *partial, unsigned *counter) {
+  __shared__ float lds[2 * kWavesPerBlock];
+  float v[2] = {0.f, 0.f};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float ti = (float)t[i];
+    v[0] = __builtin_fmaf(ti, ti, v[0]);
+    v[1] = __builtin_fmaf(ti, (float)r[i], v[1]);
+  }
+  if (grid_sum<float, 2>(v, partial, counter, lds) && threadIdx.x == 0) {
+    const float al = v[1] / v[0];
+    if (k == 0) st->nprobe += 1;
+    const float den = (float)(st->nprobe + 1);
+    const float s = (float)st->sum[k] + al;
+    st->sum[k] = s;
+    st->ome[k] = s / den;
+    st->alpha[k] = al;
+  }
+}
+
+__global__ void k_rich_reset(RichDev *st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int k = 0; k < 16; ++k) {
+      st->sum[k] = 1.0;
+      st->ome[k] = 1.0;
+      st->alpha[k] = 0.0;
+    }
+    st->nprobe = 0;
+  }
+}
+
+// ------------------------------------------------------------ the solver ---
+template <class T> struct InnerLevel {
+  int m = 0;
+  DevPtr Vbuf, Zbuf, H, cs, sn, e, inv;
+  std::vector<T *> V, Z;  // column pointers; V[0] aliases the level's input
+  int ldh() const { return m + 1; }
+  T *Hp() const { return H->as<T>(); }
+};
+
+struct Solver {
+  Context *ctx = nullptr;
+  f3r_params p{};
+  int32_t n = 0;
+  int64_t nnz = 0;
+  std::shared_ptr<SellShared> sh;
+  DevPtr A64, A32, A16;
+  std::shared_ptr<Prec> P;
+  float range_s = 1.f;
+  // outer level (fp64)
+  DevPtr V1, Z1, H1, c1, s1, e1, y1, inv1, status, b_int, x_int;
+  // inner levels (fp32 basis)
+  InnerLevel<float> L2, L3;
+  // Richardson (fp16)
+  DevPtr hin, hout, hout2, hr, tmp2, tt, rich;
+  int64_t cnt = 0, calls = 0;
+  std::vector<double> hist;
+  std::map<uint64_t, hipGraphExec_t> graphs;
+  std::vector<hipGraph_t> graph_src;
+  bool capturing = false;
+  int red_grid = 1;
+
+  ~Solver() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto &g : graph_src) (void)hipGraphDestroy(g);
+  }
+
+  hipStream_t st() const { return ctx->stream; }
+  int grid() const {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    return (int)std::min<int64_t>(g, (int64_t)ctx->num_cu * 16);
+  }
+  SellArgs args() const { return sell_args(*sh, false); }
+  unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + i; }
+  template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
+
+  // ---------------------------------------------------- inner Arnoldi ---
+  template <class V, int NV>
+  void spmv_dots(InnerLevel<float> &L, const V *val, int j) {
+    Cols<float, NV> c{};
+    for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
+    EpiDots<float, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<float>(), counter(0)};
+    launch_sell<V, float, float>(sh->K, args(), val, L.Z[j], epi, st(), red_grid);
+  }
+  template <int NV> void cgs_update(InnerLevel<float> &L, int j) {
+    Cols<float, NV> c{};
+    for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
+    hipLaunchKernelGGL((k_cgs_update<float, NV>), dim3(red_grid), dim3(kBlock), 0, st(), (int64_t)n,
+                       L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->as<float>(),
+                       L.sn->as<float>(), L.e->as<float>(), j, 1, L.inv->as<float>(), (double *)nullptr,
+                       partial<float>(), counter(1));
+  }
+  template <class V> void arnoldi(InnerLevel<float> &L, const V *val, int j) {
+    if (j + 1 <= 4) {
+      spmv_dots<V, 4>(L, val, j);
+      cgs_update<4>(L, j);
+    } else if (j + 1 <= 8) {
+      spmv_dots<V, 8>(L, val, j);
+      cgs_update<8>(L, j);
+    } else {
+      spmv_dots<V, 16>(L, val, j);
+      cgs_update<16>(L, j);
+    }
+    hipLaunchKernelGGL(k_vscale<float>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
+                       L.inv->as<float>());
+  }
+  template <class TO> void solution(InnerLevel<float> &L, TO *out) {
+    const int j = L.m;
+    auto go = [&](auto nvtag) {
+      constexpr int NV = decltype(nvtag)::value;
+      Cols<float, NV> c{};
+      for (int k = 0; k < j; ++k) c.p[k] = L.Z[k];
+      hipLaunchKernelGGL((k_solution<float, TO, NV>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
+                         L.Hp(), L.ldh(), L.e->as<float>(), out);
+    };
+    if (j <= 4) go(std::integral_constant<int, 4>());
+    else if (j <= 8) go(std::integral_constant<int, 8>());
+    else go(std::integral_constant<int, 16>());
+  }
+
+  // ------------------------------------------------- level 4 (fp16) ---
+  void level4(const float *in, float *out) {
+    ++cnt;
+    ++calls;
+    const bool probe = (cnt % p.c) == 0;
+    const int64_t nn = n;
+    const dim3 g(grid()), b(kBlock);
+    RichDev *rs = rich->as<RichDev>();
+    half_t *hi = hin->as<half_t>(), *ho = hout->as<half_t>(), *ho2 = hout2->as<half_t>();
+    const float inv_s = 1.f / range_s;
+    const bool diag = P->kind == 0 && P->diag;
+    const half_t *dw = diag ? P->dw->as<half_t>() : nullptr;
+    const half_t *dz = diag ? P->dz->as<half_t>() : nullptr;
+    const half_t *a16 = A16->as<half_t>();
+    SellArgs a = args();
+    // entry
+    hipLaunchKernelGGL(k_l4_entry, g, b, 0, st(), nn, in, range_s, hi, ho, dw, dz, (const RichDev *)rs,
+                       (diag && !probe) ? 1 : 0);
+    if (!diag) {
+      prec_apply_raw(P.get(), hi, ho, st());
+      if (!probe) hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->ome[0]);
+    }
+    if (probe) {
+      half_t *t = tt->as<half_t>();
+      launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, EpiStore<half_t, float>{t, nullptr}, st());
+      hipLaunchKernelGGL(k_probe, dim3(red_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hi, rs,
+                         0, partial<float>(), counter(2));
+      hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->alpha[0]);
+    }
+    for (int k = 1; k < p.m4; ++k) {
+      const bool last = k == p.m4 - 1;
+      half_t *hr_ = hr->as<half_t>(), *t2 = tmp2->as<half_t>();
+      if (diag) {
+        EpiL4Diag epi{hi, ho, dw, dz, &rs->ome[k], probe ? hr_ : nullptr, probe ? t2 : nullptr,
+                      ho2, last ? out : nullptr, inv_s};
+        launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, epi, st());
+      } else {
+        launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, EpiResid<half_t, half_t, float>{hr_, hi, nullptr},
+                                           st());
+        prec_apply_raw(P.get(), hr_, t2, st());
+      }
+      if (probe) {
+        half_t *t = tt->as<half_t>();
+        launch_sell<half_t, half_t, float>(sh->K, a, a16, t2, EpiStore<half_t, float>{t, nullptr}, st());
+        hipLaunchKernelGGL(k_probe, dim3(red_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hr_, rs,
+                           k, partial<float>(), counter(2));
+        hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
+                           (const double *)&rs->alpha[k], ho2, last ? out : nullptr, inv_s);
+      } else if (!diag) {
+        hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
+                           (const double *)&rs->ome[k], ho2, last ? out : nullptr, inv_s);
+      }
+      if (!last) std::swap(ho, ho2);
+    }
+    if (p.m4 == 1)  // no Richardson step: out = fp32(hout) / s
+      hipLaunchKernelGGL((k_cast2<half_t, float>), g, b, 0, st(), nn, (const half_t *)ho, out);
+  }
+
+  // ----------------------------------------------- level 3 and level 2 ---
+  void level3(const float *in, float *out) {
+    L3.V[0] = const_cast<float *>(in);
+    for (int j = 0; j < L3.m; ++j) {
+      level4(L3.V[j], L3.Z[j]);
+      arnoldi<half_t>(L3, A16->as<half_t>(), j);
+    }
+    solution<float>(L3, out);
+  }
+
+  void level2_body() {
+    for (int j = 0; j < L2.m; ++j) {
+      level3(L2.V[j], L2.Z[j]);
+      arnoldi<float>(L2, A32->as<float>(), j);
+    }
+  }
+
+  // probe positions of the next level-2 body as a bit mask (bit q: the q-th
+  // level-4 call probes); UINT64_MAX when not representable
+  uint64_t next_pattern() const {
+    const int64_t calls_per = (int64_t)L2.m * L3.m;
+    if (calls_per > 63) return UINT64_MAX;
+    uint64_t mask = 0;
+    for (int64_t q = 0; q < calls_per; ++q)
+      if (((cnt + q + 1) % p.c) == 0) mask |= 1ULL << q;
+    return mask | (1ULL << 63);
+  }
+
+  void level2(const double *in, double *out) {
+    hipLaunchKernelGGL((k_cast2<double, float>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, L2.V[0]);
+    uint64_t key = p.use_graph ? next_pattern() : UINT64_MAX;
+    if (key == UINT64_MAX || graphs.size() > 32) {
+      level2_body();
+    } else {
+      auto it = graphs.find(key);
+      if (it == graphs.end()) {
+        const int64_t cnt0 = cnt, calls0 = calls;
+        hipGraph_t gr;
+        F3R_HIP(hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        try {
+          level2_body();
+        } catch (...) {
+          capturing = false;
+          (void)hipStreamEndCapture(st(), &gr);
+          throw;
+        }
+        capturing = false;
+        F3R_HIP(hipStreamEndCapture(st(), &gr));
+        hipGraphExec_t ex;
+        F3R_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        graph_src.push_back(gr);
+        it = graphs.emplace(key, ex).first;
+        cnt = cnt0;  // capture only recorded the calls
+        calls = calls0;
+      }
+      F3R_HIP(hipGraphLaunch(it->second, st()));
+      cnt += (int64_t)L2.m * L3.m;
+      calls += (int64_t)L2.m * L3.m;
+    }
+    solution<double>(L2, out);
+  }
+
+  // -------------------------------------------------------- outer level ---
+  struct Res {
+    bool solved;
+    int iters;
+    double nrm;
+  };
+
+  Res outer_cycle(double nrm_b, bool record) {
+    const int m1 = p.m1;
+    const int64_t nn = n;
+    double *V = V1->as<double>(), *Z = Z1->as<double>(), *H = H1->as<double>();
+    double *stat = status->as<double>();
+    SellArgs a = args();
+    EpiResidNorm<double> er{V, b_int->as<double>(), e1->as<double>(), inv1->as<double>(), stat,
+                            partial<double>(), counter(3)};
+    launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), red_grid);
+    hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V, (const double *)inv1->as<double>());
+    double e = read_scalar(ctx, stat);
+    if (record) hist.push_back(e / nrm_b);
+    if (e / nrm_b < p.eps) return {true, 0, e};
+    int j = 0;
+    bool flag = false;
+    for (; j < m1; ++j) {
+      level2(V + (int64_t)j * nn, Z + (int64_t)j * nn);
+      launch_sell<double, double, double>(sh->K, a, A64->as<double>(), Z + (int64_t)j * nn,
+                                          EpiStore<double, double>{V + (int64_t)(j + 1) * nn, nullptr}, st());
+      const int ncols = j + 1;
+      dim3 gm(red_grid, (ncols + 15) / 16);
+      hipLaunchKernelGGL(k_mdot_strided<double>, gm, dim3(kBlock), 0, st(), nn,
+                         (const double *)(V + (int64_t)(j + 1) * nn), (const double *)V, ncols,
+                         H + (int64_t)j * (m1 + 1), partial<double>(), counter(8));
+      hipLaunchKernelGGL(k_cgs_update_strided<double>, dim3(red_grid), dim3(kBlock), 0, st(), nn,
+                         V + (int64_t)(j + 1) * nn, (const double *)V, ncols, H + (int64_t)j * (m1 + 1),
+                         c1->as<double>(), s1->as<double>(), e1->as<double>(), j, inv1->as<double>(), stat,
+                         partial<double>(), counter(4));
+      hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V + (int64_t)(j + 1) * nn,
+                         (const double *)inv1->as<double>());
+      e = read_scalar(ctx, stat);
+      if (record) hist.push_back(e / nrm_b);
+      if (e / nrm_b < p.eps) {
+        ++j;
+        flag = true;
+        break;
+      }
+    }
+    if (j > 0) {
+      hipLaunchKernelGGL(k_hess_solve<double>, dim3(1), dim3(64), 0, st(), (const double *)H, m1 + 1,
+                         (const double *)e1->as<double>(), y1->as<double>(), j);
+      hipLaunchKernelGGL(k_solution_strided<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, (const double *)Z,
+                         j, (const double *)y1->as<double>(), x_int->as<double>());
+    }
+    F3R_HIP(hipGetLastError());
+    return flag ? Res{true, j, e} : Res{false, m1, e};
+  }
+};
+
+// ----------------------------------------------------------- set-up ------
+static int ilog2_floor(double v) { return v < 1 ? 0 : (int)std::floor(std::log2(v)); }
+
+Solver *solver_create(Context *ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p) {
+  F3R_CHECK(p.variant == 0, F3R_ERR_UNSUPPORTED, "solver: only the HALF variant is built");
+  F3R_CHECK(p.m1 >= 1 && p.m1 <= 120, F3R_ERR_ARG, "solver: m1 must be in [1, 120]");
+  F3R_CHECK(p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16, F3R_ERR_ARG, "solver: m2, m3 in [1, 16]");
+  F3R_CHECK(p.m4 >= 1 && p.m4 <= 16 && p.c >= 1 && p.restart >= 1, F3R_ERR_ARG, "solver: m4 in [1, 16], c, restart >= 1");
+  F3R_CHECK(A.n == A.ncols && A.n > 0, F3R_ERR_ARG, "solver: square non-empty matrix");
+  auto S = std::make_unique<Solver>();
+  S->ctx = ctx;
+  S->p = p;
+  S->n = A.n;
+  S->nnz = A.nnz();
+  const int C = p.C > 0 ? p.C : 64;
+  const int sigma = p.sigma > 0 ? p.sigma : 1;
+  const int K = p.K > 0 ? p.K : 4;
+  F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
+  F3R_CHECK(K == 1 || K == 2 || K == 4, F3R_ERR_ARG, "solver: K must be 1, 2 or 4");
+  HostSell hs = build_sell(A, C, sigma, K, nullptr, true);
+  bool ident = true;
+  for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
+  S->sh = upload_sell_shared(ctx, hs, true);
+  S->sh->nnz = A.nnz();
+  S->sh->sigma = sigma;
+  S->A64 = upload_sell_values(ctx, hs, F3R_F64);
+  S->A32 = upload_sell_values(ctx, hs, F3R_F32);
+  S->A16 = upload_sell_values(ctx, hs, F3R_F16);
+  // innermost preconditioner in the internal order and fp16
+  auto P = std::make_shared<Prec>();
+  P->ctx = ctx;
+  P->kind = Pin->kind;
+  P->dtype = F3R_F16;
+  P->tol = Pin->tol;
+  P->acc = Pin->acc;
+  P->host = Pin->host;
+  F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
+  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, K);
+  S->P = P;
+  S->range_s = p.range_scale > 0 ? (float)p.range_scale
+                                 : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
+  const int64_t n = A.n;
+  S->red_grid = reduce_grid(n, ctx->num_cu);
+  auto dd = [](int64_t cnt) { return dev_alloc(sizeof(double) * std::max<int64_t>(cnt, 1)); };
+  S->V1 = dd(n * (p.m1 + 1));
+  S->Z1 = dd(n * p.m1);
+  S->H1 = dd((int64_t)(p.m1 + 1) * p.m1);
+  S->c1 = dd(p.m1);
+  S->s1 = dd(p.m1);
+  S->e1 = dd(p.m1 + 1);
+  S->y1 = dd(p.m1);
+  S->inv1 = dd(1);
+  S->status = dd(4);
+  S->b_int = dd(n);
+  S->x_int = dd(n);
+  auto mk = [&](InnerLevel<float> &L, int m) {
+    L.m = m;
+    L.Vbuf = dev_alloc(sizeof(float) * n * (m + 1));
+    L.Zbuf = dev_alloc(sizeof(float) * n * m);
+    L.H = dev_alloc(sizeof(float) * (m + 1) * m);
+    L.cs = dev_alloc(sizeof(float) * m);
+    L.sn = dev_alloc(sizeof(float) * m);
+    L.e = dev_alloc(sizeof(float) * (m + 1));
+    L.inv = dev_alloc(sizeof(float) * 4);
+    L.V.resize(m + 1);
+    L.Z.resize(m);
+    for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->as<float>() + (int64_t)k * n;
+    for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->as<float>() + (int64_t)k * n;
+  };
+  mk(S->L2, p.m2);
+  mk(S->L3, p.m3);
+  auto dh = [&](void) { return dev_alloc(sizeof(half_t) * n); };
+  S->hin = dh();
+  S->hout = dh();
+  S->hout2 = dh();
+  S->hr = dh();
+  S->tmp2 = dh();
+  S->tt = dh();
+  S->rich = dev_alloc(sizeof(RichDev));
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, ctx->stream, S->rich->as<RichDev>());
+  F3R_HIP(hipStreamSynchronize(ctx->stream));
+  return S.release();
+}
+
+void solver_solve(Solver *S, Vec *b, Vec *x, f3r_result *res) {
+  F3R_CHECK(b->dtype == F3R_F64 && x->dtype == F3R_F64 && b->n == S->n && x->n == S->n, F3R_ERR_ARG,
+            "solve: b and x must be fp64 vectors of the matrix size");
+  Context *c = S->ctx;
+  const int64_t n = S->n;
+  const dim3 g(S->grid()), bl(kBlock);
+  const bool permuted = (bool)S->sh->perm;
+  // nrm(b) before the timer, as the reference driver does (src/f3r-gpu.cpp:82)
+  const double nrm_b = std::sqrt(dot_raw(c, F3R_F64, b->ptr(), F3R_F64, b->ptr(), n));
+  if (permuted) {
+    hipLaunchKernelGGL(k_gather, g, bl, 0, c->stream, n, (const double *)b->ptr(), S->sh->perm->as<int32_t>(),
+                       S->b_int->as<double>());
+    hipLaunchKernelGGL(k_gather, g, bl, 0, c->stream, n, (const double *)x->ptr(), S->sh->perm->as<int32_t>(),
+                       S->x_int->as<double>());
+  } else {
+    F3R_HIP(hipMemcpyAsync(S->b_int->p, b->ptr(), 8 * n, hipMemcpyDeviceToDevice, c->stream));
+    F3R_HIP(hipMemcpyAsync(S->x_int->p, x->ptr(), 8 * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // reset the adaptive weights (src/f3r-gpu.cpp:277-281)
+  S->cnt = S->p.c;
+  S->calls = 0;
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, c->stream, S->rich->as<RichDev>());
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  S->hist.clear();
+  const bool record = (S->p.flags & 1) != 0;
+  auto t0 = std::chrono::steady_clock::now();
+  Solver::Res tot{false, 0, 0.0};
+  int cycles = 0;
+  for (int cyc = 0; cyc < S->p.restart; ++cyc) {
+    Solver::Res r = S->outer_cycle(nrm_b, record);
+    ++cycles;
+    tot.solved = tot.solved || r.solved;
+    tot.iters += r.iters;
+    tot.nrm = r.nrm;
+    if (r.solved) break;
+  }
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  if (permuted)
+    hipLaunchKernelGGL(k_scatter, g, bl, 0, c->stream, n, (const double *)S->x_int->as<double>(),
+                       S->sh->perm->as<int32_t>(), (double *)x->ptr());
+  else
+    F3R_HIP(hipMemcpyAsync(x->ptr(), S->x_int->p, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  res->is_solved = tot.solved ? 1 : 0;
+  res->res_iter = tot.iters;
+  res->res_nrm2 = tot.nrm;
+  res->nrm_b = nrm_b;
+  res->solve_s = std::chrono::duration<double>(t1 - t0).count();
+  res->precond_calls = S->calls;
+  res->outer_cycles = cycles;
+}
+
+}  // namespace f3r
+
+// ================================================================ C ABI ===
+using namespace f3r;
+
+struct f3r_solver_s {
+  Solver *s;
+};
+
+
+extern "C" {
+
+int f3r_params_default(f3r_params *p) {
+  if (!p) return F3R_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->m1 = 100;
+  p->m2 = 8;
+  p->m3 = 4;
+  p->m4 = 2;
+  p->c = 64;
+  p->restart = 20;
+  p->eps = 1e-8;
+  p->variant = 0;
+  p->range_scale = 0;
+  p->C = 0;
+  p->sigma = 0;
+  p->K = 0;
+  p->use_graph = 1;
+  p->flags = 1;
+  return F3R_OK;
+}
+
+int f3r_solver_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const f3r_params *params, f3r_solver *S) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && A && P && params && S, F3R_ERR_ARG, "solver_create: null argument");
+  auto m = mat_of(A);
+  F3R_CHECK(m->host, F3R_ERR_ARG, "solver_create: A must carry its CSR");
+  *S = nullptr;
+  Solver *s = solver_create(ctx_of(ctx), *m->host, prec_of(P).get(), *params);
+  *S = new f3r_solver_s{s};
+  F3R_API_END
+}
+
+int f3r_solve(f3r_solver S, f3r_vec b, f3r_vec x, f3r_result *res) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && b && x && res, F3R_ERR_ARG, "solve: null argument");
+  std::memset(res, 0, sizeof(*res));
+  solver_solve(S->s, vec_of(b), vec_of(x), res);
+  F3R_API_END
+}
+
+int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && len, F3R_ERR_ARG, "history: null argument");
+  const auto &h = S->s->hist;
+  *len = (int32_t)h.size();
+  if (hist)
+    for (int32_t i = 0; i < cap && i < (int32_t)h.size(); ++i) hist[i] = h[i];
+  F3R_API_END
+}
+
+int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && in && out && level >= 2 && level <= 4, F3R_ERR_ARG, "level_apply: bad argument");
+  Solver *s = S->s;
+  Vec *vi = vec_of(in), *vo = vec_of(out);
+  F3R_CHECK(vi->dtype == F3R_F64 && vo->dtype == F3R_F64 && vi->n == s->n && vo->n == s->n, F3R_ERR_ARG,
+            "level_apply: fp64 vectors of the matrix size");
+  F3R_CHECK(!s->sh->perm, F3R_ERR_UNSUPPORTED, "level_apply: sorted layouts");
+  const int64_t n = s->n;
+  const dim3 g(s->grid()), b(kBlock);
+  s->cnt = s->p.c;
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, s->st(), s->rich->as<RichDev>());
+  if (level == 2) {
+    const uint64_t keep = s->p.use_graph;
+    s->p.use_graph = 0;
+    s->level2((const double *)vi->ptr(), (double *)vo->ptr());
+    s->p.use_graph = (int32_t)keep;
+  } else {
+    float *a = s->L2.V[0], *r = s->L2.Z[0];
+    hipLaunchKernelGGL((k_cast2<double, float>), g, b, 0, s->st(), n, (const double *)vi->ptr(), a);
+    if (level == 3)
+      s->level3(a, r);
+    else
+      s->level4(a, r);
+    hipLaunchKernelGGL((k_cast2<float, double>), g, b, 0, s->st(), n, (const float *)r, (double *)vo->ptr());
+  }
+  F3R_HIP(hipGetLastError());
+  F3R_HIP(hipStreamSynchronize(s->st()));
+  F3R_API_END
+}
+
+int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms, double *bytes) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 3, F3R_ERR_ARG, "time_kernel: bad argument");
+  Solver *s = S->s;
+  const int64_t n = s->n, nnz = s->nnz;
+  SellArgs a = s->args();
+  half_t *x16 = s->hout->as<half_t>(), *y16 = s->tt->as<half_t>();
+  float *x32 = s->L2.V[0], *y32 = s->L2.V[1];
+  double *x64 = s->x_int->as<double>(), *y64 = s->V1->as<double>();
+  double by = 0;
+  auto launch = [&]() {
+    switch (which) {
+      case 0:  // fp16 values, fp16 x and y (the metric's fp16 SpMV)
+        launch_sell<half_t, half_t, float>(s->sh->K, a, s->A16->as<half_t>(), x16,
+                                           EpiStore<half_t, float>{y16, nullptr}, s->st());
+        by = nnz * 6.0 + n * 4.0;
+        break;
+      case 1:  // fp16 values, fp32 x and y (level 3)
+        launch_sell<half_t, float, float>(s->sh->K, a, s->A16->as<half_t>(), x32,
+                                          EpiStore<float, float>{y32, nullptr}, s->st());
+        by = nnz * 6.0 + n * 8.0;
+        break;
+      case 2:  // fp32 (level 2)
+        launch_sell<float, float, float>(s->sh->K, a, s->A32->as<float>(), x32,
+                                         EpiStore<float, float>{y32, nullptr}, s->st());
+        by = nnz * 8.0 + n * 8.0;
+        break;
+      case 3:  // fp64 (outer level)
+        launch_sell<double, double, double>(s->sh->K, a, s->A64->as<double>(), x64,
+                                            EpiStore<double, double>{y64, nullptr}, s->st());
+        by = nnz * 12.0 + n * 16.0;
+        break;
+    }
+  };
+  hipEvent_t e0, e1;
+  F3R_HIP(hipEventCreate(&e0));
+  F3R_HIP(hipEventCreate(&e1));
+  launch();
+  F3R_HIP(hipEventRecord(e0, s->st()));
+  for (int i = 0; i < reps; ++i) launch();
+  F3R_HIP(hipEventRecord(e1, s->st()));
+  F3R_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  F3R_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = (double)t / reps;
+  if (bytes) *bytes = by;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  F3R_API_END
+}
+
+int f3r_solver_destroy(f3r_solver S) {
+  F3R_API_BEGIN
+  if (S) {
+    (void)hipStreamSynchronize(S->s->st());
+    delete S->s;
+    delete S;
+  }
+  F3R_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+/*
+ * f3r_amd.h -- C ABI of the MI355X-native F3R nested mixed-precision solver.
+ *
+ * Plain pointers, sizes and opaque handles only.  Every call returns an int
+ * status (F3R_OK = 0, negative on error; see f3r_strerror).  Non-convergence
+ * is not an error: it is reported in f3r_result.is_solved.
+ *
+ * Each entry point names the reference interface it replaces
+ * (suzuki-hpc/F3R, library/umineko-{core,sparse}, work/src).
+ */
+#ifndef F3R_AMD_H
+#define F3R_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- status ---- */
+enum {
+  F3R_OK = 0,
+  F3R_ERR_ARG = -1,        /* invalid argument or shape mismatch */
+  F3R_ERR_HIP = -2,        /* a HIP runtime call failed */
+  F3R_ERR_ALLOC = -3,      /* device or host allocation failed */
+  F3R_ERR_UNSUPPORTED = -4,/* dtype / format combination not built */
+  F3R_ERR_NODEVICE = -5,   /* no usable GPU */
+  F3R_ERR_STATE = -6,      /* object used in a state that does not allow it */
+  F3R_ERR_RANGE = -7       /* size beyond what the format can index */
+};
+/* replaces: nothing (the reference exits on errors, sparse/matrix/io.hpp:123) */
+const char *f3r_strerror(int status);
+const char *f3r_last_error(void); /* detail of the last error on this thread */
+
+/* element types: the precisions of kmm::vector<T, device> */
+enum { F3R_F64 = 0, F3R_F32 = 1, F3R_F16 = 2, F3R_I32 = 3 };
+
+typedef struct f3r_ctx_s *f3r_ctx;
+typedef struct f3r_vec_s *f3r_vec;
+typedef struct f3r_mat_s *f3r_mat;
+typedef struct f3r_prec_s *f3r_prec;
+typedef struct f3r_solver_s *f3r_solver;
+
+/* ------------------------------------------------------------ context ---- */
+/* One HIP stream per context; calls on one context are serialised.
+ * replaces: the implicit default CUDA stream of exec<device>
+ * (core/loop.hpp:142-181) and the global scratch pool (core/tensor.hpp:492-507) */
+int f3r_ctx_create(int device, f3r_ctx *ctx);
+int f3r_ctx_destroy(f3r_ctx ctx);
+int f3r_ctx_sync(f3r_ctx ctx);
+int f3r_device_count(int *count);
+
+/* ------------------------------------------------------------ vectors ---- */
+/* replaces: kmm::vector<T, device> (core/tensor.hpp:204-316) and
+ * memory<device>::{alloc, copy} (core/memory.hpp:38-54) */
+int f3r_vec_create(f3r_ctx ctx, int dtype, int64_t n, f3r_vec *v);
+/* non-owning view of n elements starting at `offset` (vector(T*, shape),
+ * core/tensor.hpp:229-230; matrix column views, tensor.hpp:347-349) */
+int f3r_vec_view(f3r_vec base, int64_t offset, int64_t n, f3r_vec *v);
+int f3r_vec_destroy(f3r_vec v);
+int f3r_vec_upload(f3r_vec v, const void *host);   /* host array of v's dtype */
+int f3r_vec_download(f3r_vec v, void *host);
+int f3r_vec_info(f3r_vec v, int *dtype, int64_t *n, void **device_ptr);
+/* vector::fill (tensor.hpp:247) */
+int f3r_vec_fill(f3r_vec v, double value);
+/* precision-converting copy dst = T(src), round to nearest even
+ * (vector::copy, tensor.hpp:272-277) */
+int f3r_vec_copy(f3r_vec src, f3r_vec dst);
+/* y += alpha * x  and  x *= alpha (vector expressions, tensor.hpp:302-311) */
+int f3r_vec_axpy(double alpha, f3r_vec x, f3r_vec y);
+int f3r_vec_scale(f3r_vec x, double alpha);
+/* dot / nrm of DOT / NRM (tensor.hpp:509-571): accumulated in the wider of
+ * the two element types, fp16 data in fp32; a single-pass grid reduction. */
+int f3r_vec_dot(f3r_vec a, f3r_vec b, double *out);
+int f3r_vec_nrm2(f3r_vec a, double *out);
+
+/* ------------------------------------------------------- host set-up ---- */
+/* CPU routines; no GPU is needed.  CSR arrays are int32 row pointers and
+ * column indices (columns sorted within a row) and fp64 values.           */
+/* algorithm::scaling (sparse/matrix/algorithm.hpp:45-59), in place */
+int f3r_csr_scaling(int32_t n, const int32_t *rptr, const int32_t *col, double *val);
+/* SELL-C-sigma slot count and layout (sparse/matrix/sell.hpp:247-343):
+ * rows sorted by descending length inside windows of `sigma` rows, chunks of
+ * C rows, slots interleaved by K; see f3r_mat_to_sell. */
+int f3r_sell_nslots(int32_t n, const int32_t *rptr, int32_t C, int32_t sigma, int32_t K,
+                    int64_t *nslots);
+/* Synthetic inputs (write-only outputs sized by the *_nnz query):
+ * 27-point stencil on nx*ny*nz, diagonal 26, neighbours -1 + beta*sign(dx)
+ * (matrix/hpcg.cpp:13-57 at beta = 0). */
+int f3r_gen_stencil27_nnz(int32_t nx, int32_t ny, int32_t nz, int64_t *nnz);
+int f3r_gen_stencil27(int32_t nx, int32_t ny, int32_t nz, double beta, int32_t *rptr,
+                      int32_t *col, double *val);
+/* 7-point variable-coefficient convection-diffusion: lognormal diffusion
+ * exp(sigma * g) of a seeded unit-variance Gaussian field g with correlation
+ * length `corr` cells, harmonic face averages, first-order upwind convection
+ * of cell Peclet number `peclet` along +x (0 = pure diffusion). */
+int f3r_gen_convdiff7_nnz(int32_t nx, int32_t ny, int32_t nz, int64_t *nnz);
+int f3r_gen_convdiff7(int32_t nx, int32_t ny, int32_t nz, double sigma, double corr,
+                      double peclet, uint64_t seed, int32_t *rptr, int32_t *col,
+                      double *val);
+/* 7- or 27-point stencil with unit off-diagonals and diagonal
+ * (1 + margin) * sum|off|, for the batched systems */
+int f3r_gen_dominant_nnz(int32_t npts, int32_t nx, int32_t ny, int32_t nz, int64_t *nnz);
+int f3r_gen_dominant(int32_t npts, int32_t nx, int32_t ny, int32_t nz, double margin,
+                     int32_t *rptr, int32_t *col, double *val);
+/* seeded uniform [lo, hi) vector (splitmix64 -> 53-bit mantissa) */
+int f3r_gen_uniform(int64_t n, double lo, double hi, uint64_t seed, double *out);
+
+/* ----------------------------------------------------------- matrices ---- */
+/* CSR<double, host> copied to the device (csr.hpp:12-50); the host copy is
+ * kept for set-up (SELL build, SDAINV, ILU(0)).  ctx may be NULL: a
+ * host-only matrix, usable for preconditioner set-up but not for SpMV. */
+int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz,
+                       const int32_t *rptr, const int32_t *col, const double *val,
+                       f3r_mat *A);
+/* SELL<w, T, device>(CSR) (sell.hpp:35-97) and SELLsig (sell.hpp:99-166):
+ * C rows per chunk (multiple of 16), sigma = sort window (1 = unsorted),
+ * K = slots interleaved per row (1, 2 or 4; K = 1 is the reference layout).
+ * Rows of the result are in sorted order; y of f3r_spmv is written back to
+ * the original row order. */
+int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
+                    int32_t K, f3r_mat *sell);
+/* host-only SELL-C-sigma layout of a CSR matrix (csr_to_sell /
+ * csr_to_sell_sigma, sell.hpp:247-343): cptr[nchunks+1] element offsets,
+ * cwidth[nchunks] slot groups, sidx / sval [nslots], perm[n] (sorted row ->
+ * original row).  Sizes from f3r_sell_nslots. */
+int f3r_sell_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val,
+                    int32_t C, int32_t sigma, int32_t K, int32_t *cptr, int32_t *cwidth,
+                    int32_t *sidx, double *sval, int32_t *perm);
+/* cross-precision SELL sharing the index arrays (sell.hpp:51-53) */
+int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out);
+int f3r_mat_destroy(f3r_mat A);
+int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *dtype,
+                 int *format);
+/* y = A x  (SELL::operate / CSR::operate, sell.hpp:55-71, csr.hpp:32-40).
+ * Accumulates in fp64 for fp64 data, else in fp32 (fp16 x fp16 too). */
+int f3r_spmv(f3r_mat A, f3r_vec x, f3r_vec y);
+/* r = b - A x  (SELL::compute_residual, sell.hpp:72-89; csr.hpp:41-50) */
+int f3r_residual(f3r_mat A, f3r_vec b, f3r_vec x, f3r_vec r);
+/* mean device time of `reps` SpMVs on the context's stream (HIP events) */
+int f3r_spmv_time(f3r_mat A, f3r_vec x, f3r_vec y, int32_t reps, double *ms);
+
+/* ----------------------------------------------------- preconditioners ---- */
+/* SDAINV<T, device>(A, {tol, acc}) (ainv.hpp:27-65,129-147): M = Z D^-1 W^T,
+ * applied as two SpMVs (W^T then Z) in `dtype` with fp32 sums.  The factor
+ * set-up runs on the host (ainv.hpp:268-329).  With ctx = NULL only the host
+ * factors are built (f3r_prec_factor_get works, f3r_prec_apply does not). */
+int f3r_sdainv_create(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtype,
+                      f3r_prec *P);
+/* ILU(0) on `blocks` diagonal blocks (ilu.hpp:128-164 + algorithm::split),
+ * applied by level-scheduled forward / backward solves (the block-Jacobi
+ * solves of trsv/bj.hpp:58-98, same per-row order). */
+int f3r_ilu0_create(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype,
+                    f3r_prec *P);
+/* out = M in (IZW::operate ainv.hpp:110-114; ILU::operate ilu.hpp:57-61) */
+int f3r_prec_apply(f3r_prec P, f3r_vec in, f3r_vec out);
+/* factor sizes and a host copy of the factors (for tests):
+ * SDAINV: which 0 = Z, 1 = W^T.  ILU(0): 0 = L (unit, 1 stored last), 1 = U. */
+int f3r_prec_factor_nnz(f3r_prec P, int which, int64_t *nnz);
+int f3r_prec_factor_get(f3r_prec P, int which, int32_t *rptr, int32_t *col, double *val);
+int f3r_prec_destroy(f3r_prec P);
+
+/* --------------------------------------------------------- the solver ---- */
+typedef struct {
+  int32_t m1;          /* outer FGMRES(m1), fp64 (work/src/f3r-gpu.cpp:264: 100) */
+  int32_t m2, m3, m4;  /* level-2 / level-3 FGMRES steps, Richardson steps */
+  int32_t c;           /* probe period of the adaptive weights */
+  int32_t restart;     /* outer cycles (Restarted, restart.hpp:8-37) */
+  double eps;          /* stop when |e_j| / ||b|| < eps */
+  int32_t variant;     /* 0 HALF (fp16 innermost), 1 FLOAT, 2 DOUBLE */
+  double range_scale;  /* power-of-two factor on the fp16 level's input; 0 = auto */
+  int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = autotuned) */
+  int32_t use_graph;   /* capture the inner levels in HIP graphs */
+  int32_t flags;       /* bit 0: record the outer residual history */
+} f3r_params;
+
+typedef struct {
+  int32_t is_solved;   /* solve_res_t (sparse/models.hpp:191-200) */
+  int32_t res_iter;    /* outer Arnoldi steps summed over restarts */
+  double res_nrm2;     /* |e| of the last outer step */
+  double nrm_b;
+  double solve_s;      /* wall time of the solve only (src/f3r-gpu.cpp:87-89) */
+  int64_t precond_calls;
+  int32_t outer_cycles;
+} f3r_result;
+
+/* fills the reference defaults: m = (100; 8, 4, 2), c = 64, restart 20,
+ * eps 1e-8 (work/suite-gpu.py:49, work/src/f3r-gpu.cpp:18,264-265) */
+int f3r_params_default(f3r_params *p);
+/* Restarted<double>(A, FGMRES<double>(A, FGMRES<float>(A32, FGMRES<float>(
+ * A16, Richardson(M))))) of work/src/f3r-gpu.cpp:210-266.  `A` is the CSR
+ * matrix as solved (the caller applies f3r_csr_scaling first, as the
+ * reference driver does).  P is the innermost preconditioner (SDAINV or
+ * ILU(0)), built in the precision of the variant's innermost level. */
+int f3r_solver_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const f3r_params *params,
+                      f3r_solver *S);
+/* solve A x = b from the given x (Restarted::solve, restart.hpp:26-36);
+ * resets the adaptive weights first, as the reference driver does per
+ * repetition (src/f3r-gpu.cpp:276-281). b and x are fp64 device vectors. */
+int f3r_solve(f3r_solver S, f3r_vec b, f3r_vec x, f3r_result *res);
+/* |e_j| / ||b|| of each outer step of the last solve (flags bit 0) */
+int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len);
+/* one application of level 2, 3 or 4 as an operator on fp64 vectors, for the
+ * per-level error study */
+int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out);
+/* mean device time per launch of the solver's fp16 residual SpMV, measured
+ * with HIP events on the solver's stream over `reps` launches */
+int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms,
+                           double *bytes);
+int f3r_solver_destroy(f3r_solver S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,167 @@
+"""GPU parity of the nested solve through the C ABI, against the oracle in
+this build's arithmetic (fp32 sums of fp16 data, fused multiply-adds):
+  - one innermost (Richardson) application: bit-exact, for diagonal SDAINV,
+    general SDAINV and ILU(0);
+  - one level-3 application: within fp32 rounding of the reductions;
+  - full solves: true fp64 relative residual <= 1e-8 and outer iteration
+    count within +-10% of the oracle; graph replay identical to eager launch.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import f3r_amd as F
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+def _problem(kind, m):
+    import f3r_amd as F
+    if kind == "27pt":
+        A = F.CSR.stencil27(m, beta=0.5)
+    elif kind == "27sym":
+        A = F.CSR.stencil27(m)
+    elif kind == "lognormal":
+        A = F.CSR.convdiff7(m, sigma=1.5, corr=3.0, peclet=0.0, seed=7)
+    else:
+        A = F.CSR.convdiff7(m, sigma=1.5, corr=4.0, peclet=10.0, seed=3)
+    F.algorithm.scaling(A)
+    return A
+
+
+def _scale(n):
+    return float(2.0 ** np.floor(np.log2(np.sqrt(n) / 4.0))) if n >= 16 else 1.0
+
+
+def _prec(F, A, ctx, kind):
+    if kind == "ilu0":
+        return F.ILU0(A, blocks=8, acc=1.0, dtype="f16", ctx=ctx)
+    return F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+
+
+@pytest.mark.parametrize("kind,prec", [("27pt", "sdainv"), ("lognormal", "sdainv"), ("convdiff", "ilu0"),
+                                       ("27pt", "ilu0")])
+def test_level4_bitexact(ctx, kind, prec):
+    import f3r_amd as F
+    A = _problem(kind, 14)
+    P = _prec(F, A, ctx, prec)
+    s = _scale(A.n)
+    S = F.F3R(ctx, A, P, range_scale=s, use_graph=False)
+    v = np.random.default_rng(1).uniform(-1, 1, A.n)
+    v /= np.linalg.norm(v)
+    vin = F.vector.from_numpy(ctx, v)
+    vout = F.vector(ctx, "f64", A.n)
+    print(f"level4 {kind} {prec}", flush=True)
+    S.level_apply(4, vin, vout)
+    ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 4, v, variant=0, ref_arith=0, range_scale=s,
+                        precond=1 if prec == "ilu0" else 0, bj_blocks=8)
+    assert np.array_equal(vout.download(), ref)
+
+
+@pytest.mark.parametrize("kind", ["27pt", "lognormal"])
+def test_level3_close(ctx, kind):
+    import f3r_amd as F
+    A = _problem(kind, 16)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    s = _scale(A.n)
+    S = F.F3R(ctx, A, P, range_scale=s, use_graph=False)
+    v = np.random.default_rng(2).uniform(-1, 1, A.n)
+    v /= np.linalg.norm(v)
+    vin = F.vector.from_numpy(ctx, v)
+    vout = F.vector(ctx, "f64", A.n)
+    S.level_apply(3, vin, vout)
+    ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 3, v, variant=0, ref_arith=0, range_scale=s)
+    err = np.linalg.norm(vout.download() - ref) / np.linalg.norm(ref)
+    assert err < 1e-3, err
+
+
+def _true_res(A, x, b):
+    r = b - O.spmv_ld(A.n, A.rptr, A.idx, A.val, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kind,m,prec", [("27pt", 24, "sdainv"), ("27sym", 20, "sdainv"),
+                                         ("lognormal", 16, "sdainv"), ("convdiff", 16, "sdainv"),
+                                         ("convdiff", 16, "ilu0")])
+def test_solve_matches_oracle(ctx, kind, m, prec):
+    import f3r_amd as F
+    A = _problem(kind, m)
+    P = _prec(F, A, ctx, prec)
+    s = _scale(A.n)
+    b = F.uniform(A.n, -1.0, 1.0, seed=5)
+    S = F.F3R(ctx, A, P, range_scale=s)
+    bv = F.vector.from_numpy(ctx, b)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    print(f"solve {kind} {m} {prec}", flush=True)
+    res = S.solve(bv, xv)
+    x = xv.download()
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=s,
+                           precond=1 if prec == "ilu0" else 0, bj_blocks=8)
+    assert res.is_solved and ro.is_solved
+    assert _true_res(A, x, b) <= 1e-8
+    assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter))), (res.res_iter, ro.res_iter)
+    h = S.history()
+    assert len(h) >= res.res_iter and h[-1] < 1e-8
+
+
+def test_graph_replay_identical_to_eager(ctx):
+    import f3r_amd as F
+    A = _problem("27pt", 20)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    b = F.uniform(A.n, -1.0, 1.0, seed=9)
+    bv = F.vector.from_numpy(ctx, b)
+    outs = []
+    for g in (False, True, True):
+        S = F.F3R(ctx, A, P, use_graph=g, c=8)   # c = 8: probes inside the captured bodies
+        xv = F.vector(ctx, "f64", A.n).fill(0.0)
+        r = S.solve(bv, xv)
+        outs.append((r.res_iter, xv.download()))
+        # a second solve with the same object restarts the weights: same answer
+        xv.fill(0.0)
+        r2 = S.solve(bv, xv)
+        assert r2.res_iter == r.res_iter and np.array_equal(xv.download(), outs[-1][1])
+    assert outs[0][0] == outs[1][0] == outs[2][0]
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[1][1], outs[2][1])
+
+
+def test_solve_edge_cases(ctx):
+    import f3r_amd as F
+    A = _problem("27pt", 9)    # n = 729: not a multiple of the chunk height
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    b = F.uniform(A.n, -1.0, 1.0, seed=1)
+    bv = F.vector.from_numpy(ctx, b)
+    S = F.F3R(ctx, A, P, C=64, sigma=64, K=4)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    r = S.solve(bv, xv)
+    x = xv.download()
+    assert r.is_solved and _true_res(A, x, b) <= 1e-8
+    # starting from the solution: solved before the first Arnoldi step
+    r0 = S.solve(bv, xv)
+    assert r0.is_solved and r0.res_iter == 0
+    # outer budget exhausted: reported, not an error
+    S1 = F.F3R(ctx, A, P, m=(1, 1, 1, 1), restart=1)
+    xv.fill(0.0)
+    r1 = S1.solve(bv, xv)
+    assert not r1.is_solved and r1.res_iter == 1
+
+
+def test_c2_shape_64cube(ctx):
+    # the 27-point nonsymmetric problem of the benchmark, at 64^3
+    import f3r_amd as F
+    A = _problem("27pt", 64)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    b = F.uniform(A.n, -1.0, 1.0, seed=0)
+    S = F.F3R(ctx, A, P)
+    bv = F.vector.from_numpy(ctx, b)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    r = S.solve(bv, xv)
+    assert r.is_solved and _true_res(A, xv.download(), b) <= 1e-8
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=_scale(A.n))
+    assert abs(r.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter)))

@@ -1,0 +1,168 @@
+"""GPU parity of the building blocks through the C ABI: SELL-C-sigma / CSR
+SpMV in every precision combination the solve uses (bit-exact against the
+oracle, which fuses multiply-adds in the same order), BLAS-1 reductions, and
+the SDAINV / ILU(0) preconditioner applies."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import problems as PB
+
+pytestmark = pytest.mark.gpu
+
+COMBOS = [  # (value, x, y) precisions; sums in fp64 for fp64 data, else fp32
+    ("f64", "f64", "f64", 0, 0, 0, 0),
+    ("f32", "f32", "f32", 1, 1, 1, 1),
+    ("f16", "f32", "f32", 2, 1, 1, 1),
+    ("f16", "f16", "f16", 2, 2, 2, 1),
+    ("f16", "f16", "f32", 2, 2, 1, 1),
+]
+LAYOUTS = [(32, 1, 1), (64, 1, 4), (64, 256, 2), (128, 64, 4), (16, 1, 4)]
+NP = {"f64": np.float64, "f32": np.float32, "f16": np.float16}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import f3r_amd as F
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+def _mats():
+    import f3r_amd as F
+    A = F.CSR.stencil27(10, 9, 11, beta=0.5)
+    F.algorithm.scaling(A)
+    yield "27pt", A
+    n, rp, ix, v = PB.random_sparse(1000, 40, 5)   # ragged rows, 1..40 entries
+    yield "ragged", F.CSR(n, rp, ix, v)
+    B = F.CSR.convdiff7(13, sigma=1.5, corr=3.0, peclet=10.0, seed=1)
+    F.algorithm.scaling(B)
+    yield "convdiff", B
+
+
+@pytest.mark.parametrize("C,sigma,K", LAYOUTS)
+def test_sell_spmv_bitexact(ctx, C, sigma, K):
+    import f3r_amd as F
+    for name, A in _mats():
+        x = np.random.default_rng(1).uniform(-1, 1, A.n)
+        b = np.random.default_rng(2).uniform(-1, 1, A.n)
+        for vd, xd, yd, vp, xp, yp, ap in COMBOS:
+            print(f"spmv {name} {vd}/{xd}/{yd} C={C} sigma={sigma} K={K}", flush=True)
+            S = F.SELL(ctx, A, vd, C=C, sigma=sigma, K=K)
+            xv = F.vector.from_numpy(ctx, x.astype(NP[xd]))
+            yv = F.vector(ctx, yd, A.n)
+            S.operate(xv, yv)
+            got = yv.download()
+            ref = O.spmv(A.n, A.rptr, A.idx, A.val, x.astype(NP[xd]), vp, xp, yp, ap)
+            assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (name, vd, xd, yd)
+            if yd == xd:
+                bv = F.vector.from_numpy(ctx, b.astype(NP[yd]))
+                S.compute_residual(bv, xv, yv)
+                ref = O.spmv(A.n, A.rptr, A.idx, A.val, x.astype(NP[xd]), vp, xp, yp, ap,
+                             b=b.astype(NP[yd]))
+                assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, "resid")
+
+
+def test_cross_precision_cast_shares_layout(ctx):
+    import f3r_amd as F
+    A = F.CSR.stencil27(12, beta=0.5)
+    F.algorithm.scaling(A)
+    S64 = F.SELL(ctx, A, "f64", C=64, sigma=128, K=4)
+    S16 = S64.cast("f16")
+    assert S16.nslots() == S64.nslots()
+    x = np.random.default_rng(0).uniform(-1, 1, A.n).astype(np.float16)
+    xv = F.vector.from_numpy(ctx, x)
+    yv = F.vector(ctx, "f16", A.n)
+    S16.operate(xv, yv)
+    ref = O.spmv(A.n, A.rptr, A.idx, A.val, x, 2, 2, 2, 1)
+    assert np.array_equal(yv.download().view(np.uint16), ref.view(np.uint16))
+
+
+def test_fp64_spmv_accuracy_64cube(ctx):
+    # fp64 SpMV within 1e-14 relative of a long-double CSR sum, at 64^3
+    import f3r_amd as F
+    A = F.CSR.stencil27(64, beta=0.5)
+    F.algorithm.scaling(A)
+    x = np.random.default_rng(3).uniform(-1, 1, A.n)
+    S = F.SELL(ctx, A, "f64", C=64, sigma=1, K=4)
+    xv = F.vector.from_numpy(ctx, x)
+    yv = F.vector(ctx, "f64", A.n)
+    S.operate(xv, yv)
+    ref = O.spmv_ld(A.n, A.rptr, A.idx, A.val, x)
+    got = yv.download()
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # CSR format on the device too
+    Av = F.CSR(A.n, A.rptr, A.idx, A.val)
+    from f3r_amd._capi import lib, check
+    y2 = F.vector(ctx, "f64", A.n)
+    check(lib.f3r_spmv(Av.handle(ctx), xv.handle, y2.handle))
+    assert np.max(np.abs(y2.download() - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_blas1(ctx):
+    import f3r_amd as F
+    rng = np.random.default_rng(7)
+    for n in (1, 255, 256, 1000, 100003, 3_000_001):
+        a = rng.uniform(-1, 1, n)
+        b = rng.uniform(-1, 1, n)
+        av = F.vector.from_numpy(ctx, a)
+        bv = F.vector.from_numpy(ctx, b)
+        d = F.dot(av, bv)
+        assert abs(d - a @ b) <= 1e-13 * n
+        assert abs(F.nrm(av) - np.linalg.norm(a)) <= 1e-13 * np.sqrt(n)
+        a32 = F.vector(ctx, "f32", n).copy(av)
+        assert np.array_equal(a32.download(), a.astype(np.float32))
+        a16 = F.vector(ctx, "f16", n).copy(av)
+        assert np.array_equal(a16.download().view(np.uint16), a.astype(np.float16).view(np.uint16))
+        d16 = F.dot(a16, a16)
+        r = a.astype(np.float16).astype(np.float64)
+        assert abs(d16 - r @ r) <= 1e-5 * (r @ r) + 1e-6
+        bv.axpy(0.5, av)
+        np.testing.assert_allclose(bv.download(), b + 0.5 * a, rtol=0, atol=1e-15)
+        av.scale(3.0)
+        np.testing.assert_allclose(av.download(), 3.0 * a, rtol=1e-16)
+        av.fill(2.5)
+        assert np.all(av.download() == 2.5)
+    # views
+    v = F.vector.from_numpy(ctx, np.arange(10.0))
+    w = v.view(3, 4)
+    assert np.array_equal(w.download(), np.arange(3.0, 7.0))
+
+
+def test_sdainv_apply_bitexact(ctx):
+    import f3r_amd as F
+    A = F.CSR.convdiff7(12, sigma=2.0, corr=2.0, peclet=0.0, seed=3)
+    F.algorithm.scaling(A)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    (zr, zi, zv), (wr, wi, wv) = P.factors()
+    x = np.random.default_rng(4).uniform(-1, 1, A.n).astype(np.float16)
+    xv = F.vector.from_numpy(ctx, x)
+    yv = F.vector(ctx, "f16", A.n)
+    P.operate(xv, yv)
+    t = O.spmv(A.n, wr, wi, wv, x, 2, 2, 2, 1)
+    ref = O.spmv(A.n, zr, zi, zv, t, 2, 2, 2, 1)
+    assert np.array_equal(yv.download().view(np.uint16), ref.view(np.uint16))
+
+
+def test_ilu0_apply_matches_oracle(ctx):
+    import f3r_amd as F
+    A = F.CSR.stencil27(12, beta=0.5)
+    F.algorithm.scaling(A)
+    P = F.ILU0(A, blocks=4, acc=1.0, dtype="f64", ctx=ctx)
+    x = np.random.default_rng(5).uniform(-1, 1, A.n)
+    xv = F.vector.from_numpy(ctx, x)
+    yv = F.vector(ctx, "f64", A.n)
+    P.operate(xv, yv)
+    h = O.lib().orc_ilu0(A.n, O.P(A.rptr), O.P(A.idx), O.P(A.val), 4, 1.0)
+    out, tmp = np.zeros(A.n), np.zeros(A.n)
+    O.lib().orc_bj_solve(h, O.P(x), O.P(out), O.P(tmp))
+    O.lib().orc_factor_free(h)
+    assert np.array_equal(yv.download(), out)
+    # fp16 factors: close to the fp64 apply
+    P16 = F.ILU0(A, blocks=4, acc=1.0, dtype="f16", ctx=ctx)
+    x16 = F.vector.from_numpy(ctx, x.astype(np.float16))
+    y16 = F.vector(ctx, "f16", A.n)
+    P16.operate(x16, y16)
+    err = np.linalg.norm(y16.download().astype(np.float64) - out) / np.linalg.norm(out)
+    assert err < 5e-3
