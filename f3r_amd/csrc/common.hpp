@@ -61,14 +61,14 @@ inline DevPtr dev_alloc(size_t bytes) { return std::make_shared<DevBuf>(bytes); 
 // arrival counters.  Counters are reset by the last block of each launch, so
 // no memset is ever recorded (also under graph capture).
 struct Reducer {
-  static constexpr int kMaxBlocks = 1024;
-  static constexpr int kMaxCols = 128;
-  DevPtr partial;   // kMaxBlocks * kMaxCols doubles
+  static constexpr int64_t kPartialDoubles = 1 << 20;  // 8 MiB of partial sums
+  DevPtr partial;
   DevPtr counters;  // 64 unsigned ints
   DevPtr scalars;   // 64 doubles of results
 };
 
 struct Context {
+  ~Context();
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cu = 256;
@@ -77,7 +77,7 @@ struct Context {
 };
 
 struct Vec {
-  Context *ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   int dtype = F3R_F64;
   int64_t n = 0;
   DevPtr buf;         // owner (shared with views)
@@ -110,7 +110,7 @@ struct SellShared {  // index data shared by all precisions of one matrix
 };
 
 struct Mat {
-  Context *ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   int format = 0;  // 0 = CSR (fp64), 1 = SELL
   int dtype = F3R_F64;
   // CSR

@@ -118,19 +118,19 @@ static int grid_for(int64_t n, int cu) {
 }
 
 // ------------------------------------------------------------- context ---
-Context *ctx_create(int device) {
+std::shared_ptr<Context> ctx_create(int device) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
     throw Error{F3R_ERR_NODEVICE, "no HIP device"};
   F3R_CHECK(device >= 0 && device < count, F3R_ERR_ARG, "device index out of range");
   F3R_HIP(hipSetDevice(device));
-  auto *c = new Context();
+  auto c = std::make_shared<Context>();
   c->device = device;
   hipDeviceProp_t prop;
   F3R_HIP(hipGetDeviceProperties(&prop, device));
   c->num_cu = prop.multiProcessorCount;
   F3R_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->red.partial = dev_alloc(sizeof(double) * Reducer::kMaxBlocks * Reducer::kMaxCols);
+  c->red.partial = dev_alloc(sizeof(double) * Reducer::kPartialDoubles);
   c->red.counters = dev_alloc(sizeof(unsigned) * 64);
   c->red.scalars = dev_alloc(sizeof(double) * 64);
   F3R_HIP(hipMemsetAsync(c->red.counters->p, 0, sizeof(unsigned) * 64, c->stream));
@@ -139,13 +139,11 @@ Context *ctx_create(int device) {
   return c;
 }
 
-void ctx_destroy(Context *c) {
-  if (!c) return;
-  (void)hipStreamSynchronize(c->stream);
-  c->red = Reducer();
-  if (c->pinned) (void)hipHostFree(c->pinned);
-  (void)hipStreamDestroy(c->stream);
-  delete c;
+Context::~Context() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  red = Reducer();
+  if (pinned) (void)hipHostFree(pinned);
+  if (stream) (void)hipStreamDestroy(stream);
 }
 
 double read_scalar(Context *c, const double *dev) {
@@ -253,7 +251,7 @@ double dot_raw(Context *c, int da, const void *pa, int db, const void *pb, int64
 
 double vec_dot(Vec *a, Vec *b) {
   F3R_CHECK(a->n == b->n, F3R_ERR_ARG, "dot: length mismatch");
-  return dot_raw(a->ctx, a->dtype, a->ptr(), b->dtype, b->ptr(), a->n);
+  return dot_raw(a->ctx.get(), a->dtype, a->ptr(), b->dtype, b->ptr(), a->n);
 }
 
 // ------------------------------------------------------------ matrices ---
@@ -450,7 +448,7 @@ HostCSR permute_csr(const HostCSR &M, const std::vector<int32_t> &perm) {
 }
 
 void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K) {
-  Context *c = P->ctx;
+  Context *c = P->ctx.get();
   const int32_t n = P->host.f[0].n;
   P->n = n;
   P->tmp = dev_alloc(dtype_size(P->dtype) * std::max(1, n));
@@ -472,7 +470,7 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
       const HostCSR &M = w == 0 ? W : Z;
       HostSell hs = build_sell(M, C, 1, K, perm, perm != nullptr);
       auto m = std::make_shared<Mat>();
-      m->ctx = c;
+      m->ctx = P->ctx;
       m->format = 1;
       m->dtype = P->dtype;
       m->sh = upload_sell_shared(c, hs, perm != nullptr);
@@ -570,13 +568,13 @@ HostCSR host_csr_from(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *
 using namespace f3r;
 
 
-struct f3r_ctx_s { Context *c; };
+struct f3r_ctx_s { std::shared_ptr<Context> c; };
 struct f3r_vec_s { Vec v; };
 struct f3r_mat_s { std::shared_ptr<Mat> m; };
 struct f3r_prec_s { std::shared_ptr<Prec> p; };
 
 namespace f3r {
-Context *ctx_of(f3r_ctx c) { return c ? c->c : nullptr; }
+std::shared_ptr<Context> ctx_of(f3r_ctx c) { return c ? c->c : nullptr; }
 Vec *vec_of(f3r_vec v) { return v ? &v->v : nullptr; }
 std::shared_ptr<Mat> mat_of(f3r_mat m) { return m ? m->m : nullptr; }
 std::shared_ptr<Prec> prec_of(f3r_prec p) { return p ? p->p : nullptr; }
@@ -612,15 +610,14 @@ int f3r_ctx_create(int device, f3r_ctx *ctx) {
   F3R_API_BEGIN
   F3R_CHECK(ctx, F3R_ERR_ARG, "null ctx pointer");
   *ctx = nullptr;
-  Context *c = ctx_create(device);
-  *ctx = new f3r_ctx_s{c};
+  *ctx = new f3r_ctx_s{ctx_create(device)};
   F3R_API_END
 }
 int f3r_ctx_destroy(f3r_ctx ctx) {
   F3R_API_BEGIN
   if (!ctx) return F3R_OK;
-  ctx_destroy(ctx->c);
-  delete ctx;
+  (void)hipStreamSynchronize(ctx->c->stream);
+  delete ctx;  // objects created on the context keep it alive
   F3R_API_END
 }
 int f3r_ctx_sync(f3r_ctx ctx) {
@@ -736,9 +733,9 @@ int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz, c
   m->dtype = F3R_F64;
   m->host = std::make_shared<HostCSR>(host_csr_from(nrows, ncols, nnz, rptr, col, val));
   if (ctx) {
-    m->d_rptr = upload_i32(ctx->c, m->host->rptr);
-    m->d_idx = upload_i32(ctx->c, m->host->idx);
-    m->d_val = upload_values(ctx->c, m->host->val, F3R_F64);
+    m->d_rptr = upload_i32(ctx->c.get(), m->host->rptr);
+    m->d_idx = upload_i32(ctx->c.get(), m->host->idx);
+    m->d_val = upload_values(ctx->c.get(), m->host->val, F3R_F64);
   }
   *A = new f3r_mat_s{m};
   F3R_API_END
@@ -758,9 +755,9 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
   m->format = 1;
   m->dtype = dtype;
   m->host = csr->m->host;
-  m->sh = upload_sell_shared(ctx->c, hs, false);
+  m->sh = upload_sell_shared(ctx->c.get(), hs, false);
   m->sh->nnz = csr->m->host->nnz();
-  m->s_val = upload_sell_values(ctx->c, hs, dtype);
+  m->s_val = upload_sell_values(ctx->c.get(), hs, dtype);
   m->sell_host_val = std::make_shared<std::vector<double>>(std::move(hs.val));
   *out = new f3r_mat_s{m};
   F3R_API_END
@@ -791,7 +788,7 @@ int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
             "mat_cast: needs a SELL matrix");
   auto m = std::make_shared<Mat>(*sell->m);
   m->dtype = dtype;
-  m->s_val = upload_values(m->ctx, *m->sell_host_val, dtype);
+  m->s_val = upload_values(m->ctx.get(), *m->sell_host_val, dtype);
   *out = new f3r_mat_s{m};
   F3R_API_END
 }

@@ -27,10 +27,20 @@ template <class T> __device__ __forceinline__ T fma_(T a, T b, T c);
 template <> __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <> __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Keeps v as a rounded fp32 value in a register: the compiler may otherwise
+// fold the arithmetic that produced v and a following fp32 -> fp16 conversion
+// into one mixed-precision FMA that rounds once, straight to fp16.  The
+// arithmetic here is "fp32 result, then fp16 store" (two roundings), as the
+// oracle computes it.
+__device__ __forceinline__ float fp32_rounded(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // store conversions, round to nearest even
 template <class D> __device__ __forceinline__ D cvt(float v);
 template <class D> __device__ __forceinline__ D cvt(double v);
-template <> __device__ __forceinline__ half_t cvt<half_t>(float v) { return (half_t)v; }
+template <> __device__ __forceinline__ half_t cvt<half_t>(float v) { return (half_t)fp32_rounded(v); }
 template <> __device__ __forceinline__ float cvt<float>(float v) { return v; }
 template <> __device__ __forceinline__ double cvt<double>(float v) { return (double)v; }
 template <> __device__ __forceinline__ float cvt<float>(double v) { return (float)v; }
@@ -147,7 +157,7 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i)
-      __hip_atomic_store(&partial[i * gridDim.x + blockIdx.x], v[i], __ATOMIC_RELAXED,
+      __hip_atomic_store(&partial[(int64_t)i * gridDim.x + blockIdx.x], v[i], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
     unsigned prev = atomicAdd(counter, 1u);
@@ -158,10 +168,10 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
   __threadfence();
   T acc[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    acc[i] = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-      acc[i] += load_coherent(&partial[i * gridDim.x + b]);
+  for (int i = 0; i < NV; ++i) acc[i] = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] += load_coherent(&partial[(int64_t)i * gridDim.x + b]);
   }
   block_sum<T, NV>(acc, lds);
   if (threadIdx.x == 0) {

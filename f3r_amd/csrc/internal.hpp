@@ -10,7 +10,7 @@ namespace f3r {
 // diagonal scalings when both factors are diagonal) or ILU(0) (level-scheduled
 // triangular solves).  Values in `dtype`, sums in fp32 for fp16 / fp32 data.
 struct Prec {
-  Context *ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   int kind = 0;  // 0 SDAINV, 1 ILU(0)
   int dtype = F3R_F16;
   double tol = 0.1, acc = 1.0;
@@ -26,8 +26,7 @@ struct Prec {
   DevPtr l_rows, u_rows, l_rptr, l_idx, l_val, u_rptr, u_idx, u_val, u_dinv;
 };
 
-Context *ctx_create(int device);
-void ctx_destroy(Context *c);
+std::shared_ptr<Context> ctx_create(int device);
 double read_scalar(Context *c, const double *dev);
 SellArgs sell_args(const SellShared &sh, bool with_perm);
 DevPtr upload_values(Context *ctx, const std::vector<double> &v, int dtype);
@@ -62,7 +61,7 @@ struct f3r_vec_s;
 struct f3r_mat_s;
 struct f3r_prec_s;
 namespace f3r {
-Context *ctx_of(f3r_ctx c);
+std::shared_ptr<Context> ctx_of(f3r_ctx c);
 Vec *vec_of(f3r_vec v);
 std::shared_ptr<Mat> mat_of(f3r_mat m);
 std::shared_ptr<Prec> prec_of(f3r_prec p);

@@ -275,12 +275,12 @@ __global__ void k_l4_entry(int64_t n, const float *__restrict__ in, float s, hal
                            const half_t *__restrict__ dz, const RichDev *st, int scale_ome) {
   const float om = scale_ome ? (float)st->ome[0] : 1.f;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const half_t h = (half_t)(in[i] * s);
+    const half_t h = cvt<half_t>(in[i] * s);
     hin[i] = h;
     if (dw) {
-      const half_t t = (half_t)__builtin_fmaf((float)dw[i], (float)h, 0.f);
-      half_t o = (half_t)__builtin_fmaf((float)dz[i], (float)t, 0.f);
-      if (scale_ome) o = (half_t)((float)o * om);
+      const half_t t = cvt<half_t>(__builtin_fmaf((float)dw[i], (float)h, 0.f));
+      half_t o = cvt<half_t>(__builtin_fmaf((float)dz[i], (float)t, 0.f));
+      if (scale_ome) o = cvt<half_t>((float)o * om);
       hout[i] = o;
     }
   }
@@ -290,7 +290,7 @@ __global__ void k_l4_entry(int64_t n, const float *__restrict__ in, float s, hal
 __global__ void k_l4_scale(int64_t n, half_t *__restrict__ hout, const double *w) {
   const float a = (float)*w;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    hout[i] = (half_t)((float)hout[i] * a);
+    hout[i] = cvt<half_t>((float)hout[i] * a);
 }
 
 // hout' = fp16(fma(w_k, tmp2, hout)); last step: out = fp32(hout') / s
@@ -299,7 +299,7 @@ __global__ void k_l4_update(int64_t n, const half_t *__restrict__ tmp2, const ha
                             float inv_s) {
   const float a = (float)*w;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const half_t h = (half_t)__builtin_fmaf(a, (float)tmp2[i], (float)hout[i]);
+    const half_t h = cvt<half_t>(__builtin_fmaf(a, (float)tmp2[i], (float)hout[i]));
     if (out)
       out[i] = (float)h * inv_s;
     else
@@ -323,15 +323,15 @@ struct EpiL4Diag {
   float *__restrict__ out;
   float inv_s;
   __device__ void row(int r, float acc, State &) const {
-    const half_t hr = (half_t)((float)hin[r] - acc);
-    const half_t t = (half_t)__builtin_fmaf((float)dw[r], (float)hr, 0.f);
-    const half_t t2 = (half_t)__builtin_fmaf((float)dz[r], (float)t, 0.f);
+    const half_t hr = cvt<half_t>((float)hin[r] - acc);
+    const half_t t = cvt<half_t>(__builtin_fmaf((float)dw[r], (float)hr, 0.f));
+    const half_t t2 = cvt<half_t>(__builtin_fmaf((float)dz[r], (float)t, 0.f));
     if (hr_out) {
       hr_out[r] = hr;
       tmp2_out[r] = t2;
       return;
     }
-    const half_t h = (half_t)__builtin_fmaf((float)*w, (float)t2, (float)hout[r]);
+    const half_t h = cvt<half_t>(__builtin_fmaf((float)*w, (float)t2, (float)hout[r]));
     if (out)
       out[r] = (float)h * inv_s;
     else
@@ -374,6 +374,41 @@ __global__ void k_rich_reset(RichDev *st) {
   }
 }
 
+// ---------------------------- Richardson of the FLOAT / DOUBLE builds ---
+template <class T>
+__global__ void k_scale_t(int64_t n, T *__restrict__ out, const double *w) {
+  const T a = (T)*w;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    out[i] *= a;
+}
+template <class T>
+__global__ void k_update_t(int64_t n, const T *__restrict__ t2, T *__restrict__ out, const double *w) {
+  const T a = (T)*w;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    out[i] = fma_<T>(a, t2[i], out[i]);
+}
+// alpha_k = <t, r> / <t, t> summed in T; the last block updates sum_k, ome_k
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_probe_t(int64_t n, const T *__restrict__ t, const T *__restrict__ r,
+                                                    RichDev *st, int k, T *partial, unsigned *counter) {
+  __shared__ T lds[2 * kWavesPerBlock];
+  T v[2] = {0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const T ti = t[i];
+    v[0] = fma_<T>(ti, ti, v[0]);
+    v[1] = fma_<T>(ti, r[i], v[1]);
+  }
+  if (grid_sum<T, 2>(v, partial, counter, lds) && threadIdx.x == 0) {
+    const T al = v[1] / v[0];
+    if (k == 0) st->nprobe += 1;
+    const T den = (T)(st->nprobe + 1);
+    const T s = (T)st->sum[k] + al;
+    st->sum[k] = s;
+    st->ome[k] = s / den;
+    st->alpha[k] = al;
+  }
+}
+
 // ------------------------------------------------------------ the solver ---
 template <class T> struct InnerLevel {
   int m = 0;
@@ -384,7 +419,7 @@ template <class T> struct InnerLevel {
 };
 
 struct Solver {
-  Context *ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   f3r_params p{};
   int32_t n = 0;
   int64_t nnz = 0;
@@ -394,16 +429,17 @@ struct Solver {
   float range_s = 1.f;
   // outer level (fp64)
   DevPtr V1, Z1, H1, c1, s1, e1, y1, inv1, status, b_int, x_int;
-  // inner levels (fp32 basis)
-  InnerLevel<float> L2, L3;
-  // Richardson (fp16)
+  // inner levels: fp32 basis (HALF, FLOAT) or fp64 basis (DOUBLE)
+  InnerLevel<float> L2f, L3f;
+  InnerLevel<double> L2d, L3d;
+  // Richardson work vectors (fp16 in HALF, else the basis type)
   DevPtr hin, hout, hout2, hr, tmp2, tt, rich;
   int64_t cnt = 0, calls = 0;
   std::vector<double> hist;
   std::map<uint64_t, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_src;
-  bool capturing = false;
-  int red_grid = 1;
+  int red_grid = 1;   // capped grid of the outer level's strided reductions
+  int dots_grid = 1;  // one row per thread for the fused reductions
 
   ~Solver() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -418,45 +454,53 @@ struct Solver {
   SellArgs args() const { return sell_args(*sh, false); }
   unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + i; }
   template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
+  template <class T> InnerLevel<T> &L2() {
+    if constexpr (std::is_same<T, float>::value) return L2f; else return L2d;
+  }
+  template <class T> InnerLevel<T> &L3() {
+    if constexpr (std::is_same<T, float>::value) return L3f; else return L3d;
+  }
+  int m2() const { return p.variant == 2 ? L2d.m : L2f.m; }
+  int m3() const { return p.variant == 2 ? L3d.m : L3f.m; }
 
   // ---------------------------------------------------- inner Arnoldi ---
-  template <class V, int NV>
-  void spmv_dots(InnerLevel<float> &L, const V *val, int j) {
-    Cols<float, NV> c{};
+  template <class T, class V, int NV>
+  void spmv_dots(InnerLevel<T> &L, const V *val, int j) {
+    Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
-    EpiDots<float, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<float>(), counter(0)};
-    launch_sell<V, float, float>(sh->K, args(), val, L.Z[j], epi, st(), red_grid);
+    EpiDots<T, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<T>(), counter(0)};
+    launch_sell<V, T, T>(sh->K, args(), val, L.Z[j], epi, st(), dots_grid);
   }
-  template <int NV> void cgs_update(InnerLevel<float> &L, int j) {
-    Cols<float, NV> c{};
+  template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
+    Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
-    hipLaunchKernelGGL((k_cgs_update<float, NV>), dim3(red_grid), dim3(kBlock), 0, st(), (int64_t)n,
-                       L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->as<float>(),
-                       L.sn->as<float>(), L.e->as<float>(), j, 1, L.inv->as<float>(), (double *)nullptr,
-                       partial<float>(), counter(1));
+    hipLaunchKernelGGL((k_cgs_update<T, NV>), dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
+                       L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
+                       L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
+                       (double *)nullptr, partial<T>(), counter(1));
   }
-  template <class V> void arnoldi(InnerLevel<float> &L, const V *val, int j) {
+  template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
-      spmv_dots<V, 4>(L, val, j);
-      cgs_update<4>(L, j);
+      spmv_dots<T, V, 4>(L, val, j);
+      cgs_update<T, 4>(L, j);
     } else if (j + 1 <= 8) {
-      spmv_dots<V, 8>(L, val, j);
-      cgs_update<8>(L, j);
+      spmv_dots<T, V, 8>(L, val, j);
+      cgs_update<T, 8>(L, j);
     } else {
-      spmv_dots<V, 16>(L, val, j);
-      cgs_update<16>(L, j);
+      spmv_dots<T, V, 16>(L, val, j);
+      cgs_update<T, 16>(L, j);
     }
-    hipLaunchKernelGGL(k_vscale<float>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
-                       L.inv->as<float>());
+    hipLaunchKernelGGL(k_vscale<T>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
+                       (const T *)L.inv->template as<T>());
   }
-  template <class TO> void solution(InnerLevel<float> &L, TO *out) {
+  template <class T, class TO> void solution(InnerLevel<T> &L, TO *out) {
     const int j = L.m;
     auto go = [&](auto nvtag) {
       constexpr int NV = decltype(nvtag)::value;
-      Cols<float, NV> c{};
+      Cols<T, NV> c{};
       for (int k = 0; k < j; ++k) c.p[k] = L.Z[k];
-      hipLaunchKernelGGL((k_solution<float, TO, NV>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
-                         L.Hp(), L.ldh(), L.e->as<float>(), out);
+      hipLaunchKernelGGL((k_solution<T, TO, NV>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
+                         (const T *)L.Hp(), L.ldh(), (const T *)L.e->template as<T>(), out);
     };
     if (j <= 4) go(std::integral_constant<int, 4>());
     else if (j <= 8) go(std::integral_constant<int, 8>());
@@ -464,7 +508,7 @@ struct Solver {
   }
 
   // ------------------------------------------------- level 4 (fp16) ---
-  void level4(const float *in, float *out) {
+  void level4_half(const float *in, float *out) {
     ++cnt;
     ++calls;
     const bool probe = (cnt % p.c) == 0;
@@ -488,7 +532,7 @@ struct Solver {
     if (probe) {
       half_t *t = tt->as<half_t>();
       launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, EpiStore<half_t, float>{t, nullptr}, st());
-      hipLaunchKernelGGL(k_probe, dim3(red_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hi, rs,
+      hipLaunchKernelGGL(k_probe, dim3(dots_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hi, rs,
                          0, partial<float>(), counter(2));
       hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->alpha[0]);
     }
@@ -507,7 +551,7 @@ struct Solver {
       if (probe) {
         half_t *t = tt->as<half_t>();
         launch_sell<half_t, half_t, float>(sh->K, a, a16, t2, EpiStore<half_t, float>{t, nullptr}, st());
-        hipLaunchKernelGGL(k_probe, dim3(red_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hr_, rs,
+        hipLaunchKernelGGL(k_probe, dim3(dots_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hr_, rs,
                            k, partial<float>(), counter(2));
         hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
                            (const double *)&rs->alpha[k], ho2, last ? out : nullptr, inv_s);
@@ -521,27 +565,86 @@ struct Solver {
       hipLaunchKernelGGL((k_cast2<half_t, float>), g, b, 0, st(), nn, (const half_t *)ho, out);
   }
 
-  // ----------------------------------------------- level 3 and level 2 ---
-  void level3(const float *in, float *out) {
-    L3.V[0] = const_cast<float *>(in);
-    for (int j = 0; j < L3.m; ++j) {
-      level4(L3.V[j], L3.Z[j]);
-      arnoldi<half_t>(L3, A16->as<half_t>(), j);
+  // ------------------------------------ level 4 of the FLOAT / DOUBLE builds ---
+  // Vectors and weights in T, preconditioner in T, SpMVs with the fp64 A
+  // summed in fp64 (work/src/f3r-gpu.cpp:123-152, 174-203); `out` is updated
+  // in place.
+  template <class T> void level4_t(const T *in, T *out) {
+    ++cnt;
+    ++calls;
+    const bool probe = (cnt % p.c) == 0;
+    const int64_t nn = n;
+    const dim3 g(grid()), b(kBlock);
+    RichDev *rs = rich->as<RichDev>();
+    const double *a64 = A64->as<double>();
+    SellArgs a = args();
+    T *t = tt->as<T>(), *hr_ = hr->as<T>(), *t2 = tmp2->as<T>();
+    prec_apply_raw(P.get(), in, out, st());
+    if (probe) {
+      launch_sell<double, T, double>(sh->K, a, a64, out, EpiStore<T, double>{t, nullptr}, st());
+      hipLaunchKernelGGL(k_probe_t<T>, dim3(dots_grid), b, 0, st(), nn, (const T *)t, in, rs, 0, partial<T>(),
+                         counter(2));
+      hipLaunchKernelGGL(k_scale_t<T>, g, b, 0, st(), nn, out, (const double *)&rs->alpha[0]);
+    } else {
+      hipLaunchKernelGGL(k_scale_t<T>, g, b, 0, st(), nn, out, (const double *)&rs->ome[0]);
     }
-    solution<float>(L3, out);
+    for (int k = 1; k < p.m4; ++k) {
+      launch_sell<double, T, double>(sh->K, a, a64, out, EpiResid<T, T, double>{hr_, in, nullptr}, st());
+      prec_apply_raw(P.get(), hr_, t2, st());
+      if (probe) {
+        launch_sell<double, T, double>(sh->K, a, a64, t2, EpiStore<T, double>{t, nullptr}, st());
+        hipLaunchKernelGGL(k_probe_t<T>, dim3(dots_grid), b, 0, st(), nn, (const T *)t, (const T *)hr_, rs, k,
+                           partial<T>(), counter(2));
+        hipLaunchKernelGGL(k_update_t<T>, g, b, 0, st(), nn, (const T *)t2, out, (const double *)&rs->alpha[k]);
+      } else {
+        hipLaunchKernelGGL(k_update_t<T>, g, b, 0, st(), nn, (const T *)t2, out, (const double *)&rs->ome[k]);
+      }
+    }
   }
 
-  void level2_body() {
-    for (int j = 0; j < L2.m; ++j) {
-      level3(L2.V[j], L2.Z[j]);
-      arnoldi<float>(L2, A32->as<float>(), j);
+  template <class T> void level4(const T *in, T *out) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (p.variant == 0) {
+        level4_half(in, out);
+        return;
+      }
+    }
+    level4_t<T>(in, out);
+  }
+
+  // ----------------------------------------------- level 3 and level 2 ---
+  template <class T> void level3(const T *in, T *out) {
+    InnerLevel<T> &L = L3<T>();
+    L.V[0] = const_cast<T *>(in);
+    for (int j = 0; j < L.m; ++j) {
+      level4<T>(L.V[j], L.Z[j]);
+      if constexpr (std::is_same<T, double>::value) {
+        arnoldi<T, double>(L, A64->as<double>(), j);
+      } else {
+        if (p.variant == 0)
+          arnoldi<T, half_t>(L, A16->as<half_t>(), j);
+        else
+          arnoldi<T, float>(L, A32->as<float>(), j);
+      }
+    }
+    solution<T, T>(L, out);
+  }
+
+  template <class T> void level2_body() {
+    InnerLevel<T> &L = L2<T>();
+    for (int j = 0; j < L.m; ++j) {
+      level3<T>(L.V[j], L.Z[j]);
+      if constexpr (std::is_same<T, double>::value)
+        arnoldi<T, double>(L, A64->as<double>(), j);
+      else
+        arnoldi<T, float>(L, A32->as<float>(), j);
     }
   }
 
   // probe positions of the next level-2 body as a bit mask (bit q: the q-th
   // level-4 call probes); UINT64_MAX when not representable
   uint64_t next_pattern() const {
-    const int64_t calls_per = (int64_t)L2.m * L3.m;
+    const int64_t calls_per = (int64_t)m2() * m3();
     if (calls_per > 63) return UINT64_MAX;
     uint64_t mask = 0;
     for (int64_t q = 0; q < calls_per; ++q)
@@ -549,26 +652,24 @@ struct Solver {
     return mask | (1ULL << 63);
   }
 
-  void level2(const double *in, double *out) {
-    hipLaunchKernelGGL((k_cast2<double, float>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, L2.V[0]);
+  template <class T> void level2(const double *in, double *out) {
+    InnerLevel<T> &L = L2<T>();
+    hipLaunchKernelGGL((k_cast2<double, T>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, L.V[0]);
     uint64_t key = p.use_graph ? next_pattern() : UINT64_MAX;
     if (key == UINT64_MAX || graphs.size() > 32) {
-      level2_body();
+      level2_body<T>();
     } else {
       auto it = graphs.find(key);
       if (it == graphs.end()) {
         const int64_t cnt0 = cnt, calls0 = calls;
         hipGraph_t gr;
         F3R_HIP(hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
-        capturing = true;
         try {
-          level2_body();
+          level2_body<T>();
         } catch (...) {
-          capturing = false;
           (void)hipStreamEndCapture(st(), &gr);
           throw;
         }
-        capturing = false;
         F3R_HIP(hipStreamEndCapture(st(), &gr));
         hipGraphExec_t ex;
         F3R_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
@@ -578,10 +679,17 @@ struct Solver {
         calls = calls0;
       }
       F3R_HIP(hipGraphLaunch(it->second, st()));
-      cnt += (int64_t)L2.m * L3.m;
-      calls += (int64_t)L2.m * L3.m;
+      cnt += (int64_t)m2() * m3();
+      calls += (int64_t)m2() * m3();
     }
-    solution<double>(L2, out);
+    solution<T, double>(L, out);
+  }
+
+  void level2_any(const double *in, double *out) {
+    if (p.variant == 2)
+      level2<double>(in, out);
+    else
+      level2<float>(in, out);
   }
 
   // -------------------------------------------------------- outer level ---
@@ -599,15 +707,15 @@ struct Solver {
     SellArgs a = args();
     EpiResidNorm<double> er{V, b_int->as<double>(), e1->as<double>(), inv1->as<double>(), stat,
                             partial<double>(), counter(3)};
-    launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), red_grid);
+    launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), dots_grid);
     hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V, (const double *)inv1->as<double>());
-    double e = read_scalar(ctx, stat);
+    double e = read_scalar(ctx.get(), stat);
     if (record) hist.push_back(e / nrm_b);
     if (e / nrm_b < p.eps) return {true, 0, e};
     int j = 0;
     bool flag = false;
     for (; j < m1; ++j) {
-      level2(V + (int64_t)j * nn, Z + (int64_t)j * nn);
+      level2_any(V + (int64_t)j * nn, Z + (int64_t)j * nn);
       launch_sell<double, double, double>(sh->K, a, A64->as<double>(), Z + (int64_t)j * nn,
                                           EpiStore<double, double>{V + (int64_t)(j + 1) * nn, nullptr}, st());
       const int ncols = j + 1;
@@ -621,7 +729,7 @@ struct Solver {
                          partial<double>(), counter(4));
       hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V + (int64_t)(j + 1) * nn,
                          (const double *)inv1->as<double>());
-      e = read_scalar(ctx, stat);
+      e = read_scalar(ctx.get(), stat);
       if (record) hist.push_back(e / nrm_b);
       if (e / nrm_b < p.eps) {
         ++j;
@@ -643,8 +751,8 @@ struct Solver {
 // ----------------------------------------------------------- set-up ------
 static int ilog2_floor(double v) { return v < 1 ? 0 : (int)std::floor(std::log2(v)); }
 
-Solver *solver_create(Context *ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p) {
-  F3R_CHECK(p.variant == 0, F3R_ERR_UNSUPPORTED, "solver: only the HALF variant is built");
+Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p) {
+  F3R_CHECK(p.variant >= 0 && p.variant <= 2, F3R_ERR_ARG, "solver: variant must be 0 (HALF), 1 (FLOAT) or 2 (DOUBLE)");
   F3R_CHECK(p.m1 >= 1 && p.m1 <= 120, F3R_ERR_ARG, "solver: m1 must be in [1, 120]");
   F3R_CHECK(p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16, F3R_ERR_ARG, "solver: m2, m3 in [1, 16]");
   F3R_CHECK(p.m4 >= 1 && p.m4 <= 16 && p.c >= 1 && p.restart >= 1, F3R_ERR_ARG, "solver: m4 in [1, 16], c, restart >= 1");
@@ -662,27 +770,32 @@ Solver *solver_create(Context *ctx, const HostCSR &A, const Prec *Pin, const f3r
   HostSell hs = build_sell(A, C, sigma, K, nullptr, true);
   bool ident = true;
   for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
-  S->sh = upload_sell_shared(ctx, hs, true);
+  S->sh = upload_sell_shared(ctx.get(), hs, true);
   S->sh->nnz = A.nnz();
   S->sh->sigma = sigma;
-  S->A64 = upload_sell_values(ctx, hs, F3R_F64);
-  S->A32 = upload_sell_values(ctx, hs, F3R_F32);
-  S->A16 = upload_sell_values(ctx, hs, F3R_F16);
-  // innermost preconditioner in the internal order and fp16
+  // the level matrices: fp64 always (outer level; Richardson of FLOAT and
+  // DOUBLE), fp32 for levels 2-3 of HALF / FLOAT, fp16 for HALF
+  S->A64 = upload_sell_values(ctx.get(), hs, F3R_F64);
+  if (p.variant != 2) S->A32 = upload_sell_values(ctx.get(), hs, F3R_F32);
+  if (p.variant == 0) S->A16 = upload_sell_values(ctx.get(), hs, F3R_F16);
+  // innermost preconditioner in the internal order and the innermost precision
+  const int pdt = p.variant == 0 ? F3R_F16 : p.variant == 1 ? F3R_F32 : F3R_F64;
   auto P = std::make_shared<Prec>();
   P->ctx = ctx;
   P->kind = Pin->kind;
-  P->dtype = F3R_F16;
+  P->dtype = pdt;
   P->tol = Pin->tol;
   P->acc = Pin->acc;
   P->host = Pin->host;
   F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
   prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, K);
   S->P = P;
-  S->range_s = p.range_scale > 0 ? (float)p.range_scale
-                                 : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
+  S->range_s = p.variant != 0 ? 1.f
+               : p.range_scale > 0 ? (float)p.range_scale
+                                   : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
   const int64_t n = A.n;
   S->red_grid = reduce_grid(n, ctx->num_cu);
+  S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, Reducer::kPartialDoubles * 2 / 16);
   auto dd = [](int64_t cnt) { return dev_alloc(sizeof(double) * std::max<int64_t>(cnt, 1)); };
   S->V1 = dd(n * (p.m1 + 1));
   S->Z1 = dd(n * p.m1);
@@ -695,23 +808,31 @@ Solver *solver_create(Context *ctx, const HostCSR &A, const Prec *Pin, const f3r
   S->status = dd(4);
   S->b_int = dd(n);
   S->x_int = dd(n);
-  auto mk = [&](InnerLevel<float> &L, int m) {
+  auto mk = [&](auto &L, int m) {
+    using T = typename std::remove_reference<decltype(*L.V.data())>::type;
+    using E = typename std::remove_pointer<T>::type;
     L.m = m;
-    L.Vbuf = dev_alloc(sizeof(float) * n * (m + 1));
-    L.Zbuf = dev_alloc(sizeof(float) * n * m);
-    L.H = dev_alloc(sizeof(float) * (m + 1) * m);
-    L.cs = dev_alloc(sizeof(float) * m);
-    L.sn = dev_alloc(sizeof(float) * m);
-    L.e = dev_alloc(sizeof(float) * (m + 1));
-    L.inv = dev_alloc(sizeof(float) * 4);
+    L.Vbuf = dev_alloc(sizeof(E) * n * (m + 1));
+    L.Zbuf = dev_alloc(sizeof(E) * n * m);
+    L.H = dev_alloc(sizeof(E) * (m + 1) * m);
+    L.cs = dev_alloc(sizeof(E) * m);
+    L.sn = dev_alloc(sizeof(E) * m);
+    L.e = dev_alloc(sizeof(E) * (m + 1));
+    L.inv = dev_alloc(sizeof(E) * 4);
     L.V.resize(m + 1);
     L.Z.resize(m);
-    for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->as<float>() + (int64_t)k * n;
-    for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->as<float>() + (int64_t)k * n;
+    for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->template as<E>() + (int64_t)k * n;
+    for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->template as<E>() + (int64_t)k * n;
   };
-  mk(S->L2, p.m2);
-  mk(S->L3, p.m3);
-  auto dh = [&](void) { return dev_alloc(sizeof(half_t) * n); };
+  if (p.variant == 2) {
+    mk(S->L2d, p.m2);
+    mk(S->L3d, p.m3);
+  } else {
+    mk(S->L2f, p.m2);
+    mk(S->L3f, p.m3);
+  }
+  const size_t wsz = p.variant == 0 ? sizeof(half_t) : p.variant == 1 ? sizeof(float) : sizeof(double);
+  auto dh = [&](void) { return dev_alloc(wsz * n); };
   S->hin = dh();
   S->hout = dh();
   S->hout2 = dh();
@@ -727,7 +848,7 @@ Solver *solver_create(Context *ctx, const HostCSR &A, const Prec *Pin, const f3r
 void solver_solve(Solver *S, Vec *b, Vec *x, f3r_result *res) {
   F3R_CHECK(b->dtype == F3R_F64 && x->dtype == F3R_F64 && b->n == S->n && x->n == S->n, F3R_ERR_ARG,
             "solve: b and x must be fp64 vectors of the matrix size");
-  Context *c = S->ctx;
+  Context *c = S->ctx.get();
   const int64_t n = S->n;
   const dim3 g(S->grid()), bl(kBlock);
   const bool permuted = (bool)S->sh->perm;
@@ -851,18 +972,23 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
   s->cnt = s->p.c;
   hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, s->st(), s->rich->as<RichDev>());
   if (level == 2) {
-    const uint64_t keep = s->p.use_graph;
+    const int32_t keep = s->p.use_graph;
     s->p.use_graph = 0;
-    s->level2((const double *)vi->ptr(), (double *)vo->ptr());
-    s->p.use_graph = (int32_t)keep;
+    s->level2_any((const double *)vi->ptr(), (double *)vo->ptr());
+    s->p.use_graph = keep;
   } else {
-    float *a = s->L2.V[0], *r = s->L2.Z[0];
-    hipLaunchKernelGGL((k_cast2<double, float>), g, b, 0, s->st(), n, (const double *)vi->ptr(), a);
-    if (level == 3)
-      s->level3(a, r);
-    else
-      s->level4(a, r);
-    hipLaunchKernelGGL((k_cast2<float, double>), g, b, 0, s->st(), n, (const float *)r, (double *)vo->ptr());
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      InnerLevel<T> &L = s->template L2<T>();
+      T *a = L.V[0], *r = L.Z[0];
+      hipLaunchKernelGGL((k_cast2<double, T>), g, b, 0, s->st(), n, (const double *)vi->ptr(), a);
+      if (level == 3)
+        s->template level3<T>(a, r);
+      else
+        s->template level4<T>(a, r);
+      hipLaunchKernelGGL((k_cast2<T, double>), g, b, 0, s->st(), n, (const T *)r, (double *)vo->ptr());
+    };
+    if (s->p.variant == 2) go(double()); else go(float());
   }
   F3R_HIP(hipGetLastError());
   F3R_HIP(hipStreamSynchronize(s->st()));
@@ -875,8 +1001,9 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
   Solver *s = S->s;
   const int64_t n = s->n, nnz = s->nnz;
   SellArgs a = s->args();
+  F3R_CHECK(s->p.variant == 0 || which == 3, F3R_ERR_UNSUPPORTED, "time_kernel: fp16/fp32 kernels need the HALF variant");
   half_t *x16 = s->hout->as<half_t>(), *y16 = s->tt->as<half_t>();
-  float *x32 = s->L2.V[0], *y32 = s->L2.V[1];
+  float *x32 = s->L2f.V.empty() ? nullptr : s->L2f.V[0], *y32 = s->L2f.V.empty() ? nullptr : s->L2f.V[1];
   double *x64 = s->x_int->as<double>(), *y64 = s->V1->as<double>();
   double by = 0;
   auto launch = [&]() {

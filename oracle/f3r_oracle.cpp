@@ -1077,6 +1077,15 @@ void orc_bj_solve(void *ilu, const double *in, double *out, double *tmp) {
   bj.solve<double, false>(in, out, tmp);
 }
 
+void orc_bj_solve16(void *ilu, const uint16_t *in, uint16_t *out, uint16_t *tmp, int32_t ref_arith) {
+  const Factors &F = *(Factors *)ilu;
+  BJ<h16> bj(F, 1);
+  if (ref_arith)
+    bj.solve<float, true>((const h16 *)in, (h16 *)out, (h16 *)tmp);
+  else
+    bj.solve<float, false>((const h16 *)in, (h16 *)out, (h16 *)tmp);
+}
+
 int32_t orc_f3r_solve(int32_t n, const int32_t *rptr, const int32_t *idx, const double *val,
                       const double *b, double *x, const orc_f3r_opts *opts,
                       orc_f3r_result *res, double *hist) {

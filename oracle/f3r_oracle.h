@@ -75,6 +75,10 @@ void orc_factor_free(void *h);
 /* block-Jacobi triangular solves of trsv/bj.hpp on the split ILU(0) factors,
  * in fp64 */
 void orc_bj_solve(void *ilu, const double *in, double *out, double *tmp);
+/* the same with fp16 factors and vectors: fp32 sums (ref_arith = 0) or one
+ * fp16 rounding per statement (ref_arith = 1) */
+void orc_bj_solve16(void *ilu, const uint16_t *in, uint16_t *out, uint16_t *tmp,
+                    int32_t ref_arith);
 
 /* ---- nested solve ---- */
 typedef struct {

@@ -59,6 +59,7 @@ def lib():
                                      ctypes.c_void_p, ctypes.c_void_p]
         L.orc_factor_free.argtypes = [ctypes.c_void_p]
         L.orc_bj_solve.argtypes = [ctypes.c_void_p] * 4
+        L.orc_bj_solve16.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int32]
         _lib = L
     return _lib
 

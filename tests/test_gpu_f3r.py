@@ -62,7 +62,10 @@ def test_level4_bitexact(ctx, kind, prec):
     S.level_apply(4, vin, vout)
     ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 4, v, variant=0, ref_arith=0, range_scale=s,
                         precond=1 if prec == "ilu0" else 0, bj_blocks=8)
-    assert np.array_equal(vout.download(), ref)
+    got = vout.download()
+    bad = np.flatnonzero(got != ref)
+    print("level4 mismatches", bad.size, bad[:8], got[bad[:8]], ref[bad[:8]], flush=True)
+    assert bad.size == 0
 
 
 @pytest.mark.parametrize("kind", ["27pt", "lognormal"])
@@ -165,3 +168,42 @@ def test_c2_shape_64cube(ctx):
     assert r.is_solved and _true_res(A, xv.download(), b) <= 1e-8
     _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=_scale(A.n))
     assert abs(r.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter)))
+
+
+@pytest.mark.parametrize("variant,prec", [("float", "sdainv"), ("double", "sdainv"), ("float", "ilu0"),
+                                          ("double", "ilu0")])
+def test_level4_variants_bitexact(ctx, variant, prec):
+    # the FLOAT / DOUBLE builds: Richardson in fp32 / fp64 with the fp64 matrix
+    import f3r_amd as F
+    A = _problem("lognormal", 13)
+    P = _prec(F, A, ctx, prec)
+    S = F.F3R(ctx, A, P, variant=variant, use_graph=False)
+    v = np.random.default_rng(3).uniform(-1, 1, A.n)
+    v /= np.linalg.norm(v)
+    vin = F.vector.from_numpy(ctx, v)
+    vout = F.vector(ctx, "f64", A.n)
+    S.level_apply(4, vin, vout)
+    ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 4, v, variant={"float": 1, "double": 2}[variant],
+                        ref_arith=0, precond=1 if prec == "ilu0" else 0, bj_blocks=8)
+    got = vout.download()
+    bad = np.flatnonzero(got != ref)
+    print(variant, prec, "level4 mismatches", bad.size, flush=True)
+    assert bad.size == 0
+
+
+@pytest.mark.parametrize("variant", ["float", "double"])
+@pytest.mark.parametrize("kind", ["27pt", "convdiff"])
+def test_solve_variants_match_oracle(ctx, variant, kind):
+    import f3r_amd as F
+    A = _problem(kind, 18)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    b = F.uniform(A.n, -1.0, 1.0, seed=8)
+    S = F.F3R(ctx, A, P, variant=variant)
+    bv = F.vector.from_numpy(ctx, b)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    res = S.solve(bv, xv)
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant={"float": 1, "double": 2}[variant],
+                           ref_arith=0)
+    assert res.is_solved and ro.is_solved
+    assert _true_res(A, xv.download(), b) <= 1e-8
+    assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter))), (res.res_iter, ro.res_iter)

@@ -154,6 +154,7 @@ def test_ilu0_apply_matches_oracle(ctx):
     xv = F.vector.from_numpy(ctx, x)
     yv = F.vector(ctx, "f64", A.n)
     P.operate(xv, yv)
+    O.lib().orc_ilu0.restype = __import__("ctypes").c_void_p
     h = O.lib().orc_ilu0(A.n, O.P(A.rptr), O.P(A.idx), O.P(A.val), 4, 1.0)
     out, tmp = np.zeros(A.n), np.zeros(A.n)
     O.lib().orc_bj_solve(h, O.P(x), O.P(out), O.P(tmp))
@@ -166,3 +167,21 @@ def test_ilu0_apply_matches_oracle(ctx):
     P16.operate(x16, y16)
     err = np.linalg.norm(y16.download().astype(np.float64) - out) / np.linalg.norm(out)
     assert err < 5e-3
+    # bit-exact against the oracle's fp16 block-Jacobi solve with fp32 sums
+    for kind in ("27pt", "convdiff"):
+        if kind == "convdiff":
+            A = F.CSR.convdiff7(14, sigma=1.5, corr=4.0, peclet=10.0, seed=3)
+            F.algorithm.scaling(A)
+        P16 = F.ILU0(A, blocks=8, acc=1.0, dtype="f16", ctx=ctx)
+        xh = np.random.default_rng(6).uniform(-1, 1, A.n).astype(np.float16)
+        x16 = F.vector.from_numpy(ctx, xh)
+        y16 = F.vector(ctx, "f16", A.n)
+        P16.operate(x16, y16)
+        h = O.lib().orc_ilu0(A.n, O.P(A.rptr), O.P(A.idx), O.P(A.val), 8, 1.0)
+        o16, t16 = np.zeros(A.n, np.uint16), np.zeros(A.n, np.uint16)
+        O.lib().orc_bj_solve16(h, O.P(xh.view(np.uint16)), O.P(o16), O.P(t16), 0)
+        O.lib().orc_factor_free(h)
+        got = y16.download().view(np.uint16)
+        bad = np.flatnonzero(got != o16)
+        print(kind, "ilu16 mismatches", bad.size, bad[:8], got[bad[:8]].view(np.float16), o16[bad[:8]].view(np.float16))
+        assert bad.size == 0, kind
