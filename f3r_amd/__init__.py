@@ -163,6 +163,14 @@ class CSR:
     def nrows(self):
         return self.n
 
+    def diagonal(self):
+        """a_ii of every row (0 where the row has no diagonal entry)"""
+        d = np.zeros(self.n)
+        rows = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.rptr))
+        on = rows == self.idx
+        d[rows[on]] = self.val[on]
+        return d
+
     def handle(self, ctx=None):
         """C-ABI matrix handle on ctx (None: host-only)."""
         key = id(ctx) if ctx is not None else None
@@ -366,7 +374,7 @@ class F3R:
     Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU."""
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
-                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True):
+                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -379,6 +387,7 @@ class F3R:
         p.C, p.sigma, p.K = C, sigma, K
         p.use_graph = 1 if use_graph else 0
         p.flags = 1 if history else 0
+        p.grid = grid
         self.params = p
         h = ctypes.c_void_p()
         check(lib.f3r_solver_create(ctx.handle, A.handle(ctx), M.handle, ctypes.byref(p), ctypes.byref(h)),
@@ -397,6 +406,20 @@ class F3R:
         out = np.empty(ln.value, np.float64)
         check(lib.f3r_solver_history(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                      ln.value, ctypes.byref(ln)), "history")
+        return out
+
+    def capture(self, max_vectors, every=1):
+        """record the innermost level's input vectors of later solves (every
+        `every`-th call, up to `max_vectors`; 0 stops)"""
+        check(lib.f3r_solver_capture(self.handle, max_vectors, every), "solver_capture")
+
+    def captured(self):
+        """the recorded vectors, (count, n) float32 in the matrix's row order"""
+        k = ctypes.c_int32()
+        check(lib.f3r_solver_captured(self.handle, ctypes.byref(k), None), "solver_captured")
+        out = np.empty((k.value, self.A.n), np.float32)
+        if k.value:
+            check(lib.f3r_solver_captured(self.handle, ctypes.byref(k), _p(out)), "solver_captured")
         return out
 
     def level_apply(self, level, vin, vout):

@@ -25,7 +25,8 @@ P_dbl = ctypes.POINTER(ctypes.c_double)
 class Params(ctypes.Structure):
     _fields_ = [("m1", c_i32), ("m2", c_i32), ("m3", c_i32), ("m4", c_i32), ("c", c_i32),
                 ("restart", c_i32), ("eps", c_dbl), ("variant", c_i32), ("range_scale", c_dbl),
-                ("C", c_i32), ("sigma", c_i32), ("K", c_i32), ("use_graph", c_i32), ("flags", c_i32)]
+                ("C", c_i32), ("sigma", c_i32), ("K", c_i32), ("use_graph", c_i32), ("flags", c_i32),
+                ("grid", c_i32)]
 
 
 class Result(ctypes.Structure):
@@ -87,6 +88,25 @@ SIGNATURES = [
     ("f3r_solver_level_apply", ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     ("f3r_solver_time_kernel", ctypes.c_int, [c_vp, c_i32, c_i32, P_dbl, P_dbl]),
     ("f3r_solver_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_solver_capture", ctypes.c_int, [c_vp, c_i32, c_i32]),
+    ("f3r_solver_captured", ctypes.c_int, [c_vp, ctypes.POINTER(c_i32), c_vp]),
+    ("f3r_mlp_nparams", ctypes.c_int, [P_i64]),
+    ("f3r_mlp_create", ctypes.c_int, [c_vp, ctypes.c_uint64, ctypes.POINTER(c_vp)]),
+    ("f3r_mlp_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_mlp_get", ctypes.c_int, [c_vp, c_vp]),
+    ("f3r_mlp_set", ctypes.c_int, [c_vp, c_vp]),
+    ("f3r_mlp_forward", ctypes.c_int, [c_vp, c_i64, c_vp, c_vp]),
+    ("f3r_mlp_grad_zero", ctypes.c_int, [c_vp]),
+    ("f3r_mlp_backward", ctypes.c_int, [c_vp, c_i64, c_vp, c_vp]),
+    ("f3r_mlp_grad_get", ctypes.c_int, [c_vp, c_vp]),
+    ("f3r_mlp_adam", ctypes.c_int, [c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("f3r_learn_features", ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    ("f3r_lsys_create", ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    ("f3r_lsys_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_lsys_set_probes", ctypes.c_int, [c_vp, c_i32, c_vp]),
+    ("f3r_lsys_loss", ctypes.c_int, [c_vp, c_vp, c_i32, ctypes.POINTER(c_dbl)]),
+    ("f3r_lsys_scales", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("f3r_sdainv_rescale", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
 ]
 
 for _name, _res, _args in SIGNATURES:

@@ -63,7 +63,8 @@ inline DevPtr dev_alloc(size_t bytes) { return std::make_shared<DevBuf>(bytes); 
 struct Reducer {
   static constexpr int64_t kPartialDoubles = 1 << 20;  // 8 MiB of partial sums
   DevPtr partial;
-  DevPtr counters;  // 64 unsigned ints
+  static constexpr int kSlots = 24;                     // reduction slots of kCounterSlot counters
+  DevPtr counters;
   DevPtr scalars;   // 64 doubles of results
 };
 

@@ -131,9 +131,9 @@ std::shared_ptr<Context> ctx_create(int device) {
   c->num_cu = prop.multiProcessorCount;
   F3R_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->red.partial = dev_alloc(sizeof(double) * Reducer::kPartialDoubles);
-  c->red.counters = dev_alloc(sizeof(unsigned) * 64);
+  c->red.counters = dev_alloc(sizeof(unsigned) * Reducer::kSlots * kCounterSlot);
   c->red.scalars = dev_alloc(sizeof(double) * 64);
-  F3R_HIP(hipMemsetAsync(c->red.counters->p, 0, sizeof(unsigned) * 64, c->stream));
+  F3R_HIP(hipMemsetAsync(c->red.counters->p, 0, sizeof(unsigned) * Reducer::kSlots * kCounterSlot, c->stream));
   F3R_HIP(hipHostMalloc((void **)&c->pinned, 4096, hipHostMallocDefault));
   F3R_HIP(hipStreamSynchronize(c->stream));
   return c;
@@ -578,6 +578,7 @@ std::shared_ptr<Context> ctx_of(f3r_ctx c) { return c ? c->c : nullptr; }
 Vec *vec_of(f3r_vec v) { return v ? &v->v : nullptr; }
 std::shared_ptr<Mat> mat_of(f3r_mat m) { return m ? m->m : nullptr; }
 std::shared_ptr<Prec> prec_of(f3r_prec p) { return p ? p->p : nullptr; }
+f3r_prec wrap_prec(std::shared_ptr<Prec> p) { return new f3r_prec_s{std::move(p)}; }
 }  // namespace f3r
 
 extern "C" {

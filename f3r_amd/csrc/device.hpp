@@ -140,47 +140,96 @@ __device__ __forceinline__ void block_sum(T (&v)[NV], T *lds /* NV * kWavesPerBl
   __syncthreads();
 }
 
-template <class T> __device__ __forceinline__ T load_coherent(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+constexpr int kGroup = 64;           // blocks per first-level group of grid_sum
+constexpr int kCounterSlot = 1024;   // counters per reduction slot (1 + groups)
+
+// Single-pass grid reduction of NV sums, two-level.  Every block writes its
+// NV block sums to partial[i * G + block]; the last block to arrive in each
+// group of kGroup blocks adds that group's partials (in block order) into
+// partial[NV * G + i * ng + group]; the last group to finish adds the group
+// sums (in group order) and returns true with the totals in v of thread 0.
+// The order of every addition is fixed, so the result does not depend on
+// scheduling.  counters[0] counts groups, counters[1 + g] blocks of group g;
+// each is reset by the block that reads it last, so no memset is ever needed
+// between launches (nor recorded in a graph).
+//
+// Publication across XCDs: the partial sums are agent-scope atomic stores
+// (written through to the coherence point), completed (vmcnt 0) before the
+// counter increment.  No agent-scope release fence is used on the writer
+// side: on CDNA that fence writes back the whole L2 of the XCD, and one per
+// block costs far more than the reduction.  The block that wins a counter
+// takes an agent-scope acquire (L2 invalidate) before it reads the partials.
+template <class T>
+__device__ __forceinline__ void publish(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until this wave's stores are complete, and keep the compiler from
+// moving memory operations across
+__device__ __forceinline__ void stores_complete() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+}
+__device__ __forceinline__ unsigned arrive(unsigned *c) {
+  return __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Single-pass grid reduction of NV sums.  Every block writes its NV partial
-// sums to partial[i * gridDim.x + block]; the last block to arrive (counted
-// in *counter) adds all partials in block order -- a fixed order, so the
-// result does not depend on scheduling -- and returns true with the totals in
-// `out` of thread 0.  It resets *counter to zero for the next launch.
 template <class T, int NV>
 __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
-                                         unsigned *__restrict__ counter, T *lds) {
+                                         unsigned *__restrict__ counters, T *lds) {
   block_sum<T, NV>(v, lds);
-  __shared__ bool last;
+  const int G = gridDim.x;
+  const int ng = (G + kGroup - 1) / kGroup;
+  const int grp = blockIdx.x / kGroup;
+  const int b0 = grp * kGroup;
+  const int gsize = (G - b0) < kGroup ? (G - b0) : kGroup;
+  __shared__ int stage;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-      __hip_atomic_store(&partial[(int64_t)i * gridDim.x + blockIdx.x], v[i], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
+    for (int i = 0; i < NV; ++i) publish(&partial[(int64_t)i * G + blockIdx.x], v[i]);
+    stores_complete();
+    const unsigned prev = arrive(&counters[1 + grp]);
+    stage = (prev == (unsigned)gsize - 1) ? 1 : 0;
   }
   __syncthreads();
-  if (!last) return false;
-  __threadfence();
+  if (stage == 0) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   T acc[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+  if (threadIdx.x < gsize) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] += load_coherent(&partial[(int64_t)i * gridDim.x + b]);
+    for (int i = 0; i < NV; ++i) acc[i] = partial[(int64_t)i * G + b0 + threadIdx.x];
+  }
+  block_sum<T, NV>(acc, lds);
+  T *gpart = partial + (int64_t)NV * G;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&counters[1 + grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) publish(&gpart[(int64_t)i * ng + grp], acc[i]);
+    stores_complete();
+    const unsigned prev = arrive(&counters[0]);
+    stage = (prev == (unsigned)ng - 1) ? 2 : 0;
+  }
+  __syncthreads();
+  if (stage != 2) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0;
+  for (int g = threadIdx.x; g < ng; g += kBlock) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] += gpart[(int64_t)i * ng + g];
   }
   block_sum<T, NV>(acc, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = acc[i];
-    *counter = 0;
+    __hip_atomic_store(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;
 }
+
+// largest grid grid_sum supports (one counter per group in a slot)
+constexpr int kMaxReduceGrid = (kCounterSlot - 1) * kGroup;
 
 // Grid size of the reduction kernels: enough blocks to stream from HBM,
 // capped so that the last block's pass over the partials stays short.

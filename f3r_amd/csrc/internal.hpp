@@ -65,4 +65,5 @@ std::shared_ptr<Context> ctx_of(f3r_ctx c);
 Vec *vec_of(f3r_vec v);
 std::shared_ptr<Mat> mat_of(f3r_mat m);
 std::shared_ptr<Prec> prec_of(f3r_prec p);
+f3r_prec wrap_prec(std::shared_ptr<Prec> p);  // new handle owning a reference
 }  // namespace f3r

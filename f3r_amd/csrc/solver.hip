@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kBlock) void k_mdot_strided(int64_t n, const T *__r
     for (int k = 0; k < NV; ++k)
       if (k0 + k < ncols) v[k] = fma_<T>(wi, V[(int64_t)(k0 + k) * n + i], v[k]);
   }
-  if (grid_sum<T, NV>(v, partial + (int64_t)blockIdx.y * NV * gridDim.x, counters + blockIdx.y, lds) &&
+  const int ng = (gridDim.x + kGroup - 1) / kGroup;
+  if (grid_sum<T, NV>(v, partial + (int64_t)blockIdx.y * NV * (gridDim.x + ng), counters + (int64_t)blockIdx.y * kCounterSlot, lds) &&
       threadIdx.x == 0)
     for (int k = 0; k < NV && k0 + k < ncols; ++k) Hcol[k0 + k] = v[k];
 }
@@ -440,6 +441,11 @@ struct Solver {
   std::vector<hipGraph_t> graph_src;
   int red_grid = 1;   // capped grid of the outer level's strided reductions
   int dots_grid = 1;  // one row per thread for the fused reductions
+  // capture of level-4 inputs (fp32, internal row order): every cap_every-th
+  // call until cap_max vectors are held; level 2 runs eagerly meanwhile
+  DevPtr cap;
+  int32_t cap_max = 0, cap_every = 1, cap_n = 0;
+  bool capturing() const { return cap_n < cap_max; }
 
   ~Solver() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -452,7 +458,7 @@ struct Solver {
     return (int)std::min<int64_t>(g, (int64_t)ctx->num_cu * 16);
   }
   SellArgs args() const { return sell_args(*sh, false); }
-  unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + i; }
+  unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + (int64_t)i * kCounterSlot; }
   template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
   template <class T> InnerLevel<T> &L2() {
     if constexpr (std::is_same<T, float>::value) return L2f; else return L2d;
@@ -603,6 +609,14 @@ struct Solver {
   }
 
   template <class T> void level4(const T *in, T *out) {
+    if (capturing() && calls % cap_every == 0) {
+      float *dst = cap->as<float>() + (int64_t)cap_n * n;
+      if constexpr (std::is_same<T, float>::value)
+        F3R_HIP(hipMemcpyAsync(dst, in, sizeof(float) * n, hipMemcpyDeviceToDevice, st()));
+      else
+        hipLaunchKernelGGL((k_cast2<T, float>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, dst);
+      ++cap_n;
+    }
     if constexpr (std::is_same<T, float>::value) {
       if (p.variant == 0) {
         level4_half(in, out);
@@ -655,7 +669,7 @@ struct Solver {
   template <class T> void level2(const double *in, double *out) {
     InnerLevel<T> &L = L2<T>();
     hipLaunchKernelGGL((k_cast2<double, T>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, L.V[0]);
-    uint64_t key = p.use_graph ? next_pattern() : UINT64_MAX;
+    uint64_t key = p.use_graph && !capturing() ? next_pattern() : UINT64_MAX;
     if (key == UINT64_MAX || graphs.size() > 32) {
       level2_body<T>();
     } else {
@@ -795,7 +809,8 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
                                    : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
   const int64_t n = A.n;
   S->red_grid = reduce_grid(n, ctx->num_cu);
-  S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, Reducer::kPartialDoubles * 2 / 16);
+  S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxReduceGrid);
+  if (p.grid > 0) S->dots_grid = std::min(S->dots_grid, (int)p.grid);
   auto dd = [](int64_t cnt) { return dev_alloc(sizeof(double) * std::max<int64_t>(cnt, 1)); };
   S->V1 = dd(n * (p.m1 + 1));
   S->Z1 = dd(n * p.m1);
@@ -927,6 +942,7 @@ int f3r_params_default(f3r_params *p) {
   p->K = 0;
   p->use_graph = 1;
   p->flags = 1;
+  p->grid = 0;
   return F3R_OK;
 }
 
@@ -956,6 +972,45 @@ int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len) {
   *len = (int32_t)h.size();
   if (hist)
     for (int32_t i = 0; i < cap && i < (int32_t)h.size(); ++i) hist[i] = h[i];
+  F3R_API_END
+}
+
+int f3r_solver_capture(f3r_solver S, int32_t max_vectors, int32_t every) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && max_vectors >= 0 && every >= 1, F3R_ERR_ARG, "capture: bad argument");
+  Solver *s = S->s;
+  F3R_HIP(hipStreamSynchronize(s->st()));
+  s->cap.reset();
+  s->cap_n = 0;
+  s->cap_max = 0;
+  s->cap_every = every;
+  if (max_vectors > 0) {
+    s->cap = dev_alloc(sizeof(float) * s->n * (int64_t)max_vectors);
+    s->cap_max = max_vectors;
+  }
+  F3R_API_END
+}
+
+int f3r_solver_captured(f3r_solver S, int32_t *count, float *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && count, F3R_ERR_ARG, "captured: null argument");
+  Solver *s = S->s;
+  F3R_HIP(hipStreamSynchronize(s->st()));
+  *count = s->cap_n;
+  if (out && s->cap_n > 0) {
+    const int64_t n = s->n;
+    std::vector<float> h((size_t)n * s->cap_n);
+    F3R_HIP(hipMemcpy(h.data(), s->cap->p, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    const std::vector<int32_t> &perm = s->sh->h_perm;
+    for (int32_t k = 0; k < s->cap_n; ++k) {
+      const float *src = h.data() + (int64_t)k * n;
+      float *dst = out + (int64_t)k * n;
+      if (s->sh->perm)
+        for (int64_t i = 0; i < n; ++i) dst[perm[i]] = src[i];
+      else
+        std::memcpy(dst, src, sizeof(float) * n);
+    }
+  }
   F3R_API_END
 }
 

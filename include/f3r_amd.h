@@ -172,6 +172,7 @@ typedef struct {
   int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = autotuned) */
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history */
+  int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = one row per thread */
 } f3r_params;
 
 typedef struct {
@@ -208,6 +209,56 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
 int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms,
                            double *bytes);
 int f3r_solver_destroy(f3r_solver S);
+/* record copies of the innermost level's input vectors during later solves:
+ * every `every`-th call, up to `max_vectors` (0 stops and frees).  Level 2
+ * runs without HIP graphs while vectors are being recorded.  These are the
+ * probe vectors of the learned correction below; the reference has no
+ * counterpart (its Richardson operator receives the same vectors, src/f3r-gpu.cpp:229-231). */
+int f3r_solver_capture(f3r_solver S, int32_t max_vectors, int32_t every);
+/* the recorded vectors as fp32, count x n, in the matrix's row order; `out`
+ * may be NULL to query the count */
+int f3r_solver_captured(f3r_solver S, int32_t *count, float *out);
+
+/* ---- learned correction of SDAINV (no counterpart in the reference) ----
+ * A per-row MLP 12 -> 64 -> 64 -> 1 (ReLU; bf16 weights, fp32 accumulation
+ * on MFMA) maps row features to y_i; the corrected preconditioner is
+ * M = Z diag(s) W^T with s_i = exp(clamp(y_i, -3, 3)), SDAINV's factors
+ * (ainv.hpp:57-64) with the columns of Z rescaled.  Training minimises the
+ * mean over probe vectors z of ||(I - A M) z||^2 / ||z||^2. */
+#define F3R_MLP_NIN 16   /* feature columns per row; columns 12..15 are zero */
+#define F3R_MLP_NFEAT 12
+typedef struct f3r_mlp_s *f3r_mlp;
+typedef struct f3r_lsys_s *f3r_lsys;
+int f3r_mlp_nparams(int64_t *n);
+/* He-uniform hidden layers from `seed`, last layer zero (M = SDAINV) */
+int f3r_mlp_create(f3r_ctx ctx, uint64_t seed, f3r_mlp *m);
+int f3r_mlp_destroy(f3r_mlp m);
+/* fp32 master parameters: W1 [64][16], b1 [64], W2 [64][64], b2 [64],
+ * w3 [64], b3 (row-major, out x in); set also clears the Adam moments */
+int f3r_mlp_get(f3r_mlp m, float *params);
+int f3r_mlp_set(f3r_mlp m, const float *params);
+/* y = MLP(feat) for n rows of F3R_MLP_NIN host floats (rounded to bf16) */
+int f3r_mlp_forward(f3r_mlp m, int64_t n, const float *feat, float *y);
+int f3r_mlp_grad_zero(f3r_mlp m);
+/* gradient += d(sum_r gy_r y_r)/d(params) */
+int f3r_mlp_backward(f3r_mlp m, int64_t n, const float *feat, const float *gy);
+int f3r_mlp_grad_get(f3r_mlp m, float *grad);
+/* one Adam step on the accumulated gradient (the gradient is kept) */
+int f3r_mlp_adam(f3r_mlp m, float lr, float beta1, float beta2, float eps);
+/* the 12 features of every row (n x F3R_MLP_NIN) from the scaled CSR matrix,
+ * its SDAINV preconditioner and, optionally, the unscaled diagonal */
+int f3r_learn_features(f3r_mat A, f3r_prec P, const double *diag, float *feat);
+/* a training system: A, A^T, Z, Z^T, W^T in fp32 SELL and bf16 features */
+int f3r_lsys_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const float *feat, f3r_lsys *L);
+int f3r_lsys_destroy(f3r_lsys L);
+/* k probe vectors, k x n host floats */
+int f3r_lsys_set_probes(f3r_lsys L, int32_t k, const float *probes);
+/* the loss; with grad != 0 its gradient is added to the MLP's */
+int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t grad, double *loss);
+/* s = exp(clamp(MLP(features))) for every row */
+int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, float *s);
+/* a new SDAINV preconditioner with Z's column i scaled by s_i (s > 0) */
+int f3r_sdainv_rescale(f3r_ctx ctx, f3r_prec P, const float *s, f3r_prec *out);
 
 #ifdef __cplusplus
 }
