@@ -8,7 +8,12 @@ reference driver does; b uniform in [-1, 1) from a seed; SDAINV (tol 0.1,
 acc 1.0); m = (100; 8, 4, 2), c = 64, restart 20.  All inputs are resident in
 HBM before the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--no-cpu]
+Learned vs plain (the second half of the metric): the per-row MLP correction
+of SDAINV, trained on the 7-point lognormal-diffusion family at 96^3
+(f3r_amd/learn.py; parameters in f3r_amd/data/), against plain SDAINV F3R on
+held-out systems of that family, reported under "learned".
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--no-cpu] [--learned N]
 
 Prints one JSON line.
 """
@@ -46,6 +51,35 @@ def cpu_baseline(A, b, outer_gpu, scale):
                       f"{threads} OpenMP threads; setup excluded"}
 
 
+def learned_vs_plain(F, ctx, count, nx=96, reps=3):
+    """plain vs learned F3R on `count` held-out systems of the learned
+    correction's family (seeds 100000...), best of `reps` solves each"""
+    import numpy as np
+    from f3r_amd import learn
+    path = os.path.join(ROOT, "f3r_amd", "data", "learned_sdainv_96.npz")
+    if not os.path.exists(path):
+        return {"status": "no trained parameters at f3r_amd/data/learned_sdainv_96.npz"}
+    d = np.load(path, allow_pickle=False)
+    mlp = learn.MLP(ctx, params=d["params"])
+    seeds = list(range(100000, 100000 + count))
+    rows = []
+    for hs in learn.host_systems(nx, seeds, threads=min(8, count)):
+        r = learn.evaluate(ctx, mlp, hs, reps=reps)
+        log(f"[bench] learned: seed {hs['seed']} plain {r['plain']['outer']} steps {r['plain']['solve_s'] * 1e3:.2f} ms, "
+            f"learned {r['learned']['outer']} steps {r['learned']['solve_s'] * 1e3:.2f} ms")
+        rows.append({"seed": hs["seed"], "sigma": round(hs["sigma"], 3), "corr": round(hs["corr"], 2),
+                     "plain_s": r["plain"]["solve_s"], "learned_s": r["learned"]["solve_s"],
+                     "plain_outer": r["plain"]["outer"], "learned_outer": r["learned"]["outer"],
+                     "plain_res": r["plain"]["true_rel_res"], "learned_res": r["learned"]["true_rel_res"]})
+    ps = float(np.mean([r["plain_s"] for r in rows]))
+    ls = float(np.mean([r["learned_s"] for r in rows]))
+    return {"workload": f"7-point lognormal diffusion {nx}^3 (sigma ~ U[0.5,2], corr ~ U[2,16]), held-out seeds "
+                        f"{seeds[0]}..{seeds[-1]}; MLP 12-64-64-1 bf16 trained on {int(d['train'])} systems",
+            "plain_s": round(ps, 6), "learned_s": round(ls, 6), "speedup": round(ps / ls, 4),
+            "all_solved": all(r["plain_res"] <= 1e-8 and r["learned_res"] <= 1e-8 for r in rows),
+            "systems": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +89,11 @@ def main():
     ap.add_argument("--blocks", type=int, default=0, help="blocks of the fused SpMV+reduction kernels (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--learned", type=int, default=4, metavar="N",
+                    help="held-out 96^3 lognormal systems for learned vs plain F3R (0: skip)")
+    ap.add_argument("--kernel-only", type=int, default=0, metavar="REPS",
+                    help="no solves: launch the dominant fp16 SpMV REPS times and a plain copy of a known "
+                         "size REPS times (for hardware-counter runs), print their timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the solve is not sharded across GPUs "
@@ -76,6 +115,21 @@ def main():
     S = F.F3R(ctx, A, P, use_graph=not args.no_graph, grid=args.blocks)
     setup_s = time.time() - t0
     log(f"[bench] setup {setup_s:.1f} s")
+    if args.kernel_only:
+        ms16, by16 = S.time_kernel(0, reps=args.kernel_only)
+        # calibration copy: fp16 x -> fp16 y of n elements (reads and writes 2n bytes)
+        xc = F.vector.from_numpy(ctx, np.ones(A.n), "f16")
+        yc = F.vector(ctx, "f16", A.n)
+        ctx.sync()
+        tc = time.time()
+        for _ in range(args.kernel_only):
+            yc.copy(xc)
+        ctx.sync()
+        tcopy = (time.time() - tc) / args.kernel_only
+        print(json.dumps({"spmv_f16_us": round(ms16 * 1e3, 2), "spmv_f16_algorithmic_bytes": by16,
+                          "copy_elements": A.n, "copy_bytes_read": 2 * A.n, "copy_bytes_written": 2 * A.n,
+                          "copy_us_wall": round(tcopy * 1e6, 2), "reps": args.kernel_only}), flush=True)
+        return
     bv = F.vector.from_numpy(ctx, b)
     xv = F.vector(ctx, "f64", A.n)
     for w in range(args.warmup):
@@ -128,6 +182,8 @@ def main():
                      "us_per_launch": round(ms16 * 1e3, 2), "algorithmic_bytes": by16},
         "kernels": kern,
     }
+    if args.learned > 0:
+        out["learned"] = learned_vs_plain(F, ctx, args.learned)
     if not args.no_cpu:
         scale = float(2.0 ** np.floor(np.log2(np.sqrt(A.n) / 4.0)))
         log("[bench] cpu baseline (oracle, one outer step) ...")

@@ -267,15 +267,15 @@ def sell_layout(A, C=32, sigma=1, K=1):
 class SELL:
     """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166)."""
 
-    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, _handle=None):
+    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, _handle=None):
         self.ctx = ctx
         self.A = A
         self.dtype = _dtype(dtype)
-        self.C, self.sigma, self.K = C, sigma, K
+        self.C, self.sigma, self.K, self.offsets = C, sigma, K, offsets
         if _handle is None:
             h = ctypes.c_void_p()
-            check(lib.f3r_mat_to_sell(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, ctypes.byref(h)),
-                  "mat_to_sell")
+            check(lib.f3r_mat_to_sell_ex(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, 1 if offsets else 0,
+                                         ctypes.byref(h)), "mat_to_sell")
             self.handle = h
         else:
             self.handle = _handle
@@ -284,12 +284,18 @@ class SELL:
         """cross-precision copy sharing the index arrays (sell.hpp:51-53)"""
         h = ctypes.c_void_p()
         check(lib.f3r_mat_cast(self.handle, _dtype(dtype), ctypes.byref(h)), "mat_cast")
-        return SELL(self.ctx, self.A, dtype, self.C, self.sigma, self.K, _handle=h)
+        return SELL(self.ctx, self.A, dtype, self.C, self.sigma, self.K, self.offsets, _handle=h)
 
     def nslots(self):
         ns = ctypes.c_int64()
         check(lib.f3r_mat_info(self.handle, None, None, ctypes.byref(ns), None, None), "mat_info")
         return ns.value
+
+    def chunks(self):
+        """(chunks, chunks in the chunk-offset format)"""
+        c, o = ctypes.c_int32(), ctypes.c_int32()
+        check(lib.f3r_mat_chunks(self.handle, ctypes.byref(c), ctypes.byref(o)), "mat_chunks")
+        return c.value, o.value
 
     def operate(self, x, y):
         check(lib.f3r_spmv(self.handle, x.handle, y.handle), "spmv")
@@ -374,7 +380,7 @@ class F3R:
     Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU."""
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
-                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0):
+                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -388,6 +394,7 @@ class F3R:
         p.use_graph = 1 if use_graph else 0
         p.flags = 1 if history else 0
         p.grid = grid
+        p.index = index
         self.params = p
         h = ctypes.c_void_p()
         check(lib.f3r_solver_create(ctx.handle, A.handle(ctx), M.handle, ctypes.byref(p), ctypes.byref(h)),

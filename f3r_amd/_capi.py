@@ -26,7 +26,7 @@ class Params(ctypes.Structure):
     _fields_ = [("m1", c_i32), ("m2", c_i32), ("m3", c_i32), ("m4", c_i32), ("c", c_i32),
                 ("restart", c_i32), ("eps", c_dbl), ("variant", c_i32), ("range_scale", c_dbl),
                 ("C", c_i32), ("sigma", c_i32), ("K", c_i32), ("use_graph", c_i32), ("flags", c_i32),
-                ("grid", c_i32)]
+                ("grid", c_i32), ("index", c_i32)]
 
 
 class Result(ctypes.Structure):
@@ -88,6 +88,9 @@ SIGNATURES = [
     ("f3r_solver_level_apply", ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     ("f3r_solver_time_kernel", ctypes.c_int, [c_vp, c_i32, c_i32, P_dbl, P_dbl]),
     ("f3r_solver_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_mat_to_sell_ex", ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i32, c_i32, c_i32, c_i32,
+                                          ctypes.POINTER(c_vp)]),
+    ("f3r_mat_chunks", ctypes.c_int, [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
     ("f3r_solver_capture", ctypes.c_int, [c_vp, c_i32, c_i32]),
     ("f3r_solver_captured", ctypes.c_int, [c_vp, ctypes.POINTER(c_i32), c_vp]),
     ("f3r_mlp_nparams", ctypes.c_int, [P_i64]),
@@ -104,7 +107,7 @@ SIGNATURES = [
     ("f3r_lsys_create", ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     ("f3r_lsys_destroy", ctypes.c_int, [c_vp]),
     ("f3r_lsys_set_probes", ctypes.c_int, [c_vp, c_i32, c_vp]),
-    ("f3r_lsys_loss", ctypes.c_int, [c_vp, c_vp, c_i32, ctypes.POINTER(c_dbl)]),
+    ("f3r_lsys_loss", ctypes.c_int, [c_vp, c_vp, c_i32, c_i32, ctypes.POINTER(c_dbl)]),
     ("f3r_lsys_scales", ctypes.c_int, [c_vp, c_vp, c_vp]),
     ("f3r_sdainv_rescale", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
 ]

@@ -107,6 +107,8 @@ struct SellShared {  // index data shared by all precisions of one matrix
   int64_t nslots = 0, nnz = 0;
   bool internal = false;
   DevPtr cptr, cwidth, idx, perm;  // perm null when identity
+  DevPtr iptr, coff;               // chunk-offset format (null when not used)
+  int32_t offset_chunks = 0;
   std::vector<int32_t> h_perm;     // host copy of the permutation
 };
 
@@ -131,11 +133,19 @@ struct HostSell {
   int32_t n, C, K, nchunks;
   std::vector<int32_t> cptr, cwidth, idx, perm;
   std::vector<double> val;
+  // chunk-offset index format (empty when not used): iptr[s] >= 0 is the
+  // first column index of explicit chunk s in `idx`; iptr[s] < 0 marks an
+  // offset chunk whose slot t of row r reads column r + coff[-iptr[s] - 1 + t]
+  std::vector<int32_t> iptr, coff;
+  int32_t offset_chunks = 0;
 };
 // Builds the layout; when `renumber` the columns are mapped through the
-// inverse permutation (the matrix P A P^T in sorted order).
+// inverse permutation (the matrix P A P^T in sorted order).  With `offsets`
+// (and C >= 64) every chunk whose rows share a small set of column offsets
+// col - row, all in range and in CSR order, stores that set once instead of
+// a column index per slot.
 HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
-                    const std::vector<int32_t> *perm_override, bool renumber);
+                    const std::vector<int32_t> *perm_override, bool renumber, bool offsets = false);
 struct HostFactors {
   HostCSR f[2];
 };

@@ -265,6 +265,8 @@ SellArgs sell_args(const SellShared &sh, bool with_perm) {
   a.cwidth = sh.cwidth->as<int32_t>();
   a.idx = sh.idx->as<int32_t>();
   a.perm = (with_perm && sh.perm) ? sh.perm->as<int32_t>() : nullptr;
+  a.iptr = sh.iptr ? sh.iptr->as<int32_t>() : nullptr;
+  a.coff = sh.coff ? sh.coff->as<int32_t>() : nullptr;
   return a;
 }
 
@@ -274,7 +276,8 @@ std::shared_ptr<SellShared> upload_sell_shared(Context *ctx, const HostSell &hs,
   sh->C = hs.C;
   sh->K = hs.K;
   sh->nchunks = hs.nchunks;
-  sh->nslots = (int64_t)hs.idx.size();
+  sh->nslots = (int64_t)hs.val.size();
+  sh->offset_chunks = hs.offset_chunks;
   sh->internal = internal;
   hipStream_t st = ctx->stream;
   sh->cptr = dev_alloc(sizeof(int32_t) * hs.cptr.size());
@@ -285,6 +288,13 @@ std::shared_ptr<SellShared> upload_sell_shared(Context *ctx, const HostSell &hs,
     F3R_HIP(hipMemcpyAsync(sh->cwidth->p, hs.cwidth.data(), sizeof(int32_t) * hs.cwidth.size(), hipMemcpyHostToDevice, st));
   if (!hs.idx.empty())
     F3R_HIP(hipMemcpyAsync(sh->idx->p, hs.idx.data(), sizeof(int32_t) * hs.idx.size(), hipMemcpyHostToDevice, st));
+  if (!hs.iptr.empty()) {
+    sh->iptr = dev_alloc(sizeof(int32_t) * hs.iptr.size());
+    sh->coff = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, hs.coff.size()));
+    F3R_HIP(hipMemcpyAsync(sh->iptr->p, hs.iptr.data(), sizeof(int32_t) * hs.iptr.size(), hipMemcpyHostToDevice, st));
+    if (!hs.coff.empty())
+      F3R_HIP(hipMemcpyAsync(sh->coff->p, hs.coff.data(), sizeof(int32_t) * hs.coff.size(), hipMemcpyHostToDevice, st));
+  }
   bool ident = true;
   for (int32_t r = 0; r < hs.n && ident; ++r) ident = hs.perm[r] == r;
   if (!ident) {
@@ -447,7 +457,7 @@ HostCSR permute_csr(const HostCSR &M, const std::vector<int32_t> &perm) {
   return R;
 }
 
-void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K) {
+void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K, bool offsets) {
   Context *c = P->ctx.get();
   const int32_t n = P->host.f[0].n;
   P->n = n;
@@ -468,7 +478,7 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
     }
     for (int w = 0; w < 2; ++w) {
       const HostCSR &M = w == 0 ? W : Z;
-      HostSell hs = build_sell(M, C, 1, K, perm, perm != nullptr);
+      HostSell hs = build_sell(M, C, 1, K, perm, perm != nullptr, offsets);
       auto m = std::make_shared<Mat>();
       m->ctx = P->ctx;
       m->format = 1;
@@ -743,6 +753,11 @@ int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz, c
 }
 
 int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma, int32_t K, f3r_mat *out) {
+  return f3r_mat_to_sell_ex(ctx, csr, dtype, C, sigma, K, 0, out);
+}
+
+int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma, int32_t K, int32_t flags,
+                       f3r_mat *out) {
   F3R_API_BEGIN
   F3R_CHECK(ctx && csr && out && csr->m->format == 0, F3R_ERR_ARG, "to_sell: needs a context and a CSR matrix");
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "to_sell: C must be a power of two >= 16");
@@ -750,7 +765,8 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
   F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
   F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
   *out = nullptr;
-  HostSell hs = build_sell(*csr->m->host, C, sigma, K, nullptr, false);
+  F3R_CHECK((flags & ~1) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  HostSell hs = build_sell(*csr->m->host, C, sigma, K, nullptr, false, (flags & 1) != 0);
   auto m = std::make_shared<Mat>();
   m->ctx = ctx->c;
   m->format = 1;
@@ -812,6 +828,15 @@ int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *
   if (nslots) *nslots = m.format == 1 ? m.sh->nslots : (m.host ? m.host->nnz() : 0);
   if (dtype) *dtype = m.dtype;
   if (format) *format = m.format;
+  F3R_API_END
+}
+
+int f3r_mat_chunks(f3r_mat A, int32_t *chunks, int32_t *offset_chunks) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && chunks && offset_chunks, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(A->m->format == 1, F3R_ERR_ARG, "mat_chunks: needs a SELL matrix");
+  *chunks = A->m->sh->nchunks;
+  *offset_chunks = A->m->sh->offset_chunks;
   F3R_API_END
 }
 

@@ -140,8 +140,9 @@ __device__ __forceinline__ void block_sum(T (&v)[NV], T *lds /* NV * kWavesPerBl
   __syncthreads();
 }
 
-constexpr int kGroup = 64;           // blocks per first-level group of grid_sum
-constexpr int kCounterSlot = 1024;   // counters per reduction slot (1 + groups)
+constexpr int kGroup = 64;            // blocks per first-level group of grid_sum
+constexpr int kCounterStride = 64;    // 256 B between counters: one cache line each
+constexpr int kCounterSlot = 256 * kCounterStride;  // one reduction slot: 1 + 255 groups
 
 // Single-pass grid reduction of NV sums, two-level.  Every block writes its
 // NV block sums to partial[i * G + block]; the last block to arrive in each
@@ -149,19 +150,27 @@ constexpr int kCounterSlot = 1024;   // counters per reduction slot (1 + groups)
 // partial[NV * G + i * ng + group]; the last group to finish adds the group
 // sums (in group order) and returns true with the totals in v of thread 0.
 // The order of every addition is fixed, so the result does not depend on
-// scheduling.  counters[0] counts groups, counters[1 + g] blocks of group g;
+// scheduling.  counters[0] counts groups, counters[(1 + g) * kCounterStride]
+// the blocks of group g;
 // each is reset by the block that reads it last, so no memset is ever needed
-// between launches (nor recorded in a graph).
+// between launches (nor recorded in a graph).  Each counter has a cache line
+// of its own: atomics on one line are serialised, and 64 groups sharing four
+// lines made the arrivals, not the sums, the cost of the reduction.
 //
-// Publication across XCDs: the partial sums are agent-scope atomic stores
-// (written through to the coherence point), completed (vmcnt 0) before the
-// counter increment.  No agent-scope release fence is used on the writer
-// side: on CDNA that fence writes back the whole L2 of the XCD, and one per
-// block costs far more than the reduction.  The block that wins a counter
-// takes an agent-scope acquire (L2 invalidate) before it reads the partials.
+// Publication across XCDs: the partial sums are agent-scope atomic stores,
+// completed (vmcnt 0) before the counter increment, and the block that wins
+// a counter reads them with agent-scope atomic loads, issued after the
+// increment returned.  Only atomic accesses touch the partials, so neither
+// side needs a cache-wide fence: an agent-scope release writes back the
+// whole L2 of the XCD and an acquire invalidates it, and one of either per
+// block cost more than the reduction itself.
 template <class T>
 __device__ __forceinline__ void publish(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T collect(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // wait until this wave's stores are complete, and keep the compiler from
 // moving memory operations across
@@ -187,23 +196,22 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
 #pragma unroll
     for (int i = 0; i < NV; ++i) publish(&partial[(int64_t)i * G + blockIdx.x], v[i]);
     stores_complete();
-    const unsigned prev = arrive(&counters[1 + grp]);
+    const unsigned prev = arrive(&counters[(1 + grp) * kCounterStride]);
     stage = (prev == (unsigned)gsize - 1) ? 1 : 0;
   }
   __syncthreads();
   if (stage == 0) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   T acc[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = 0;
   if (threadIdx.x < gsize) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = partial[(int64_t)i * G + b0 + threadIdx.x];
+    for (int i = 0; i < NV; ++i) acc[i] = collect(&partial[(int64_t)i * G + b0 + threadIdx.x]);
   }
   block_sum<T, NV>(acc, lds);
   T *gpart = partial + (int64_t)NV * G;
   if (threadIdx.x == 0) {
-    __hip_atomic_store(&counters[1 + grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&counters[(1 + grp) * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int i = 0; i < NV; ++i) publish(&gpart[(int64_t)i * ng + grp], acc[i]);
     stores_complete();
@@ -212,12 +220,11 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
   }
   __syncthreads();
   if (stage != 2) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = 0;
   for (int g = threadIdx.x; g < ng; g += kBlock) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] += gpart[(int64_t)i * ng + g];
+    for (int i = 0; i < NV; ++i) acc[i] += collect(&gpart[(int64_t)i * ng + g]);
   }
   block_sum<T, NV>(acc, lds);
   if (threadIdx.x == 0) {
@@ -229,7 +236,7 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
 }
 
 // largest grid grid_sum supports (one counter per group in a slot)
-constexpr int kMaxReduceGrid = (kCounterSlot - 1) * kGroup;
+constexpr int kMaxReduceGrid = (kCounterSlot / kCounterStride - 1) * kGroup;
 
 // Grid size of the reduction kernels: enough blocks to stream from HBM,
 // capped so that the last block's pass over the partials stays short.

@@ -61,7 +61,7 @@ int64_t sell_nslots(int32_t n, const int32_t *rptr, int32_t C, int32_t sigma, in
 }
 
 HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
-                    const std::vector<int32_t> *perm_override, bool renumber) {
+                    const std::vector<int32_t> *perm_override, bool renumber, bool offsets) {
   F3R_CHECK(C >= 1 && K >= 1 && sigma >= 1, F3R_ERR_ARG, "SELL: C, K and sigma must be >= 1");
   HostSell S;
   const int32_t n = A.n;
@@ -74,10 +74,15 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
     inv.resize(n);
     for (int32_t r = 0; r < n; ++r) inv[S.perm[r]] = r;
   }
+  auto col = [&](int32_t k) { return renumber ? inv[A.idx[k]] : A.idx[k]; };
+  // a wavefront of 64 rows must lie in one chunk for the offset format
+  offsets = offsets && C >= 64;
   S.nchunks = (n + C - 1) / C;
   S.cptr.assign(S.nchunks + 1, 0);
   S.cwidth.assign(S.nchunks, 0);
+  std::vector<std::vector<int32_t>> uni(offsets ? S.nchunks : 0);
   int64_t tot = 0;
+  std::vector<int32_t> u;
   for (int32_t s = 0; s < S.nchunks; ++s) {
     int64_t r0 = (int64_t)s * C;
     int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
@@ -86,18 +91,77 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
       int32_t o = S.perm[r0 + j];
       mx = std::max(mx, A.rptr[o + 1] - A.rptr[o]);
     }
-    int32_t groups = round_up(mx, K) / K;
+    int32_t width = mx;
+    if (offsets && mx > 0) {
+      // the union of col - row over the chunk; eligible when every row's
+      // columns ascend (CSR order = offset order), every row + offset is a
+      // row of the matrix, and the union is at most 5/4 of the widest row
+      u.clear();
+      bool ok = true;
+      for (int32_t j = 0; j < tc && ok; ++j) {
+        const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
+        int32_t prev = INT32_MIN;
+        for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
+          const int32_t c = col(k);
+          if (c <= prev) ok = false;
+          prev = c;
+          u.push_back(c - r);
+        }
+      }
+      if (ok) {
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        ok = (int64_t)u.size() * 4 <= (int64_t)mx * 5 && r0 + u.front() >= 0 && r0 + tc - 1 + u.back() < n;
+      }
+      if (ok) {
+        uni[s] = u;
+        width = (int32_t)u.size();
+      }
+    }
+    int32_t groups = round_up(width, K) / K;
     S.cwidth[s] = groups;
     tot += (int64_t)groups * K * tc;
     F3R_CHECK(tot < INT32_MAX, F3R_ERR_RANGE, "SELL: more than 2^31 slots");
     S.cptr[s + 1] = (int32_t)tot;
   }
-  S.idx.assign(tot, 0);
   S.val.assign(tot, 0.0);
+  int64_t itot = 0;
+  if (offsets) {
+    S.iptr.assign(S.nchunks, 0);
+    for (int32_t s = 0; s < S.nchunks; ++s) {
+      const int64_t w = (int64_t)S.cwidth[s] * K;
+      if (!uni[s].empty()) {
+        S.iptr[s] = -(int32_t)S.coff.size() - 1;
+        for (int64_t t = 0; t < w; ++t) S.coff.push_back(t < (int64_t)uni[s].size() ? uni[s][t] : 0);
+        ++S.offset_chunks;
+      } else {
+        S.iptr[s] = (int32_t)itot;
+        itot += (S.cptr[s + 1] - S.cptr[s]);
+      }
+    }
+  } else {
+    itot = tot;
+  }
+  S.idx.assign(itot, 0);
   for (int32_t s = 0; s < S.nchunks; ++s) {
     int64_t r0 = (int64_t)s * C;
     int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
     int32_t width = S.cwidth[s] * K;
+    if (offsets && !uni[s].empty()) {
+      // offset chunk: the value of column r + uni[t] goes to slot t, the
+      // other slots stay 0 (their columns r + uni[t] are in range)
+      const std::vector<int32_t> &us = uni[s];
+      for (int32_t j = 0; j < tc; ++j) {
+        const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
+        for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
+          const int32_t t = (int32_t)(std::lower_bound(us.begin(), us.end(), col(k) - r) - us.begin());
+          const int64_t pos = S.cptr[s] + ((int64_t)(t / K) * tc + j) * K + (t % K);
+          S.val[pos] = A.val[k];
+        }
+      }
+      continue;
+    }
+    const int64_t ibase = offsets ? (int64_t)S.iptr[s] - S.cptr[s] : 0;
     for (int32_t j = 0; j < tc; ++j) {
       int32_t o = S.perm[r0 + j];
       int32_t len = A.rptr[o + 1] - A.rptr[o];
@@ -105,12 +169,11 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
       for (int32_t q = 0; q < width; ++q) {
         int64_t pos = S.cptr[s] + ((int64_t)(q / K) * tc + j) * K + (q % K);
         if (q < len) {
-          int32_t c = A.idx[A.rptr[o] + q];
-          last = renumber ? inv[c] : c;
-          S.idx[pos] = last;
+          last = col(A.rptr[o] + q);
+          S.idx[pos + ibase] = last;
           S.val[pos] = A.val[A.rptr[o] + q];
         } else {  // padding: value 0, the previous slot's column (sell.hpp:281-285)
-          S.idx[pos] = last;
+          S.idx[pos + ibase] = last;
           S.val[pos] = 0.0;
         }
       }

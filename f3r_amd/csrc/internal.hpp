@@ -31,7 +31,7 @@ double read_scalar(Context *c, const double *dev);
 SellArgs sell_args(const SellShared &sh, bool with_perm);
 DevPtr upload_values(Context *ctx, const std::vector<double> &v, int dtype);
 HostCSR permute_csr(const HostCSR &M, const std::vector<int32_t> &perm);
-void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K);
+void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int32_t K, bool offsets = true);
 void prec_apply_raw(Prec *P, const void *in, void *out, hipStream_t st);
 void vec_copy(Vec *s, Vec *d);
 // dot product of two device arrays (fp16 data summed in fp32), host result

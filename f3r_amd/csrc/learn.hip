@@ -351,11 +351,37 @@ __global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t n, const float *
     out[1] = v[1];
   }
 }
-// gy += -2 wgt / ||z||^2 * g2 * u * s, zero where the clamp is active
+// <z, q>, <q, q>, <z, z> (single-pass grid reduction)
+__global__ __launch_bounds__(kBlock) void k_qdots(int64_t n, const float *__restrict__ z,
+                                                  const float *__restrict__ q, float *partial, unsigned *counters,
+                                                  float *out) {
+  __shared__ float lds[3 * kWavesPerBlock];
+  float v[3] = {0.f, 0.f, 0.f};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    v[0] = __builtin_fmaf(z[i], q[i], v[0]);
+    v[1] = __builtin_fmaf(q[i], q[i], v[1]);
+    v[2] = __builtin_fmaf(z[i], z[i], v[2]);
+  }
+  if (grid_sum<float, 3>(v, partial, counters, lds) && threadIdx.x == 0) {
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = v[2];
+  }
+}
+// r = <q,q> z - <z,q> q: minus the q-gradient of 1 - <z,q>^2 / (<z,z><q,q>),
+// up to the factor 2 <z,q> / (<z,z> <q,q>^2)
+__global__ void k_rscaled(int64_t n, const float *__restrict__ z, const float *__restrict__ q,
+                          const float *__restrict__ d, float *__restrict__ r) {
+  const float a = d[0], b = d[1];
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    r[i] = b * z[i] - a * q[i];
+}
+// gy += c * g2 * u * s, zero where the clamp is active; c = -2 wgt / ||z||^2
+// (kind 0) or -2 wgt <z,q> / (<z,z> <q,q>^2) (kind 1)
 __global__ void k_gy(int64_t n, const float *__restrict__ g2, const float *__restrict__ u,
                      const float *__restrict__ s, const float *__restrict__ y, const float *__restrict__ norms,
-                     float wgt, float *__restrict__ gy) {
-  const float c = -2.f * wgt / norms[1];
+                     int kind, float wgt, float *__restrict__ gy) {
+  const float c = kind == 0 ? -2.f * wgt / norms[1] : -2.f * wgt * norms[0] / (norms[2] * norms[1] * norms[1]);
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const bool live = fabsf(y[i]) < kYClamp;
     gy[i] += live ? c * g2[i] * u[i] * s[i] : 0.f;
@@ -543,9 +569,13 @@ LSys *lsys_create(std::shared_ptr<Context> ctx, const HostCSR &A, const HostFact
   return L.release();
 }
 
-// mean over the probes of ||(I - A Z diag(s) W^T) z||^2 / ||z||^2; with
-// `grad` the gradient of that mean is added to the MLP's gradient buffer.
-double lsys_loss(LSys *L, MLP *m, bool grad) {
+// Mean over the probes z of, with q = A Z diag(s) W^T z,
+//   kind 0: ||z - q||^2 / ||z||^2
+//   kind 1: min_w ||z - w q||^2 / ||z||^2 = 1 - <z,q>^2 / (<z,z> <q,q>)
+// (kind 1 is blind to a global scale of M, which the adaptive Richardson
+// weights absorb anyway).  With `grad` the gradient of the mean is added to
+// the MLP's gradient buffer.
+double lsys_loss(LSys *L, MLP *m, int kind, bool grad) {
   Context *c = L->ctx.get();
   hipStream_t st = c->stream;
   const int64_t n = L->n;
@@ -561,27 +591,38 @@ double lsys_loss(LSys *L, MLP *m, bool grad) {
   const int rgrid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxReduceGrid);
   for (int k = 0; k < L->nprobes; ++k) {
     const float *z = L->probes->as<float>() + (int64_t)k * n;
-    float *nk = norms + 2 * k;
+    float *nk = norms + 4 * k;
     spmv32(L->Wt, z, u, st);
     hipLaunchKernelGGL(k_mul, g, b, 0, st, n, (const float *)s, (const float *)u, v);
     spmv32(L->Z, v, pz, st);
     spmv32(L->A, pz, q, st);
-    hipLaunchKernelGGL(k_resid_norms, dim3(rgrid), b, 0, st, n, z, (const float *)q, r,
-                       c->red.partial->as<float>(), c->red.counters->as<unsigned>() + 20 * kCounterSlot, nk);
+    unsigned *cnt = c->red.counters->as<unsigned>() + 20 * kCounterSlot;
+    if (kind == 0) {
+      hipLaunchKernelGGL(k_resid_norms, dim3(rgrid), b, 0, st, n, z, (const float *)q, r,
+                         c->red.partial->as<float>(), cnt, nk);
+    } else {
+      hipLaunchKernelGGL(k_qdots, dim3(rgrid), b, 0, st, n, z, (const float *)q, c->red.partial->as<float>(), cnt,
+                         nk);
+      if (grad) hipLaunchKernelGGL(k_rscaled, g, b, 0, st, n, z, (const float *)q, (const float *)nk, r);
+    }
     if (grad) {
       spmv32(L->At, r, g1, st);
       spmv32(L->Zt, g1, g2, st);
       hipLaunchKernelGGL(k_gy, g, b, 0, st, n, (const float *)g2, (const float *)u, (const float *)s,
-                         (const float *)y, (const float *)nk, 1.f / (float)L->nprobes, gy);
+                         (const float *)y, (const float *)nk, kind, 1.f / (float)L->nprobes, gy);
     }
   }
   if (grad && L->nprobes > 0) mlp_backward_dev(m, feat, n, gy, st);
   F3R_HIP(hipGetLastError());
-  std::vector<float> h(2 * (size_t)L->nprobes);
+  std::vector<float> h(4 * (size_t)L->nprobes);
   F3R_HIP(hipMemcpyAsync(h.data(), norms, 4 * h.size(), hipMemcpyDeviceToHost, st));
   F3R_HIP(hipStreamSynchronize(st));
   double total = 0;
-  for (int k = 0; k < L->nprobes; ++k) total += (double)h[2 * k] / (double)h[2 * k + 1];
+  for (int k = 0; k < L->nprobes; ++k) {
+    const float *e = &h[4 * (size_t)k];
+    total += kind == 0 ? (double)e[0] / (double)e[1]
+                       : 1.0 - (double)e[0] * (double)e[0] / ((double)e[2] * (double)e[1]);
+  }
   return L->nprobes > 0 ? total / L->nprobes : 0.0;
 }
 
@@ -787,18 +828,19 @@ int f3r_lsys_set_probes(f3r_lsys L, int32_t k, const float *probes) {
   }
   S->probes = dev_alloc(4 * std::max<int64_t>(n * k, 1));
   if (k) F3R_HIP(hipMemcpyAsync(S->probes->p, probes, 4 * n * k, hipMemcpyHostToDevice, c->stream));
-  S->norms = dev_alloc(8 * std::max<int32_t>(k, 1));
+  S->norms = dev_alloc(16 * std::max<int32_t>(k, 1));
   S->nprobes = k;
   sync(c);
   F3R_API_END
 }
 
-int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t grad, double *loss) {
+int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t kind, int32_t grad, double *loss) {
   F3R_API_BEGIN
   F3R_CHECK(L && m && loss, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(kind == 0 || kind == 1, F3R_ERR_ARG, "lsys_loss: kind must be 0 or 1");
   F3R_CHECK(L->L->ctx == m->m->ctx, F3R_ERR_ARG, "lsys_loss: different contexts");
   F3R_CHECK(L->L->nprobes > 0, F3R_ERR_STATE, "lsys_loss: no probe vectors");
-  *loss = lsys_loss(L->L.get(), m->m.get(), grad != 0);
+  *loss = lsys_loss(L->L.get(), m->m.get(), kind, grad != 0);
   F3R_API_END
 }
 

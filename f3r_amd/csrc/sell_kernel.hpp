@@ -19,6 +19,12 @@ struct SellArgs {
   const int32_t *__restrict__ cwidth;  // slot groups of each chunk
   const int32_t *__restrict__ idx;     // column of each slot
   const int32_t *__restrict__ perm;    // sorted row -> original row (null = identity)
+  // chunk-offset format (null = every chunk explicit): iptr[s] >= 0 is the
+  // start of chunk s in idx; iptr[s] < 0: slot t of row r reads column
+  // r + coff[-iptr[s] - 1 + t], and the chunk has no column indices.  Used
+  // only with C >= 64, so that a wavefront's rows share one chunk.
+  const int32_t *__restrict__ iptr;
+  const int32_t *__restrict__ coff;
 };
 
 // Sum of row r (in sorted order) of A*x with accumulator ACC.
@@ -34,9 +40,34 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
   int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K;
   const int64_t step = (int64_t)tc * K;
   ACC acc = 0;
-  for (int g = 0; g < groups; ++g, pos += step) {
+  const int32_t *__restrict__ ip = a.idx + pos;
+  if (a.iptr) {
+    // the chunk index is uniform over the wavefront: scalar loads of the
+    // chunk header and of the offset table
+    const int su = __builtin_amdgcn_readfirstlane(s);
+    const int32_t start = a.iptr[su];
+    if (start < 0) {
+      // the offset table is read-only for the whole kernel: constant
+      // address space, so the uniform reads become scalar loads
+      typedef const __attribute__((address_space(4))) int32_t *const_i32;
+      const_i32 off = (const_i32)(a.coff + (-start - 1));
+      const X *__restrict__ xr = x + r;
+#pragma unroll 2
+      for (int g = 0; g < groups; ++g, pos += step) {
+        const Pack<V, K> pv = load_pack<V, K>(val + pos);
+        X xv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) xv[k] = xr[off[g * K + k]];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
+      }
+      return acc;
+    }
+    ip = a.idx + (int64_t)start + (int64_t)j * K;
+  }
+  for (int g = 0; g < groups; ++g, pos += step, ip += step) {
     const Pack<V, K> pv = load_pack<V, K>(val + pos);
-    const Pack<int32_t, K> pi = load_pack<int32_t, K>(a.idx + pos);
+    const Pack<int32_t, K> pi = load_pack<int32_t, K>(ip);
 #pragma unroll
     for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[pi.v[k]]), acc);
   }

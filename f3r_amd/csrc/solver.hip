@@ -781,7 +781,7 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   const int K = p.K > 0 ? p.K : 4;
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
   F3R_CHECK(K == 1 || K == 2 || K == 4, F3R_ERR_ARG, "solver: K must be 1, 2 or 4");
-  HostSell hs = build_sell(A, C, sigma, K, nullptr, true);
+  HostSell hs = build_sell(A, C, sigma, K, nullptr, true, p.index != 0);
   bool ident = true;
   for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
   S->sh = upload_sell_shared(ctx.get(), hs, true);
@@ -802,7 +802,7 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   P->acc = Pin->acc;
   P->host = Pin->host;
   F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
-  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, K);
+  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, K, p.index != 0);
   S->P = P;
   S->range_s = p.variant != 0 ? 1.f
                : p.range_scale > 0 ? (float)p.range_scale
@@ -943,6 +943,7 @@ int f3r_params_default(f3r_params *p) {
   p->use_graph = 1;
   p->flags = 1;
   p->grid = 0;
+  p->index = 1;
   return F3R_OK;
 }
 

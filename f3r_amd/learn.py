@@ -9,9 +9,11 @@ unchanged solver.  The reference has no learned component; this module has no
 reference counterpart.
 
 Training minimises the mean over probe vectors z of
-||(I - A M) z||^2 / ||z||^2; the probes are the vectors the solver feeds to its
-innermost level (F3R.capture), or random vectors.  Forward, backward, the
-loss and its gradient and Adam all run on the GPU.
+min_w ||(I - w A M) z||^2 / ||z||^2 (the residual after the best weight, which
+is what the adaptive Richardson weights leave; LearnSystem.loss kind 1) or of
+||(I - A M) z||^2 / ||z||^2 (kind 0); the probes are the vectors the solver
+feeds to its innermost level (F3R.capture), or random vectors.  Forward,
+backward, the loss and its gradient and Adam all run on the GPU.
 
 Family of systems (7-point lognormal diffusion):
     A = convdiff7(nx, nx, nx, sigma, corr, peclet=0, seed),
@@ -152,9 +154,11 @@ class LearnSystem:
         check(lib.f3r_lsys_set_probes(self.handle, z.shape[0], _p(z)), "lsys_set_probes")
         self.nprobes = z.shape[0]
 
-    def loss(self, mlp, grad=False):
+    def loss(self, mlp, grad=False, kind=1):
+        """kind 1: mean of 1 - <z,q>^2/(|z|^2 |q|^2), q = A M z (the residual
+        after the best weight); kind 0: mean of |z - q|^2 / |z|^2"""
         out = ctypes.c_double()
-        check(lib.f3r_lsys_loss(self.handle, mlp.handle, 1 if grad else 0, ctypes.byref(out)), "lsys_loss")
+        check(lib.f3r_lsys_loss(self.handle, mlp.handle, kind, 1 if grad else 0, ctypes.byref(out)), "lsys_loss")
         return out.value
 
     def scales(self, mlp):
@@ -220,7 +224,7 @@ def random_probes(n, k, seed):
     return rng.choice(np.array([-1.0, 1.0], np.float32), size=(k, n))
 
 
-def train(ctx, systems, epochs=20, lr=2e-3, seed=0, log=None):
+def train(ctx, systems, epochs=20, lr=2e-3, seed=0, log=None, kind=1):
     """Adam over the systems (one step per system, shuffled each epoch).
     `systems` are LearnSystem objects with probes.  Returns the MLP and the
     mean loss of each epoch."""
@@ -231,7 +235,7 @@ def train(ctx, systems, epochs=20, lr=2e-3, seed=0, log=None):
         tot = 0.0
         for i in rng.permutation(len(systems)):
             mlp.grad_zero()
-            tot += systems[i].loss(mlp, grad=True)
+            tot += systems[i].loss(mlp, grad=True, kind=kind)
             mlp.adam(lr)
         hist.append(tot / len(systems))
         if log:

@@ -120,6 +120,13 @@ int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz,
  * the original row order. */
 int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
                     int32_t K, f3r_mat *sell);
+/* the same with flags; bit 0: chunk-offset index format (C >= 64): a chunk
+ * whose rows share a few column offsets col - row (stencil matrices) stores
+ * them once per chunk instead of a column index per slot.  The sums keep the
+ * CSR order of every row.  No counterpart in the reference. */
+#define F3R_SELL_OFFSETS 1
+int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
+                       int32_t K, int32_t flags, f3r_mat *sell);
 /* host-only SELL-C-sigma layout of a CSR matrix (csr_to_sell /
  * csr_to_sell_sigma, sell.hpp:247-343): cptr[nchunks+1] element offsets,
  * cwidth[nchunks] slot groups, sidx / sval [nslots], perm[n] (sorted row ->
@@ -132,6 +139,8 @@ int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out);
 int f3r_mat_destroy(f3r_mat A);
 int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *dtype,
                  int *format);
+/* SELL: chunks in total and chunks stored in the chunk-offset format */
+int f3r_mat_chunks(f3r_mat A, int32_t *chunks, int32_t *offset_chunks);
 /* y = A x  (SELL::operate / CSR::operate, sell.hpp:55-71, csr.hpp:32-40).
  * Accumulates in fp64 for fp64 data, else in fp32 (fp16 x fp16 too). */
 int f3r_spmv(f3r_mat A, f3r_vec x, f3r_vec y);
@@ -173,6 +182,8 @@ typedef struct {
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = one row per thread */
+  int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
+                          column offsets store the offsets once (default; needs C >= 64) */
 } f3r_params;
 
 typedef struct {
@@ -223,8 +234,8 @@ int f3r_solver_captured(f3r_solver S, int32_t *count, float *out);
  * A per-row MLP 12 -> 64 -> 64 -> 1 (ReLU; bf16 weights, fp32 accumulation
  * on MFMA) maps row features to y_i; the corrected preconditioner is
  * M = Z diag(s) W^T with s_i = exp(clamp(y_i, -3, 3)), SDAINV's factors
- * (ainv.hpp:57-64) with the columns of Z rescaled.  Training minimises the
- * mean over probe vectors z of ||(I - A M) z||^2 / ||z||^2. */
+ * (ainv.hpp:57-64) with the columns of Z rescaled.  Training minimises a
+ * residual loss over probe vectors (f3r_lsys_loss). */
 #define F3R_MLP_NIN 16   /* feature columns per row; columns 12..15 are zero */
 #define F3R_MLP_NFEAT 12
 typedef struct f3r_mlp_s *f3r_mlp;
@@ -253,8 +264,12 @@ int f3r_lsys_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const float *feat, f3r_l
 int f3r_lsys_destroy(f3r_lsys L);
 /* k probe vectors, k x n host floats */
 int f3r_lsys_set_probes(f3r_lsys L, int32_t k, const float *probes);
-/* the loss; with grad != 0 its gradient is added to the MLP's */
-int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t grad, double *loss);
+/* the mean over the probes z of, with q = A M z,
+ *   kind 0: ||z - q||^2 / ||z||^2
+ *   kind 1: min_w ||z - w q||^2 / ||z||^2 = 1 - <z,q>^2 / (<z,z><q,q>)
+ * (kind 1 ignores a global scale of M, which the adaptive Richardson weights
+ * absorb); with grad != 0 its gradient is added to the MLP's */
+int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t kind, int32_t grad, double *loss);
 /* s = exp(clamp(MLP(features))) for every row */
 int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, float *s);
 /* a new SDAINV preconditioner with Z's column i scaled by s_i (s > 0) */

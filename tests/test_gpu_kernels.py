@@ -17,7 +17,9 @@ COMBOS = [  # (value, x, y) precisions; sums in fp64 for fp64 data, else fp32
     ("f16", "f16", "f16", 2, 2, 2, 1),
     ("f16", "f16", "f32", 2, 2, 1, 1),
 ]
-LAYOUTS = [(32, 1, 1), (64, 1, 4), (64, 256, 2), (128, 64, 4), (16, 1, 4)]
+# (C, sigma, K, chunk offsets)
+LAYOUTS = [(32, 1, 1, False), (64, 1, 4, False), (64, 256, 2, False), (128, 64, 4, False), (16, 1, 4, False),
+           (64, 1, 4, True), (128, 1, 1, True), (256, 1, 2, True), (64, 256, 4, True)]
 NP = {"f64": np.float64, "f32": np.float32, "f16": np.float16}
 
 
@@ -41,15 +43,22 @@ def _mats():
     yield "convdiff", B
 
 
-@pytest.mark.parametrize("C,sigma,K", LAYOUTS)
-def test_sell_spmv_bitexact(ctx, C, sigma, K):
+@pytest.mark.parametrize("C,sigma,K,offsets", LAYOUTS)
+def test_sell_spmv_bitexact(ctx, C, sigma, K, offsets):
     import f3r_amd as F
     for name, A in _mats():
         x = np.random.default_rng(1).uniform(-1, 1, A.n)
         b = np.random.default_rng(2).uniform(-1, 1, A.n)
         for vd, xd, yd, vp, xp, yp, ap in COMBOS:
-            print(f"spmv {name} {vd}/{xd}/{yd} C={C} sigma={sigma} K={K}", flush=True)
-            S = F.SELL(ctx, A, vd, C=C, sigma=sigma, K=K)
+            print(f"spmv {name} {vd}/{xd}/{yd} C={C} sigma={sigma} K={K} offsets={offsets}", flush=True)
+            S = F.SELL(ctx, A, vd, C=C, sigma=sigma, K=K, offsets=offsets)
+            nch, noff = S.chunks()
+            if offsets and sigma == 1 and name != "ragged":
+                # stencils: every chunk but the first and last few (whose
+                # offsets would leave the matrix) uses offsets
+                assert noff > 0 and noff >= nch - 4, (name, nch, noff)
+            if not offsets:
+                assert noff == 0
             xv = F.vector.from_numpy(ctx, x.astype(NP[xd]))
             yv = F.vector(ctx, yd, A.n)
             S.operate(xv, yv)

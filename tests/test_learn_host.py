@@ -135,3 +135,13 @@ def test_family_params_in_range():
         assert 0.5 <= s <= 2.0 and 2.0 <= c <= 16.0
     assert learn.family_params(3) == learn.family_params(3)
     assert learn.family_params(3) != learn.family_params(4)
+
+
+def test_batch_family():
+    from f3r_amd import batch
+    fam = batch.family(0, 64)
+    assert len(fam) == 64
+    assert sum(1 for f in fam if f[1] == 7) == 32 and sum(1 for f in fam if f[1] == 27) == 32
+    assert all(16 <= f[2] <= 64 and 0.5 <= f[3] <= 2.0 for f in fam)
+    assert len({f[2] for f in fam}) > 10
+    assert batch.family(0, 64) == fam and batch.family(1, 64) != fam
