@@ -108,7 +108,7 @@ SIGNATURES = [
     ("f3r_lsys_destroy", ctypes.c_int, [c_vp]),
     ("f3r_lsys_set_probes", ctypes.c_int, [c_vp, c_i32, c_vp]),
     ("f3r_lsys_loss", ctypes.c_int, [c_vp, c_vp, c_i32, c_i32, ctypes.POINTER(c_dbl)]),
-    ("f3r_lsys_scales", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("f3r_lsys_scales", ctypes.c_int, [c_vp, c_vp, c_i32, c_vp]),
     ("f3r_sdainv_rescale", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
 ]
 

@@ -761,7 +761,7 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   F3R_API_BEGIN
   F3R_CHECK(ctx && csr && out && csr->m->format == 0, F3R_ERR_ARG, "to_sell: needs a context and a CSR matrix");
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "to_sell: C must be a power of two >= 16");
-  F3R_CHECK(K == 1 || K == 2 || K == 4, F3R_ERR_ARG, "to_sell: K must be 1, 2 or 4");
+  F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "to_sell: K must be 1, 2, 4 or 8");
   F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
   F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
   *out = nullptr;

@@ -415,7 +415,7 @@ struct LSys {
 };
 
 static LSys::M upload_f32(Context *c, const HostCSR &M) {
-  HostSell hs = build_sell(M, 64, 1, 1, nullptr, false);
+  HostSell hs = build_sell(M, 64, 1, 1, nullptr, false, true);
   LSys::M m;
   m.sh = upload_sell_shared(c, hs, false);
   m.val = upload_sell_values(c, hs, F3R_F32);
@@ -844,7 +844,7 @@ int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t kind, int32_t grad, double *los
   F3R_API_END
 }
 
-int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, float *s) {
+int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, int32_t normalize, float *s) {
   F3R_API_BEGIN
   F3R_CHECK(L && m && s, F3R_ERR_ARG, "null argument");
   F3R_CHECK(L->L->ctx == m->m->ctx, F3R_ERR_ARG, "lsys_scales: different contexts");
@@ -857,6 +857,12 @@ int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, float *s) {
   F3R_HIP(hipGetLastError());
   F3R_HIP(hipMemcpyAsync(s, S->s->p, 4 * n, hipMemcpyDeviceToHost, c->stream));
   sync(c);
+  if (normalize) {  // divide by the geometric mean: the loss of kind 1 cannot see a global scale
+    double lm = 0;
+    for (int64_t i = 0; i < n; ++i) lm += std::log((double)s[i]);
+    const double g = std::exp(lm / (double)n);
+    for (int64_t i = 0; i < n; ++i) s[i] = (float)((double)s[i] / g);
+  }
   F3R_API_END
 }
 

@@ -10,6 +10,14 @@
 
 #include "device.hpp"
 
+// groups of K slots of an offset chunk issued back to back (memory-level
+// parallelism of one lane)
+#ifndef F3R_OFF_UNROLL
+#define F3R_OFF_UNROLL 2
+#endif
+#define F3R_PRAGMA_(x) _Pragma(#x)
+#define F3R_PRAGMA(x) F3R_PRAGMA_(x)
+
 namespace f3r {
 
 struct SellArgs {
@@ -52,7 +60,7 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
       typedef const __attribute__((address_space(4))) int32_t *const_i32;
       const_i32 off = (const_i32)(a.coff + (-start - 1));
       const X *__restrict__ xr = x + r;
-#pragma unroll 2
+      F3R_PRAGMA(unroll F3R_OFF_UNROLL)
       for (int g = 0; g < groups; ++g, pos += step) {
         const Pack<V, K> pv = load_pack<V, K>(val + pos);
         X xv[K];
@@ -101,6 +109,7 @@ inline void launch_sell(int K, const SellArgs &a, const V *val, const X *x, Epi 
     case 1: hipLaunchKernelGGL((k_sell<V, X, ACC, 1, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
     case 2: hipLaunchKernelGGL((k_sell<V, X, ACC, 2, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
     case 4: hipLaunchKernelGGL((k_sell<V, X, ACC, 4, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+    case 8: hipLaunchKernelGGL((k_sell<V, X, ACC, 8, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
     default: break;
   }
 }

@@ -496,8 +496,11 @@ struct Solver {
       spmv_dots<T, V, 16>(L, val, j);
       cgs_update<T, 16>(L, j);
     }
-    hipLaunchKernelGGL(k_vscale<T>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
-                       (const T *)L.inv->template as<T>());
+    // the last basis vector of an inner level is never read (the level's
+    // output is Z y), so it is left unnormalised
+    if (j + 1 < L.m)
+      hipLaunchKernelGGL(k_vscale<T>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
+                         (const T *)L.inv->template as<T>());
   }
   template <class T, class TO> void solution(InnerLevel<T> &L, TO *out) {
     const int j = L.m;
@@ -780,7 +783,7 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   const int sigma = p.sigma > 0 ? p.sigma : 1;
   const int K = p.K > 0 ? p.K : 4;
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
-  F3R_CHECK(K == 1 || K == 2 || K == 4, F3R_ERR_ARG, "solver: K must be 1, 2 or 4");
+  F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "solver: K must be 1, 2, 4 or 8");
   HostSell hs = build_sell(A, C, sigma, K, nullptr, true, p.index != 0);
   bool ident = true;
   for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
@@ -809,8 +812,10 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
                                    : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
   const int64_t n = A.n;
   S->red_grid = reduce_grid(n, ctx->num_cu);
-  S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxReduceGrid);
-  if (p.grid > 0) S->dots_grid = std::min(S->dots_grid, (int)p.grid);
+  // fused SpMV + reduction kernels: eight blocks per CU (each arrival at a
+  // grid reduction costs an atomic round trip, so fewer, longer blocks)
+  S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, std::min(kMaxReduceGrid, 8 * ctx->num_cu));
+  if (p.grid > 0) S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, std::min((int)p.grid, kMaxReduceGrid));
   auto dd = [](int64_t cnt) { return dev_alloc(sizeof(double) * std::max<int64_t>(cnt, 1)); };
   S->V1 = dd(n * (p.m1 + 1));
   S->Z1 = dd(n * p.m1);

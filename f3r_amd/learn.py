@@ -161,9 +161,11 @@ class LearnSystem:
         check(lib.f3r_lsys_loss(self.handle, mlp.handle, kind, 1 if grad else 0, ctypes.byref(out)), "lsys_loss")
         return out.value
 
-    def scales(self, mlp):
+    def scales(self, mlp, normalize=True):
+        """s = exp(clamp(MLP(features))), divided by its geometric mean unless
+        normalize=False (the scale-blind loss leaves the global scale free)"""
         s = np.empty(self.n, np.float32)
-        check(lib.f3r_lsys_scales(self.handle, mlp.handle, _p(s)), "lsys_scales")
+        check(lib.f3r_lsys_scales(self.handle, mlp.handle, 1 if normalize else 0, _p(s)), "lsys_scales")
         return s
 
     def __del__(self):
@@ -207,11 +209,14 @@ def rhs(n, seed):
 
 def solver_probes(ctx, A, P, k=8, every=4, seed=0):
     """k vectors the solver feeds its innermost level during one level-2
-    application to a right-hand side (the plain SDAINV preconditioner)"""
+    application to the unit right-hand side (the plain SDAINV
+    preconditioner); level 2 expects a unit-norm input, as the outer Krylov
+    vectors it is applied to are"""
     Pd = Rescaled(P, np.ones(A.n, np.float32), ctx)
     S = F3R(ctx, A, Pd, use_graph=False)
     S.capture(k, every)
-    b = vector.from_numpy(ctx, rhs(A.n, seed))
+    bh = rhs(A.n, seed)
+    b = vector.from_numpy(ctx, bh / np.linalg.norm(bh))
     out = vector(ctx, "f64", A.n)
     S.level_apply(2, b, out)
     Z = S.captured()
@@ -243,7 +248,7 @@ def train(ctx, systems, epochs=20, lr=2e-3, seed=0, log=None, kind=1):
     return mlp, hist
 
 
-def evaluate(ctx, mlp, hs, reps=1):
+def evaluate(ctx, mlp, hs, reps=1, normalize=True):
     """plain vs learned F3R on one host system `hs`: outer steps, solve
     times and true residuals"""
     A, P = hs["A"], hs["P"]
@@ -251,7 +256,7 @@ def evaluate(ctx, mlp, hs, reps=1):
     b = vector.from_numpy(ctx, rhs(n, hs["seed"]))
     out = {}
     L = LearnSystem(ctx, A, P, hs["feat"])
-    s = L.scales(mlp)
+    s = L.scales(mlp, normalize)
     del L
     for name, prec in (("plain", Rescaled(P, np.ones(n, np.float32), ctx)), ("learned", Rescaled(P, s, ctx))):
         S = F3R(ctx, A, prec)
@@ -310,22 +315,37 @@ def main(argv=None):
             L.set_probes(solver_probes(ctx, hs["A"], hs["P"], k=a.probes, seed=hs["seed"]))
             systems.append(L)
         log(f"{len(systems)} training systems ready")
+    m0 = MLP(ctx, 0)
+    base = float(np.mean([L.loss(m0) for L in systems]))
+    log(f"plain SDAINV loss {base:.6f}")
     mlp, hist = train(ctx, systems, a.epochs, a.lr, log=log)
-    np.savez(a.out, params=mlp.get(), loss=np.array(hist), nx=a.nx, train=a.train)
+    np.savez(a.out, params=mlp.get(), loss=np.array(hist), base_loss=base, nx=a.nx, train=a.train,
+             probes=a.probes, epochs=a.epochs)
     log(f"parameters saved to {a.out}")
     del systems
     rep = []
     test_seeds = range(100000, 100000 + a.test)
     for c0 in range(0, a.test, chunk):
         for hs in host_systems(a.nx, list(test_seeds)[c0:c0 + chunk], a.threads):
-            r = evaluate(ctx, mlp, hs)
+            r = evaluate(ctx, mlp, hs, reps=3)
             r.update(seed=hs["seed"], sigma=hs["sigma"], corr=hs["corr"])
             rep.append(r)
             log(f"held-out {hs['seed']}: plain {r['plain']['outer']} steps {r['plain']['solve_s'] * 1e3:.1f} ms, "
                 f"learned {r['learned']['outer']} steps {r['learned']['solve_s'] * 1e3:.1f} ms")
+    ps = float(np.mean([r["plain"]["solve_s"] for r in rep]))
+    ls = float(np.mean([r["learned"]["solve_s"] for r in rep]))
+    po = float(np.mean([r["plain"]["outer"] for r in rep]))
+    lo = float(np.mean([r["learned"]["outer"] for r in rep]))
+    ok = all(r[k]["solved"] and r[k]["true_rel_res"] <= 1e-8 for r in rep for k in ("plain", "learned"))
+    summary = {"train": a.train, "test": a.test, "nx": a.nx, "base_loss": base, "final_loss": hist[-1],
+               "plain_mean_s": ps, "learned_mean_s": ls, "speedup": ps / ls, "plain_mean_outer": po,
+               "learned_mean_outer": lo, "all_solved": ok}
+    log(f"held-out mean: plain {ps * 1e3:.2f} ms ({po:.2f} outer steps), learned {ls * 1e3:.2f} ms "
+        f"({lo:.2f}); speedup {ps / ls:.4f}; all solved to 1e-8: {ok}")
     if a.report:
         with open(a.report, "w") as f:
-            json.dump({"loss": hist, "held_out": rep}, f, indent=1)
+            json.dump({"summary": summary, "loss": hist, "held_out": rep}, f, indent=1)
+    print(json.dumps(summary), flush=True)
     return 0
 
 

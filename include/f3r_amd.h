@@ -115,7 +115,7 @@ int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz,
                        f3r_mat *A);
 /* SELL<w, T, device>(CSR) (sell.hpp:35-97) and SELLsig (sell.hpp:99-166):
  * C rows per chunk (multiple of 16), sigma = sort window (1 = unsorted),
- * K = slots interleaved per row (1, 2 or 4; K = 1 is the reference layout).
+ * K = slots interleaved per row (1, 2, 4 or 8; K = 1 is the reference layout).
  * Rows of the result are in sorted order; y of f3r_spmv is written back to
  * the original row order. */
 int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
@@ -181,7 +181,7 @@ typedef struct {
   int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = autotuned) */
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history */
-  int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = one row per thread */
+  int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */
 } f3r_params;
@@ -270,8 +270,9 @@ int f3r_lsys_set_probes(f3r_lsys L, int32_t k, const float *probes);
  * (kind 1 ignores a global scale of M, which the adaptive Richardson weights
  * absorb); with grad != 0 its gradient is added to the MLP's */
 int f3r_lsys_loss(f3r_lsys L, f3r_mlp m, int32_t kind, int32_t grad, double *loss);
-/* s = exp(clamp(MLP(features))) for every row */
-int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, float *s);
+/* s = exp(clamp(MLP(features))) for every row; with normalize != 0 divided
+ * by its geometric mean (for models trained with the scale-blind loss) */
+int f3r_lsys_scales(f3r_lsys L, f3r_mlp m, int32_t normalize, float *s);
 /* a new SDAINV preconditioner with Z's column i scaled by s_i (s > 0) */
 int f3r_sdainv_rescale(f3r_ctx ctx, f3r_prec P, const float *s, f3r_prec *out);
 

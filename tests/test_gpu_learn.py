@@ -190,8 +190,10 @@ def test_lsys_loss_and_gradient(ctx):
     m = learn.MLP(ctx, seed=0, params=p)
     y = m.forward(feat).astype(np.float64)
     assert np.abs(y).max() < 2.5  # away from the clamp
-    s_gpu = L.scales(m)
+    s_gpu = L.scales(m, normalize=False)
     assert np.allclose(s_gpu, np.exp(y), rtol=1e-5)
+    s_n = L.scales(m)
+    assert np.allclose(s_n, np.exp(y - y.mean()), rtol=1e-4)
     # fp64 reference of both losses and of dL/dy
     Am = _dense(n, A.rptr, A.idx, A.val)
     (zr, zi, zv), (wr, wi, wv) = P.factors()
