@@ -109,6 +109,13 @@ struct SellShared {  // index data shared by all precisions of one matrix
   DevPtr cptr, cwidth, idx, perm;  // perm null when identity
   DevPtr iptr, coff;               // chunk-offset format (null when not used)
   int32_t offset_chunks = 0;
+  int64_t offset_slots = 0, explicit_nnz = 0;
+  int32_t off_bias = 0;
+  // bytes one SpMV must move: values of offset chunks, values + int32 index
+  // of the other nonzeros, x and y
+  double format_bytes(int sv, int sx, int sy) const {
+    return (double)offset_slots * sv + (double)explicit_nnz * (sv + 4) + (double)n * (sx + sy);
+  }
   std::vector<int32_t> h_perm;     // host copy of the permutation
 };
 
@@ -138,6 +145,9 @@ struct HostSell {
   // offset chunk whose slot t of row r reads column r + coff[-iptr[s] - 1 + t]
   std::vector<int32_t> iptr, coff;
   int32_t offset_chunks = 0;
+  int64_t offset_slots = 0;  // value slots of offset chunks (read in full)
+  int32_t off_bias = 0;      // -(smallest offset of any offset chunk), >= 0
+  int64_t explicit_nnz = 0;  // nonzeros of int32-index chunks
 };
 // Builds the layout; when `renumber` the columns are mapped through the
 // inverse permutation (the matrix P A P^T in sorted order).  With `offsets`

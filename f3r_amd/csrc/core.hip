@@ -267,6 +267,7 @@ SellArgs sell_args(const SellShared &sh, bool with_perm) {
   a.perm = (with_perm && sh.perm) ? sh.perm->as<int32_t>() : nullptr;
   a.iptr = sh.iptr ? sh.iptr->as<int32_t>() : nullptr;
   a.coff = sh.coff ? sh.coff->as<int32_t>() : nullptr;
+  a.off_bias = sh.off_bias;
   return a;
 }
 
@@ -278,6 +279,9 @@ std::shared_ptr<SellShared> upload_sell_shared(Context *ctx, const HostSell &hs,
   sh->nchunks = hs.nchunks;
   sh->nslots = (int64_t)hs.val.size();
   sh->offset_chunks = hs.offset_chunks;
+  sh->offset_slots = hs.offset_slots;
+  sh->explicit_nnz = hs.explicit_nnz;
+  sh->off_bias = hs.off_bias;
   sh->internal = internal;
   hipStream_t st = ctx->stream;
   sh->cptr = dev_alloc(sizeof(int32_t) * hs.cptr.size());

@@ -75,8 +75,9 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
     for (int32_t r = 0; r < n; ++r) inv[S.perm[r]] = r;
   }
   auto col = [&](int32_t k) { return renumber ? inv[A.idx[k]] : A.idx[k]; };
-  // a wavefront of 64 rows must lie in one chunk for the offset format
-  offsets = offsets && C >= 64;
+  // a wavefront of 64 rows must lie in one chunk for the offset format, and
+  // the x gathers address (n + offset range) elements with 32-bit byte offsets
+  offsets = offsets && C >= 64 && n < (1 << 27);
   S.nchunks = (n + C - 1) / C;
   S.cptr.assign(S.nchunks + 1, 0);
   S.cwidth.assign(S.nchunks, 0);
@@ -132,6 +133,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
       const int64_t w = (int64_t)S.cwidth[s] * K;
       if (!uni[s].empty()) {
         S.iptr[s] = -(int32_t)S.coff.size() - 1;
+        S.off_bias = std::max(S.off_bias, -uni[s].front());
         for (int64_t t = 0; t < w; ++t) S.coff.push_back(t < (int64_t)uni[s].size() ? uni[s][t] : 0);
         ++S.offset_chunks;
       } else {
@@ -151,6 +153,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
       // offset chunk: the value of column r + uni[t] goes to slot t, the
       // other slots stay 0 (their columns r + uni[t] are in range)
       const std::vector<int32_t> &us = uni[s];
+      S.offset_slots += (int64_t)width * tc;
       for (int32_t j = 0; j < tc; ++j) {
         const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
         for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
@@ -165,6 +168,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
     for (int32_t j = 0; j < tc; ++j) {
       int32_t o = S.perm[r0 + j];
       int32_t len = A.rptr[o + 1] - A.rptr[o];
+      S.explicit_nnz += len;
       int32_t last = 0;
       for (int32_t q = 0; q < width; ++q) {
         int64_t pos = S.cptr[s] + ((int64_t)(q / K) * tc + j) * K + (q % K);

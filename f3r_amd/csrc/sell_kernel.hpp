@@ -17,6 +17,13 @@
 #endif
 #define F3R_PRAGMA_(x) _Pragma(#x)
 #define F3R_PRAGMA(x) F3R_PRAGMA_(x)
+// x gathers of offset chunks through a buffer resource: the per-lane byte
+// offset r * sizeof(X) is fixed for the row and the chunk's column offset
+// rides in the scalar offset, so a gather is one instruction without
+// per-lane address arithmetic
+#ifndef F3R_X_BUFFER
+#define F3R_X_BUFFER 1
+#endif
 
 namespace f3r {
 
@@ -33,7 +40,22 @@ struct SellArgs {
   // only with C >= 64, so that a wavefront's rows share one chunk.
   const int32_t *__restrict__ iptr;
   const int32_t *__restrict__ coff;
+  int32_t off_bias;  // -(smallest column offset), >= 0: makes every scalar offset non-negative
 };
+
+// word 3 of a raw buffer resource on gfx9-family GPUs (no format, no swizzle)
+constexpr int kBufferWord3 = 0x00020000;
+
+template <class X>
+__device__ __forceinline__ X buffer_gather(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  if constexpr (sizeof(X) == 2) {
+    return __builtin_bit_cast(X, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff, 0));
+  } else if constexpr (sizeof(X) == 4) {
+    return __builtin_bit_cast(X, (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(X, (unsigned long long)__builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+  }
+}
 
 // Sum of row r (in sorted order) of A*x with accumulator ACC.
 template <class V, class X, class ACC, int K>
@@ -59,6 +81,24 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
       // address space, so the uniform reads become scalar loads
       typedef const __attribute__((address_space(4))) int32_t *const_i32;
       const_i32 off = (const_i32)(a.coff + (-start - 1));
+#if F3R_X_BUFFER
+      // base x - off_bias, so that row + offset = (r) + (offset + off_bias) - off_bias
+      // with a non-negative scalar part; out-of-range reads would return 0
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)((uintptr_t)x - (uintptr_t)a.off_bias * sizeof(X)), (short)0,
+          (int)(((int64_t)a.n + a.off_bias) * (int64_t)sizeof(X)), kBufferWord3);
+      const int vo = r * (int)sizeof(X);
+      const int bias = a.off_bias;
+      F3R_PRAGMA(unroll F3R_OFF_UNROLL)
+      for (int g = 0; g < groups; ++g, pos += step) {
+        const Pack<V, K> pv = load_pack<V, K>(val + pos);
+        X xv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) xv[k] = buffer_gather<X>(rs, vo, (off[g * K + k] + bias) * (int)sizeof(X));
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
+      }
+#else
       const X *__restrict__ xr = x + r;
       F3R_PRAGMA(unroll F3R_OFF_UNROLL)
       for (int g = 0; g < groups; ++g, pos += step) {
@@ -69,6 +109,7 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
 #pragma unroll
         for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
       }
+#endif
       return acc;
     }
     ip = a.idx + (int64_t)start + (int64_t)j * K;

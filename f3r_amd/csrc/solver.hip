@@ -1060,7 +1060,6 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
   F3R_API_BEGIN
   F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 3, F3R_ERR_ARG, "time_kernel: bad argument");
   Solver *s = S->s;
-  const int64_t n = s->n, nnz = s->nnz;
   SellArgs a = s->args();
   F3R_CHECK(s->p.variant == 0 || which == 3, F3R_ERR_UNSUPPORTED, "time_kernel: fp16/fp32 kernels need the HALF variant");
   half_t *x16 = s->hout->as<half_t>(), *y16 = s->tt->as<half_t>();
@@ -1072,22 +1071,22 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
       case 0:  // fp16 values, fp16 x and y (the metric's fp16 SpMV)
         launch_sell<half_t, half_t, float>(s->sh->K, a, s->A16->as<half_t>(), x16,
                                            EpiStore<half_t, float>{y16, nullptr}, s->st());
-        by = nnz * 6.0 + n * 4.0;
+        by = s->sh->format_bytes(2, 2, 2);
         break;
       case 1:  // fp16 values, fp32 x and y (level 3)
         launch_sell<half_t, float, float>(s->sh->K, a, s->A16->as<half_t>(), x32,
                                           EpiStore<float, float>{y32, nullptr}, s->st());
-        by = nnz * 6.0 + n * 8.0;
+        by = s->sh->format_bytes(2, 4, 4);
         break;
       case 2:  // fp32 (level 2)
         launch_sell<float, float, float>(s->sh->K, a, s->A32->as<float>(), x32,
                                          EpiStore<float, float>{y32, nullptr}, s->st());
-        by = nnz * 8.0 + n * 8.0;
+        by = s->sh->format_bytes(4, 4, 4);
         break;
       case 3:  // fp64 (outer level)
         launch_sell<double, double, double>(s->sh->K, a, s->A64->as<double>(), x64,
                                             EpiStore<double, double>{y64, nullptr}, s->st());
-        by = nnz * 12.0 + n * 16.0;
+        by = s->sh->format_bytes(8, 8, 8);
         break;
     }
   };

@@ -215,8 +215,11 @@ int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len);
 /* one application of level 2, 3 or 4 as an operator on fp64 vectors, for the
  * per-level error study */
 int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out);
-/* mean device time per launch of the solver's fp16 residual SpMV, measured
- * with HIP events on the solver's stream over `reps` launches */
+/* mean device time per launch of one of the solver's SpMVs (which: 0 fp16
+ * values and vectors, 1 fp16 values / fp32 vectors, 2 fp32, 3 fp64), measured
+ * with HIP events on the solver's stream over `reps` launches; `bytes` = the
+ * bytes the layout must move per launch (values and int32 indices of the
+ * nonzeros of int32-index chunks, every value slot of chunk-offset chunks, x, y) */
 int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms,
                            double *bytes);
 int f3r_solver_destroy(f3r_solver S);
