@@ -271,12 +271,20 @@ __global__ void k_scatter(int64_t n, const double *__restrict__ s, const int32_t
 // Entry of one call (work/src/f3r-gpu.cpp:231-243 in this build's
 // arithmetic): hin = fp16(in * s); with diagonal SDAINV factors also
 // hout = fp16(z * fp16(w * hin)) and, outside probe calls, hout *= ome_0.
-__global__ void k_l4_entry(int64_t n, const float *__restrict__ in, float s, half_t *__restrict__ hin,
-                           half_t *__restrict__ hout, const half_t *__restrict__ dw,
+// `inv` (optional): `in` is a basis vector not yet normalised; it is scaled
+// by *inv in place first (the normalisation of the Arnoldi step that made it)
+__global__ void k_l4_entry(int64_t n, float *__restrict__ in, const float *__restrict__ inv, float s,
+                           half_t *__restrict__ hin, half_t *__restrict__ hout, const half_t *__restrict__ dw,
                            const half_t *__restrict__ dz, const RichDev *st, int scale_ome) {
   const float om = scale_ome ? (float)st->ome[0] : 1.f;
+  const float iv = inv ? *inv : 1.f;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const half_t h = cvt<half_t>(in[i] * s);
+    float v = in[i];
+    if (inv) {
+      v *= iv;
+      in[i] = v;
+    }
+    const half_t h = cvt<half_t>(v * s);
     hin[i] = h;
     if (dw) {
       const half_t t = cvt<half_t>(__builtin_fmaf((float)dw[i], (float)h, 0.f));
@@ -439,6 +447,9 @@ struct Solver {
   std::vector<double> hist;
   std::map<uint64_t, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_src;
+  // 1/||w|| of a basis vector left unnormalised by the last Arnoldi step;
+  // the next innermost entry kernel (its first reader) applies it
+  const float *pending_inv = nullptr;
   int red_grid = 1;   // capped grid of the outer level's strided reductions
   int dots_grid = 1;  // one row per thread for the fused reductions
   // capture of level-4 inputs (fp32, internal row order): every cap_every-th
@@ -497,10 +508,20 @@ struct Solver {
       cgs_update<T, 16>(L, j);
     }
     // the last basis vector of an inner level is never read (the level's
-    // output is Z y), so it is left unnormalised
-    if (j + 1 < L.m)
+    // output is Z y), so it is left unnormalised.  In the HALF variant the
+    // first reader of V[j+1] is the next innermost entry kernel (level 3's
+    // next step, or level 3's start when this is level 2), which normalises
+    // it in passing.
+    if (j + 1 < L.m) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (p.variant == 0) {
+          pending_inv = L.inv->template as<float>();
+          return;
+        }
+      }
       hipLaunchKernelGGL(k_vscale<T>, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, L.V[j + 1],
                          (const T *)L.inv->template as<T>());
+    }
   }
   template <class T, class TO> void solution(InnerLevel<T> &L, TO *out) {
     const int j = L.m;
@@ -531,9 +552,11 @@ struct Solver {
     const half_t *dz = diag ? P->dz->as<half_t>() : nullptr;
     const half_t *a16 = A16->as<half_t>();
     SellArgs a = args();
-    // entry
-    hipLaunchKernelGGL(k_l4_entry, g, b, 0, st(), nn, in, range_s, hi, ho, dw, dz, (const RichDev *)rs,
-                       (diag && !probe) ? 1 : 0);
+    // entry (normalising `in` first when the Arnoldi step that made it did not)
+    const float *pinv = pending_inv;
+    pending_inv = nullptr;
+    hipLaunchKernelGGL(k_l4_entry, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw, dz,
+                       (const RichDev *)rs, (diag && !probe) ? 1 : 0);
     if (!diag) {
       prec_apply_raw(P.get(), hi, ho, st());
       if (!probe) hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->ome[0]);
