@@ -51,6 +51,19 @@ def cpu_baseline(A, b, outer_gpu, scale):
                       f"{threads} OpenMP threads; setup excluded"}
 
 
+def measured_traffic(us_per_launch):
+    """HBM bytes per launch of the fp16 SpMV from the committed hardware-counter
+    summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a
+    copy kernel of known size); None when absent"""
+    path = os.path.join(ROOT, "profiles", "spmv_f16_c2_128_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes": t["hbm_bytes_per_launch"], "fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
+            "copy_scale": t["copy_scale"], "source": "profiles/spmv_f16_c2_128_traffic.md"}
+
+
 def learned_vs_plain(F, ctx, count, nx=96, reps=3):
     """plain vs learned F3R on `count` held-out systems of the learned
     correction's family (seeds 100000...), best of `reps` solves each"""
@@ -188,12 +201,17 @@ def main():
                    "setup_s": round(setup_s, 2), "graph": not args.no_graph},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "traffic_detail": None,
                      "kernel": "fp16 SELL SpMV y = A16 x (fp16 x, y; fp32 sums), chunk-offset layout "
                                "(C=64, K=4): bytes are those of the layout (values; no column indices)",
                      "us_per_launch": round(ms16 * 1e3, 2), "algorithmic_bytes": by16,
                      "int32_index_equivalent_GBs": round((6.0 * A.nnz + 4.0 * A.n) / (ms16 * 1e-3) / 1e9, 1)},
         "kernels": kern,
     }
+    tr = measured_traffic(ms16 * 1e3)
+    if tr:
+        out["roofline"]["traffic"] = tr["bytes"]
+        out["roofline"]["traffic_detail"] = tr
     if args.learned > 0:
         out["learned"] = learned_vs_plain(F, ctx, args.learned)
     if not args.no_cpu:

@@ -249,6 +249,24 @@ class algorithm:
         return A
 
 
+def sell_offset_layout(A, C=64, K=4):
+    """host chunk-offset layout of CSR A (sigma = 1): dict of cptr, cwidth,
+    iptr, idx, coff, val"""
+    ns, ni, nc = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    args = (A.n, _p(A.rptr), _p(A.idx), _p(A.val), C, K)
+    check(lib.f3r_sell_offset_layout(*args, ctypes.byref(ns), ctypes.byref(ni), ctypes.byref(nc),
+                                     None, None, None, None, None, None), "sell_offset_layout")
+    nch = (A.n + C - 1) // C
+    out = {"cptr": np.empty(nch + 1, np.int32), "cwidth": np.empty(nch, np.int32), "iptr": np.empty(nch, np.int32),
+           "idx": np.empty(max(ni.value, 1), np.int32), "coff": np.empty(max(nc.value, 1), np.int32),
+           "val": np.empty(max(ns.value, 1), np.float64)}
+    check(lib.f3r_sell_offset_layout(*args, ctypes.byref(ns), ctypes.byref(ni), ctypes.byref(nc),
+                                     *[_p(out[k]) for k in ("cptr", "cwidth", "iptr", "idx", "coff", "val")]),
+          "sell_offset_layout")
+    out["idx"], out["coff"], out["val"] = out["idx"][:ni.value], out["coff"][:nc.value], out["val"][:ns.value]
+    return out
+
+
 def sell_layout(A, C=32, sigma=1, K=1):
     """Host SELL-C-sigma layout (cptr, cwidth, idx, val, perm) of a CSR."""
     ns = ctypes.c_int64()

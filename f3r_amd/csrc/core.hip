@@ -803,6 +803,33 @@ int f3r_sell_layout(int32_t n, const int32_t *rptr, const int32_t *col, const do
   F3R_API_END
 }
 
+int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val, int32_t C,
+                           int32_t K, int64_t *nslots, int64_t *nidx, int64_t *ncoff, int32_t *cptr, int32_t *cwidth,
+                           int32_t *iptr, int32_t *idx, int32_t *coff, double *sval) {
+  F3R_API_BEGIN
+  F3R_CHECK(n >= 0 && rptr && nslots && nidx && ncoff && C >= 64 && (C & (C - 1)) == 0 && K >= 1, F3R_ERR_ARG,
+            "sell_offset_layout: bad argument (C a power of two >= 64)");
+  HostCSR h = host_csr_from(n, n, rptr[n], rptr, col, val);
+  HostSell hs = build_sell(h, C, 1, K, nullptr, false, true);
+  *nslots = (int64_t)hs.val.size();
+  *nidx = (int64_t)hs.idx.size();
+  *ncoff = (int64_t)hs.coff.size();
+  if (cptr) {
+    F3R_CHECK(cwidth && iptr && (hs.idx.empty() || idx) && (hs.coff.empty() || coff) && (hs.val.empty() || sval),
+              F3R_ERR_ARG, "sell_offset_layout: null output");
+    std::memcpy(cptr, hs.cptr.data(), sizeof(int32_t) * hs.cptr.size());
+    std::memcpy(cwidth, hs.cwidth.data(), sizeof(int32_t) * hs.cwidth.size());
+    if (hs.iptr.empty())
+      for (int32_t s = 0; s < hs.nchunks; ++s) iptr[s] = hs.cptr[s];
+    else
+      std::memcpy(iptr, hs.iptr.data(), sizeof(int32_t) * hs.iptr.size());
+    if (!hs.idx.empty()) std::memcpy(idx, hs.idx.data(), sizeof(int32_t) * hs.idx.size());
+    if (!hs.coff.empty()) std::memcpy(coff, hs.coff.data(), sizeof(int32_t) * hs.coff.size());
+    if (!hs.val.empty()) std::memcpy(sval, hs.val.data(), sizeof(double) * hs.val.size());
+  }
+  F3R_API_END
+}
+
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
   F3R_API_BEGIN
   F3R_CHECK(sell && out && sell->m->format == 1 && sell->m->sell_host_val, F3R_ERR_ARG,

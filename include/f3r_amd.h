@@ -134,6 +134,15 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
 int f3r_sell_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val,
                     int32_t C, int32_t sigma, int32_t K, int32_t *cptr, int32_t *cwidth,
                     int32_t *sidx, double *sval, int32_t *perm);
+/* host-only chunk-offset layout (F3R_SELL_OFFSETS, sigma = 1).  A first call
+ * with the array outputs NULL returns the sizes: value slots, int32 column
+ * indices (of the explicit chunks) and offset-table entries.  Then cptr[nchunks+1],
+ * cwidth[nchunks], iptr[nchunks] (>= 0: start of an explicit chunk in idx;
+ * < 0: offset chunk with table coff[-iptr-1 ...]), idx, coff and sval. */
+int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val,
+                           int32_t C, int32_t K, int64_t *nslots, int64_t *nidx, int64_t *ncoff,
+                           int32_t *cptr, int32_t *cwidth, int32_t *iptr, int32_t *idx,
+                           int32_t *coff, double *sval);
 /* cross-precision SELL sharing the index arrays (sell.hpp:51-53) */
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out);
 int f3r_mat_destroy(f3r_mat A);

@@ -178,3 +178,62 @@ def test_sdainv_host_matches_oracle_on_lognormal():
     assert np.array_equal(zr, ozr) and np.array_equal(zi, ozi) and np.array_equal(zv, ozv)
     assert np.array_equal(wr, owr) and np.array_equal(wi, owi) and np.array_equal(wv, owv)
     assert len(zi) > A.n  # non-trivial pattern on the 7-point lognormal family
+
+
+def _rows_from_offset_layout(L, n, C, K):
+    """(columns, values) of every row, in slot order, from a chunk-offset layout"""
+    rows = []
+    nch = (n + C - 1) // C
+    for s in range(nch):
+        r0 = s * C
+        tc = min(C, n - r0)
+        w = L["cwidth"][s] * K
+        for j in range(tc):
+            r = r0 + j
+            cols, vals = [], []
+            for t in range(w):
+                pos = L["cptr"][s] + ((t // K) * tc + j) * K + t % K
+                if L["iptr"][s] < 0:
+                    c = r + L["coff"][-L["iptr"][s] - 1 + t]
+                else:
+                    c = L["idx"][L["iptr"][s] + pos - L["cptr"][s]]
+                cols.append(c)
+                vals.append(L["val"][pos])
+            rows.append((np.array(cols), np.array(vals), L["iptr"][s] < 0))
+    return rows
+
+
+@pytest.mark.parametrize("kind,C,K", [("27pt", 64, 4), ("27pt", 128, 2), ("7pt", 64, 1), ("ragged", 64, 4)])
+def test_offset_layout_reproduces_csr_rows(kind, C, K):
+    # every nonzero of every row appears once, in CSR (column) order; the other
+    # slots hold 0 at an in-range column; stencil chunks use offsets
+    import f3r_amd as F
+    if kind == "27pt":
+        A = F.CSR.stencil27(9, 8, 10, beta=0.5)
+    elif kind == "7pt":
+        A = F.CSR.convdiff7(11, sigma=1.0, corr=3.0, peclet=5.0, seed=2)
+    else:
+        n, rp, ix, v = PB.random_sparse(700, 30, 4)
+        A = F.CSR(n, rp, ix, v)
+    L = F.sell_offset_layout(A, C, K)
+    rows = _rows_from_offset_layout(L, A.n, C, K)
+    n_off = 0
+    for r, (cols, vals, is_off) in enumerate(rows):
+        lo, hi = A.rptr[r], A.rptr[r + 1]
+        assert np.all((cols >= 0) & (cols < A.n))
+        if is_off:
+            n_off += 1
+            nz = vals != 0
+            assert np.array_equal(cols[nz], A.idx[lo:hi][A.val[lo:hi] != 0])
+            assert np.array_equal(vals[nz], A.val[lo:hi][A.val[lo:hi] != 0])
+            assert np.all(np.diff(cols[nz]) > 0)      # nonzeros summed in column (CSR) order
+        else:
+            k = hi - lo
+            assert np.array_equal(cols[:k], A.idx[lo:hi]) and np.array_equal(vals[:k], A.val[lo:hi])
+            assert np.all(vals[k:] == 0)
+    if kind != "ragged":
+        assert n_off >= A.n // 2
+    # eligibility rule: an offset chunk has at most 5/4 of the int32 layout's
+    # slots, so the layout never grows by more than that
+    plain = F.sell_layout(A, C, 1, K)[3]
+    assert L["val"].size <= 1.25 * plain.size + K * ((A.n + C - 1) // C) * C
