@@ -10,10 +10,25 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "common.hpp"
 
 namespace f3r {
+
+// fn(begin, end) over [0, n) split into contiguous ranges, one per hardware
+// thread (host set-up only; every range writes disjoint outputs)
+template <class Fn> static void parallel_ranges(int64_t n, Fn fn) {
+  const int64_t hw = std::max<int64_t>(1, std::thread::hardware_concurrency());
+  const int64_t nt = std::min<int64_t>(hw, std::max<int64_t>(1, n / 256));
+  if (nt <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+  for (auto &x : th) x.join();
+}
 
 // algorithm::scaling (sparse/matrix/algorithm.hpp:45-59): every entry divided
 // by sqrt|a_ii| * sqrt|a_jj|; rows without a diagonal use 1.
@@ -82,46 +97,60 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
   S.cptr.assign(S.nchunks + 1, 0);
   S.cwidth.assign(S.nchunks, 0);
   std::vector<std::vector<int32_t>> uni(offsets ? S.nchunks : 0);
-  int64_t tot = 0;
-  std::vector<int32_t> u;
-  for (int32_t s = 0; s < S.nchunks; ++s) {
-    int64_t r0 = (int64_t)s * C;
-    int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
-    int32_t mx = 0;
-    for (int32_t j = 0; j < tc; ++j) {
-      int32_t o = S.perm[r0 + j];
-      mx = std::max(mx, A.rptr[o + 1] - A.rptr[o]);
-    }
-    int32_t width = mx;
-    if (offsets && mx > 0) {
-      // the union of col - row over the chunk; eligible when every row's
-      // columns ascend (CSR order = offset order), every row + offset is a
-      // row of the matrix, and the union is at most 5/4 of the widest row
-      u.clear();
-      bool ok = true;
-      for (int32_t j = 0; j < tc && ok; ++j) {
-        const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
-        int32_t prev = INT32_MIN;
-        for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
-          const int32_t c = col(k);
-          if (c <= prev) ok = false;
-          prev = c;
-          u.push_back(c - r);
+  // pass 1 (parallel over chunks): the widest row and, for the offset
+  // format, the union of col - row
+  parallel_ranges(S.nchunks, [&](int64_t c0, int64_t c1) {
+    std::vector<int32_t> u, first;
+    for (int64_t s = c0; s < c1; ++s) {
+      int64_t r0 = s * C;
+      int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
+      int32_t mx = 0;
+      for (int32_t j = 0; j < tc; ++j) {
+        int32_t o = S.perm[r0 + j];
+        mx = std::max(mx, A.rptr[o + 1] - A.rptr[o]);
+      }
+      int32_t width = mx;
+      if (offsets && mx > 0) {
+        // eligible when every row's columns ascend (CSR order = offset
+        // order), every row + offset is a row of the matrix, and the union
+        // is at most 5/4 of the widest row.  Rows repeating the previous
+        // row's offsets (the stencil case) skip the merge.
+        u.clear();
+        first.clear();
+        bool ok = true;
+        for (int32_t j = 0; j < tc && ok; ++j) {
+          const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
+          const int32_t len = A.rptr[o + 1] - A.rptr[o];
+          bool same = j > 0 && len == (int32_t)first.size();
+          int32_t prev = INT32_MIN;
+          for (int32_t q = 0; q < len; ++q) {
+            const int32_t c = col(A.rptr[o] + q);
+            if (c <= prev) ok = false;
+            prev = c;
+            if (same && first[q] != c - r) same = false;
+          }
+          if (same) continue;
+          first.clear();
+          for (int32_t q = 0; q < len; ++q) first.push_back(col(A.rptr[o] + q) - r);
+          u.insert(u.end(), first.begin(), first.end());
+        }
+        if (ok) {
+          std::sort(u.begin(), u.end());
+          u.erase(std::unique(u.begin(), u.end()), u.end());
+          ok = (int64_t)u.size() * 4 <= (int64_t)mx * 5 && r0 + u.front() >= 0 && r0 + tc - 1 + u.back() < n;
+        }
+        if (ok) {
+          uni[s] = u;
+          width = (int32_t)u.size();
         }
       }
-      if (ok) {
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        ok = (int64_t)u.size() * 4 <= (int64_t)mx * 5 && r0 + u.front() >= 0 && r0 + tc - 1 + u.back() < n;
-      }
-      if (ok) {
-        uni[s] = u;
-        width = (int32_t)u.size();
-      }
+      S.cwidth[s] = round_up(width, K) / K;
     }
-    int32_t groups = round_up(width, K) / K;
-    S.cwidth[s] = groups;
-    tot += (int64_t)groups * K * tc;
+  });
+  int64_t tot = 0;
+  for (int32_t s = 0; s < S.nchunks; ++s) {
+    int32_t tc = (int32_t)std::min<int64_t>(C, n - (int64_t)s * C);
+    tot += (int64_t)S.cwidth[s] * K * tc;
     F3R_CHECK(tot < INT32_MAX, F3R_ERR_RANGE, "SELL: more than 2^31 slots");
     S.cptr[s + 1] = (int32_t)tot;
   }
@@ -145,19 +174,25 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
     itot = tot;
   }
   S.idx.assign(itot, 0);
-  for (int32_t s = 0; s < S.nchunks; ++s) {
-    int64_t r0 = (int64_t)s * C;
+  // pass 3 (parallel over chunks): values and indices; every chunk writes
+  // its own range
+  std::vector<int64_t> off_slots(S.nchunks, 0), expl_nnz(S.nchunks, 0);
+  parallel_ranges(S.nchunks, [&](int64_t c0, int64_t c1) {
+  for (int64_t s = c0; s < c1; ++s) {
+    int64_t r0 = s * C;
     int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
     int32_t width = S.cwidth[s] * K;
     if (offsets && !uni[s].empty()) {
       // offset chunk: the value of column r + uni[t] goes to slot t, the
       // other slots stay 0 (their columns r + uni[t] are in range)
       const std::vector<int32_t> &us = uni[s];
-      S.offset_slots += (int64_t)width * tc;
+      off_slots[s] = (int64_t)width * tc;
       for (int32_t j = 0; j < tc; ++j) {
         const int32_t o = S.perm[r0 + j], r = (int32_t)(r0 + j);
+        int32_t t = 0;
         for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
-          const int32_t t = (int32_t)(std::lower_bound(us.begin(), us.end(), col(k) - r) - us.begin());
+          const int32_t d = col(k) - r;
+          while (us[t] < d) ++t;  // columns ascend, so the slot only moves forward
           const int64_t pos = S.cptr[s] + ((int64_t)(t / K) * tc + j) * K + (t % K);
           S.val[pos] = A.val[k];
         }
@@ -168,7 +203,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
     for (int32_t j = 0; j < tc; ++j) {
       int32_t o = S.perm[r0 + j];
       int32_t len = A.rptr[o + 1] - A.rptr[o];
-      S.explicit_nnz += len;
+      expl_nnz[s] += len;
       int32_t last = 0;
       for (int32_t q = 0; q < width; ++q) {
         int64_t pos = S.cptr[s] + ((int64_t)(q / K) * tc + j) * K + (q % K);
@@ -182,6 +217,11 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
         }
       }
     }
+  }
+  });
+  for (int32_t s = 0; s < S.nchunks; ++s) {
+    S.offset_slots += off_slots[s];
+    S.explicit_nnz += expl_nnz[s];
   }
   return S;
 }
