@@ -229,7 +229,7 @@ def random_probes(n, k, seed):
     return rng.choice(np.array([-1.0, 1.0], np.float32), size=(k, n))
 
 
-def train(ctx, systems, epochs=20, lr=2e-3, seed=0, log=None, kind=1):
+def train(ctx, systems, epochs=20, lr=1e-2, seed=0, log=None, kind=0):
     """Adam over the systems (one step per system, shuffled each epoch).
     `systems` are LearnSystem objects with probes.  Returns the MLP and the
     mean loss of each epoch."""
@@ -296,7 +296,8 @@ def main(argv=None):
     ap.add_argument("--test", type=int, default=64)
     ap.add_argument("--probes", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=10)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--kind", type=int, default=0, help="loss: 0 = |(I - AM)z|^2/|z|^2, 1 = best-weight residual")
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--out", default="learned_sdainv.npz")
     ap.add_argument("--report", default=None)
@@ -316,18 +317,18 @@ def main(argv=None):
             systems.append(L)
         log(f"{len(systems)} training systems ready")
     m0 = MLP(ctx, 0)
-    base = float(np.mean([L.loss(m0) for L in systems]))
+    base = float(np.mean([L.loss(m0, kind=a.kind) for L in systems]))
     log(f"plain SDAINV loss {base:.6f}")
-    mlp, hist = train(ctx, systems, a.epochs, a.lr, log=log)
+    mlp, hist = train(ctx, systems, a.epochs, a.lr, log=log, kind=a.kind)
     np.savez(a.out, params=mlp.get(), loss=np.array(hist), base_loss=base, nx=a.nx, train=a.train,
-             probes=a.probes, epochs=a.epochs)
+             probes=a.probes, epochs=a.epochs, kind=a.kind, lr=a.lr)
     log(f"parameters saved to {a.out}")
     del systems
     rep = []
     test_seeds = range(100000, 100000 + a.test)
     for c0 in range(0, a.test, chunk):
         for hs in host_systems(a.nx, list(test_seeds)[c0:c0 + chunk], a.threads):
-            r = evaluate(ctx, mlp, hs, reps=3)
+            r = evaluate(ctx, mlp, hs, reps=3, normalize=a.kind == 1)
             r.update(seed=hs["seed"], sigma=hs["sigma"], corr=hs["corr"])
             rep.append(r)
             log(f"held-out {hs['seed']}: plain {r['plain']['outer']} steps {r['plain']['solve_s'] * 1e3:.1f} ms, "
@@ -337,7 +338,8 @@ def main(argv=None):
     po = float(np.mean([r["plain"]["outer"] for r in rep]))
     lo = float(np.mean([r["learned"]["outer"] for r in rep]))
     ok = all(r[k]["solved"] and r[k]["true_rel_res"] <= 1e-8 for r in rep for k in ("plain", "learned"))
-    summary = {"train": a.train, "test": a.test, "nx": a.nx, "base_loss": base, "final_loss": hist[-1],
+    summary = {"train": a.train, "test": a.test, "nx": a.nx, "kind": a.kind, "base_loss": base,
+               "final_loss": hist[-1],
                "plain_mean_s": ps, "learned_mean_s": ls, "speedup": ps / ls, "plain_mean_outer": po,
                "learned_mean_outer": lo, "all_solved": ok}
     log(f"held-out mean: plain {ps * 1e3:.2f} ms ({po:.2f} outer steps), learned {ls * 1e3:.2f} ms "

@@ -187,7 +187,8 @@ typedef struct {
   double eps;          /* stop when |e_j| / ||b|| < eps */
   int32_t variant;     /* 0 HALF (fp16 innermost), 1 FLOAT, 2 DOUBLE */
   double range_scale;  /* power-of-two factor on the fp16 level's input; 0 = auto */
-  int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = autotuned) */
+  int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = default: C 64, sigma 1,
+                          K 8 for rows of 16 or more entries on average, else 4) */
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */

@@ -264,13 +264,13 @@ def test_training_lowers_loss_and_learned_solve_converges(ctx):
         systems.append(L)
         hs.append({"A": A, "P": P, "feat": feat, "seed": seed})
     m0 = learn.MLP(ctx, seed=0)
-    base = np.mean([L.loss(m0) for L in systems])
-    mlp, hist = learn.train(ctx, systems, epochs=25, lr=3e-3, seed=0)
-    final = np.mean([L.loss(mlp) for L in systems])
+    base = np.mean([L.loss(m0, kind=0) for L in systems])
+    mlp, hist = learn.train(ctx, systems, epochs=25, lr=3e-3, seed=0, kind=0)
+    final = np.mean([L.loss(mlp, kind=0) for L in systems])
     print(f"training loss: plain SDAINV {base:.5f} -> learned {final:.5f}; per epoch {np.round(hist, 5)}")
     assert hist[0] == pytest.approx(base, rel=0.2)
     assert final < base
-    r = learn.evaluate(ctx, mlp, hs[0])
+    r = learn.evaluate(ctx, mlp, hs[0], normalize=False)
     print(f"held-in system: {r}")
     for k in ("plain", "learned"):
         assert r[k]["solved"] and r[k]["true_rel_res"] <= 1e-8
