@@ -208,7 +208,7 @@ def main():
                      "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
                      "traffic_detail": None,
                      "kernel": "fp16 SELL SpMV y = A16 x (fp16 x, y; fp32 sums), chunk-offset layout "
-                               "(C=64, K=4): bytes are those of the layout (values; no column indices)",
+                               "(C=64, K=8): bytes are those of the layout (values; no column indices)",
                      "us_per_launch": round(ms16 * 1e3, 2), "algorithmic_bytes": by16,
                      "int32_index_equivalent_GBs": round((6.0 * A.nnz + 4.0 * A.n) / (ms16 * 1e-3) / 1e9, 1)},
         "kernels": kern,

@@ -125,17 +125,37 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
 
 // An epilogue provides
 //   struct State;                                   per-thread carry (may be empty)
-//   __device__ void row(int r, ACC acc, State &s);  called for every row r < n
+//   struct Pre;                                     per-row operands loaded early
+//   __device__ Pre pre(int r);                      issued before the row's SpMV,
+//                                                   so its loads overlap the gathers
+//   __device__ void row(int r, ACC acc, const Pre &p, State &s);  for every row r < n
 //   __device__ void finish(State &s);               called once by every thread
 // finish() may reduce over the block and the grid.
 struct NoState {};
+struct NoPre {};
+
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by
+// blockIdx.  Renumbering them so that XCD k takes the k-th contiguous eighth
+// of the blocks keeps the rows one XCD works on, and so the x entries it
+// gathers, together in that XCD's L2 instead of every XCD fetching all of x.
+constexpr int kXcds = 8;
+#ifndef F3R_XCD_MAP
+#define F3R_XCD_MAP 1
+#endif
+__device__ __forceinline__ int xcd_block() {
+  const int g = gridDim.x, b = blockIdx.x;
+  if (!F3R_XCD_MAP || g % kXcds) return b;
+  return (b % kXcds) * (g / kXcds) + b / kXcds;
+}
 
 template <class V, class X, class ACC, int K, class Epi>
 __global__ __launch_bounds__(kBlock) void k_sell(SellArgs a, const V *__restrict__ val,
                                                  const X *__restrict__ x, Epi epi) {
   typename Epi::State st{};
-  for (int r = blockIdx.x * kBlock + threadIdx.x; r < a.n; r += gridDim.x * kBlock)
-    epi.row(r, sell_row<V, X, ACC, K>(a, val, x, r), st);
+  for (int r = xcd_block() * kBlock + threadIdx.x; r < a.n; r += gridDim.x * kBlock) {
+    const typename Epi::Pre p = epi.pre(r);
+    epi.row(r, sell_row<V, X, ACC, K>(a, val, x, r), p, st);
+  }
   epi.finish(st);
 }
 
@@ -158,9 +178,11 @@ inline void launch_sell(int K, const SellArgs &a, const V *val, const X *x, Epi 
 // ------------------------------------------------------------ epilogues ---
 template <class Y, class ACC> struct EpiStore {
   using State = NoState;
+  using Pre = NoPre;
   Y *__restrict__ y;
   const int32_t *__restrict__ perm;
-  __device__ void row(int r, ACC acc, State &) const {
+  __device__ Pre pre(int) const { return {}; }
+  __device__ void row(int r, ACC acc, const Pre &, State &) const {
     const int row = perm ? perm[r] : r;
     y[row] = cvt<Y>(acc);
   }
@@ -169,12 +191,16 @@ template <class Y, class ACC> struct EpiStore {
 
 template <class Y, class B, class ACC> struct EpiResid {
   using State = NoState;
+  struct Pre {
+    B b;
+  };
   Y *__restrict__ y;
   const B *__restrict__ b;
   const int32_t *__restrict__ perm;
-  __device__ void row(int r, ACC acc, State &) const {
+  __device__ Pre pre(int r) const { return {b[perm ? perm[r] : r]}; }
+  __device__ void row(int r, ACC acc, const Pre &p, State &) const {
     const int row = perm ? perm[r] : r;
-    y[row] = cvt<Y>((ACC)to_acc(b[row]) - acc);
+    y[row] = cvt<Y>((ACC)to_acc(p.b) - acc);
   }
   __device__ void finish(State &) const {}
 };

@@ -78,17 +78,26 @@ template <class T, int NV> struct EpiDots {
   struct State {
     T v[NV];
   };
+  struct Pre {
+    T v[NV];
+  };
   T *__restrict__ w;
   Cols<T, NV> cols;
   int ncols;
   T *Hcol;
   T *partial;
   unsigned *counter;
-  __device__ void row(int r, T acc, State &s) const {
+  __device__ Pre pre(int r) const {
+    Pre p;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) p.v[k] = k < ncols ? cols.p[k][r] : (T)0;
+    return p;
+  }
+  __device__ void row(int r, T acc, const Pre &p, State &s) const {
     w[r] = acc;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      if (k < ncols) s.v[k] = fma_<T>(acc, cols.p[k][r], s.v[k]);
+      if (k < ncols) s.v[k] = fma_<T>(acc, p.v[k], s.v[k]);
   }
   __device__ void finish(State &s) const {
     __shared__ T lds[NV * kWavesPerBlock];
@@ -103,6 +112,7 @@ template <class T> struct EpiResidNorm {
   struct State {
     T v[1];
   };
+  using Pre = NoPre;
   T *__restrict__ v0;
   const T *__restrict__ b;
   T *e;
@@ -110,7 +120,8 @@ template <class T> struct EpiResidNorm {
   double *status;
   T *partial;
   unsigned *counter;
-  __device__ void row(int r, T acc, State &s) const {
+  __device__ Pre pre(int) const { return {}; }
+  __device__ void row(int r, T acc, const Pre &, State &s) const {
     T rv = b[r] - acc;
     v0[r] = rv;
     s.v[0] = fma_<T>(rv, rv, s.v[0]);
@@ -321,6 +332,9 @@ __global__ void k_l4_update(int64_t n, const half_t *__restrict__ tmp2, const ha
 // probe calls hout' = fp16(fma(ome_k, tmp2, hout)) (last step: out = hout'/s).
 struct EpiL4Diag {
   using State = NoState;
+  struct Pre {
+    half_t hin, dw, dz, hout;
+  };
   const half_t *__restrict__ hin;
   const half_t *__restrict__ hout;
   const half_t *__restrict__ dw;
@@ -331,16 +345,17 @@ struct EpiL4Diag {
   half_t *__restrict__ hnext;
   float *__restrict__ out;
   float inv_s;
-  __device__ void row(int r, float acc, State &) const {
-    const half_t hr = cvt<half_t>((float)hin[r] - acc);
-    const half_t t = cvt<half_t>(__builtin_fmaf((float)dw[r], (float)hr, 0.f));
-    const half_t t2 = cvt<half_t>(__builtin_fmaf((float)dz[r], (float)t, 0.f));
+  __device__ Pre pre(int r) const { return {hin[r], dw[r], dz[r], hr_out ? (half_t)0 : hout[r]}; }
+  __device__ void row(int r, float acc, const Pre &p, State &) const {
+    const half_t hr = cvt<half_t>((float)p.hin - acc);
+    const half_t t = cvt<half_t>(__builtin_fmaf((float)p.dw, (float)hr, 0.f));
+    const half_t t2 = cvt<half_t>(__builtin_fmaf((float)p.dz, (float)t, 0.f));
     if (hr_out) {
       hr_out[r] = hr;
       tmp2_out[r] = t2;
       return;
     }
-    const half_t h = cvt<half_t>(__builtin_fmaf((float)*w, (float)t2, (float)hout[r]));
+    const half_t h = cvt<half_t>(__builtin_fmaf((float)*w, (float)t2, (float)p.hout));
     if (out)
       out[r] = (float)h * inv_s;
     else
@@ -432,7 +447,8 @@ struct Solver {
   f3r_params p{};
   int32_t n = 0;
   int64_t nnz = 0;
-  std::shared_ptr<SellShared> sh;
+  std::shared_ptr<SellShared> sh;    // layout of A64 / A32
+  std::shared_ptr<SellShared> sh16;  // layout of A16 (sh itself when K agrees)
   DevPtr A64, A32, A16;
   std::shared_ptr<Prec> P;
   float range_s = 1.f;
@@ -469,6 +485,11 @@ struct Solver {
     return (int)std::min<int64_t>(g, (int64_t)ctx->num_cu * 16);
   }
   SellArgs args() const { return sell_args(*sh, false); }
+  // the layout of the matrix with value type V (the fp16 copy may differ in K)
+  template <class V> const SellShared &shv() const {
+    if constexpr (std::is_same<V, half_t>::value) return *sh16; else return *sh;
+  }
+  template <class V> SellArgs argsv() const { return sell_args(shv<V>(), false); }
   unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + (int64_t)i * kCounterSlot; }
   template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
   template <class T> InnerLevel<T> &L2() {
@@ -486,7 +507,7 @@ struct Solver {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     EpiDots<T, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<T>(), counter(0)};
-    launch_sell<V, T, T>(sh->K, args(), val, L.Z[j], epi, st(), dots_grid);
+    launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), dots_grid);
   }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
@@ -551,7 +572,8 @@ struct Solver {
     const half_t *dw = diag ? P->dw->as<half_t>() : nullptr;
     const half_t *dz = diag ? P->dz->as<half_t>() : nullptr;
     const half_t *a16 = A16->as<half_t>();
-    SellArgs a = args();
+    SellArgs a = argsv<half_t>();
+    const int K16 = sh16->K;
     // entry (normalising `in` first when the Arnoldi step that made it did not)
     const float *pinv = pending_inv;
     pending_inv = nullptr;
@@ -563,7 +585,7 @@ struct Solver {
     }
     if (probe) {
       half_t *t = tt->as<half_t>();
-      launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, EpiStore<half_t, float>{t, nullptr}, st());
+      launch_sell<half_t, half_t, float>(K16, a, a16, ho, EpiStore<half_t, float>{t, nullptr}, st());
       hipLaunchKernelGGL(k_probe, dim3(dots_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hi, rs,
                          0, partial<float>(), counter(2));
       hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->alpha[0]);
@@ -574,15 +596,15 @@ struct Solver {
       if (diag) {
         EpiL4Diag epi{hi, ho, dw, dz, &rs->ome[k], probe ? hr_ : nullptr, probe ? t2 : nullptr,
                       ho2, last ? out : nullptr, inv_s};
-        launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, epi, st());
+        launch_sell<half_t, half_t, float>(K16, a, a16, ho, epi, st());
       } else {
-        launch_sell<half_t, half_t, float>(sh->K, a, a16, ho, EpiResid<half_t, half_t, float>{hr_, hi, nullptr},
+        launch_sell<half_t, half_t, float>(K16, a, a16, ho, EpiResid<half_t, half_t, float>{hr_, hi, nullptr},
                                            st());
         prec_apply_raw(P.get(), hr_, t2, st());
       }
       if (probe) {
         half_t *t = tt->as<half_t>();
-        launch_sell<half_t, half_t, float>(sh->K, a, a16, t2, EpiStore<half_t, float>{t, nullptr}, st());
+        launch_sell<half_t, half_t, float>(K16, a, a16, t2, EpiStore<half_t, float>{t, nullptr}, st());
         hipLaunchKernelGGL(k_probe, dim3(dots_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hr_, rs,
                            k, partial<float>(), counter(2));
         hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
@@ -804,9 +826,12 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   S->nnz = A.nnz();
   const int C = p.C > 0 ? p.C : 64;
   const int sigma = p.sigma > 0 ? p.sigma : 1;
-  // K = 0: eight slots per lane for long rows (27-point: 4 slot groups per
-  // row instead of 7, measured 27 vs 30 us at 128^3), four otherwise
-  const int K = p.K > 0 ? p.K : (A.n > 0 && A.nnz() >= (int64_t)16 * A.n ? 8 : 4);
+  // K = 0: four slots per lane for the fp64 / fp32 matrices (16- or 32-byte
+  // loads); for the fp16 matrix eight when rows are long (27-point: 4 slot
+  // groups per row instead of 7, 27 vs 30 us at 128^3), else four.  The two
+  // layouts share the row order; their slot padding differs.
+  const int K = p.K > 0 ? p.K : 4;
+  const int K16 = p.K > 0 ? p.K : (A.n > 0 && A.nnz() >= (int64_t)16 * A.n ? 8 : 4);
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
   F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "solver: K must be 1, 2, 4 or 8");
   HostSell hs = build_sell(A, C, sigma, K, nullptr, true, p.index != 0);
@@ -815,11 +840,22 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   S->sh = upload_sell_shared(ctx.get(), hs, true);
   S->sh->nnz = A.nnz();
   S->sh->sigma = sigma;
+  S->sh16 = S->sh;
   // the level matrices: fp64 always (outer level; Richardson of FLOAT and
   // DOUBLE), fp32 for levels 2-3 of HALF / FLOAT, fp16 for HALF
   S->A64 = upload_sell_values(ctx.get(), hs, F3R_F64);
   if (p.variant != 2) S->A32 = upload_sell_values(ctx.get(), hs, F3R_F32);
-  if (p.variant == 0) S->A16 = upload_sell_values(ctx.get(), hs, F3R_F16);
+  if (p.variant == 0) {
+    if (K16 == K) {
+      S->A16 = upload_sell_values(ctx.get(), hs, F3R_F16);
+    } else {
+      HostSell h16 = build_sell(A, C, sigma, K16, nullptr, true, p.index != 0);
+      S->sh16 = upload_sell_shared(ctx.get(), h16, true);
+      S->sh16->nnz = A.nnz();
+      S->sh16->sigma = sigma;
+      S->A16 = upload_sell_values(ctx.get(), h16, F3R_F16);
+    }
+  }
   // innermost preconditioner in the internal order and the innermost precision
   const int pdt = p.variant == 0 ? F3R_F16 : p.variant == 1 ? F3R_F32 : F3R_F64;
   auto P = std::make_shared<Prec>();
@@ -830,7 +866,7 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   P->acc = Pin->acc;
   P->host = Pin->host;
   F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
-  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, K, p.index != 0);
+  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, pdt == F3R_F16 ? K16 : K, p.index != 0);
   S->P = P;
   S->range_s = p.variant != 0 ? 1.f
                : p.range_scale > 0 ? (float)p.range_scale
@@ -1085,7 +1121,7 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
   F3R_API_BEGIN
   F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 3, F3R_ERR_ARG, "time_kernel: bad argument");
   Solver *s = S->s;
-  SellArgs a = s->args();
+  SellArgs a = s->args(), a16 = s->argsv<half_t>();
   F3R_CHECK(s->p.variant == 0 || which == 3, F3R_ERR_UNSUPPORTED, "time_kernel: fp16/fp32 kernels need the HALF variant");
   half_t *x16 = s->hout->as<half_t>(), *y16 = s->tt->as<half_t>();
   float *x32 = s->L2f.V.empty() ? nullptr : s->L2f.V[0], *y32 = s->L2f.V.empty() ? nullptr : s->L2f.V[1];
@@ -1094,14 +1130,14 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
   auto launch = [&]() {
     switch (which) {
       case 0:  // fp16 values, fp16 x and y (the metric's fp16 SpMV)
-        launch_sell<half_t, half_t, float>(s->sh->K, a, s->A16->as<half_t>(), x16,
+        launch_sell<half_t, half_t, float>(s->sh16->K, a16, s->A16->as<half_t>(), x16,
                                            EpiStore<half_t, float>{y16, nullptr}, s->st());
-        by = s->sh->format_bytes(2, 2, 2);
+        by = s->sh16->format_bytes(2, 2, 2);
         break;
       case 1:  // fp16 values, fp32 x and y (level 3)
-        launch_sell<half_t, float, float>(s->sh->K, a, s->A16->as<half_t>(), x32,
+        launch_sell<half_t, float, float>(s->sh16->K, a16, s->A16->as<half_t>(), x32,
                                           EpiStore<float, float>{y32, nullptr}, s->st());
-        by = s->sh->format_bytes(2, 4, 4);
+        by = s->sh16->format_bytes(2, 4, 4);
         break;
       case 2:  // fp32 (level 2)
         launch_sell<float, float, float>(s->sh->K, a, s->A32->as<float>(), x32,
