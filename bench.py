@@ -102,7 +102,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--blocks", type=int, default=0, help="blocks of the fused SpMV+reduction kernels (0 = auto)")
+    ap.add_argument("--blocks", type=int, default=0, help="blocks of the streaming reduction kernels (0 = auto)")
+    ap.add_argument("--cap-spmv", action="store_true", help="run the fused SpMV+reduction kernels on that grid too")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--learned", type=int, default=4, metavar="N",
@@ -128,7 +129,7 @@ def main():
     ctx = F.Context(0)
     t0 = time.time()
     P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
-    S = F.F3R(ctx, A, P, use_graph=not args.no_graph, grid=args.blocks)
+    S = F.F3R(ctx, A, P, use_graph=not args.no_graph, grid=args.blocks, cap_spmv=args.cap_spmv)
     setup_s = time.time() - t0
     log(f"[bench] setup {setup_s:.1f} s")
     if args.kernel_only:

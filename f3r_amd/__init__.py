@@ -398,7 +398,8 @@ class F3R:
     Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU."""
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
-                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1):
+                 range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1,
+                 cap_spmv=False):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -410,7 +411,7 @@ class F3R:
         p.range_scale = range_scale
         p.C, p.sigma, p.K = C, sigma, K
         p.use_graph = 1 if use_graph else 0
-        p.flags = 1 if history else 0
+        p.flags = (1 if history else 0) | (2 if cap_spmv else 0)
         p.grid = grid
         p.index = index
         self.params = p

@@ -148,7 +148,7 @@ constexpr int kCounterSlot = 256 * kCounterStride;  // one reduction slot: 1 + 2
 // NV block sums to partial[i * G + block]; the last block to arrive in each
 // group of kGroup blocks adds that group's partials (in block order) into
 // partial[NV * G + i * ng + group]; the last group to finish adds the group
-// sums (in group order) and returns true with the totals in v of thread 0.
+// sums (in group order) and returns true in thread 0, with the totals in its v.
 // The order of every addition is fixed, so the result does not depend on
 // scheduling.  counters[0] counts groups, counters[(1 + g) * kCounterStride]
 // the blocks of group g;
@@ -185,54 +185,68 @@ __device__ __forceinline__ unsigned arrive(unsigned *c) {
 template <class T, int NV>
 __device__ __forceinline__ bool grid_sum(T (&v)[NV], T *__restrict__ partial,
                                          unsigned *__restrict__ counters, T *lds) {
-  block_sum<T, NV>(v, lds);
+  // block totals into thread 0; after the one barrier only wave 0 stays, so
+  // a block waiting for its arrival to come back holds one wave slot, not
+  // four (the SpMV kernels with fused reductions keep their occupancy)
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[i * kWavesPerBlock + wid] = v[i];
+  }
+  __syncthreads();
+  if (wid != 0) return false;
   const int G = gridDim.x;
   const int ng = (G + kGroup - 1) / kGroup;
   const int grp = blockIdx.x / kGroup;
   const int b0 = grp * kGroup;
   const int gsize = (G - b0) < kGroup ? (G - b0) : kGroup;
-  __shared__ int stage;
-  if (threadIdx.x == 0) {
+  int last = 0;
+  if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) publish(&partial[(int64_t)i * G + blockIdx.x], v[i]);
+    for (int i = 0; i < NV; ++i) {
+      T t = lds[i * kWavesPerBlock];
+#pragma unroll
+      for (int w = 1; w < kWavesPerBlock; ++w) t += lds[i * kWavesPerBlock + w];
+      publish(&partial[(int64_t)i * G + blockIdx.x], t);
+    }
     stores_complete();
     const unsigned prev = arrive(&counters[(1 + grp) * kCounterStride]);
-    stage = (prev == (unsigned)gsize - 1) ? 1 : 0;
+    last = prev == (unsigned)gsize - 1;
   }
-  __syncthreads();
-  if (stage == 0) return false;
+  if (!__builtin_amdgcn_readfirstlane(last)) return false;
+  // last block of its group: add the group's partials in block order
   T acc[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = 0;
-  if (threadIdx.x < gsize) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = collect(&partial[(int64_t)i * G + b0 + threadIdx.x]);
+  for (int i = 0; i < NV; ++i) {
+    acc[i] = lane < gsize ? collect(&partial[(int64_t)i * G + b0 + lane]) : (T)0;
+    acc[i] = wave_sum(acc[i]);
   }
-  block_sum<T, NV>(acc, lds);
   T *gpart = partial + (int64_t)NV * G;
-  if (threadIdx.x == 0) {
+  last = 0;
+  if (lane == 0) {
     __hip_atomic_store(&counters[(1 + grp) * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int i = 0; i < NV; ++i) publish(&gpart[(int64_t)i * ng + grp], acc[i]);
     stores_complete();
     const unsigned prev = arrive(&counters[0]);
-    stage = (prev == (unsigned)ng - 1) ? 2 : 0;
+    last = prev == (unsigned)ng - 1;
   }
-  __syncthreads();
-  if (stage != 2) return false;
+  if (!__builtin_amdgcn_readfirstlane(last)) return false;
+  // last group: add the group sums in group order
 #pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = 0;
-  for (int g = threadIdx.x; g < ng; g += kBlock) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] += collect(&gpart[(int64_t)i * ng + g]);
+  for (int i = 0; i < NV; ++i) {
+    T t = 0;
+    for (int g = lane; g < ng; g += kWave) t += collect(&gpart[(int64_t)i * ng + g]);
+    acc[i] = wave_sum(t);
   }
-  block_sum<T, NV>(acc, lds);
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = acc[i];
     __hip_atomic_store(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  return true;
+  return lane == 0;
 }
 
 // largest grid grid_sum supports (one counter per group in a slot)

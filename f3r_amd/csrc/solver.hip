@@ -27,6 +27,21 @@
 
 namespace f3r {
 
+template <class T> struct V4 {
+  T v[4];
+};
+template <class T> __device__ __forceinline__ V4<T> ld4(const T *p) {
+  V4<T> r;
+  __builtin_memcpy(&r, __builtin_assume_aligned(p, 4 * sizeof(T)), sizeof(r));
+  return r;
+}
+template <class T> __device__ __forceinline__ void st4(T *p, const V4<T> &r) {
+  __builtin_memcpy(__builtin_assume_aligned(p, 4 * sizeof(T)), &r, sizeof(r));
+}
+// true when p is aligned to 4 elements (the vectorised kernels need it)
+template <class T> inline bool aligned4(const T *p) { return ((uintptr_t)p % (4 * sizeof(T))) == 0; }
+
+
 // ------------------------------------------------------- device state ---
 // Adaptive Richardson weights of the innermost level (work/src/f3r-gpu.cpp:
 // 224-227, 276-281): sum_k starts at 1, ome_k = sum_k / den, den = number of
@@ -139,7 +154,9 @@ template <class T> struct EpiResidNorm {
 
 // V_{j+1} -= V H(j)[0..j] (MULMV order: V_0 h_0, then fused adds), ||V_{j+1}||,
 // then the last block: H(j, j+1), 1/h, Givens, |e_{j+1}| for the host.
-template <class T, int NV>
+// VEC: every pointer is 4-element aligned; each lane then takes 4
+// consecutive rows per step with 16-byte (fp32) loads and stores
+template <class T, int NV, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict__ w, Cols<T, NV> cols,
                                                        int ncols, T *Hcol, T *cs, T *sn, T *e, int j,
                                                        int e0_one, T *inv, double *status, T *partial,
@@ -149,7 +166,31 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
 #pragma unroll
   for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
   T v[1] = {0};
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
+  int64_t i0 = 0;
+  if constexpr (VEC) {
+    const int64_t n4 = n / 4;
+    for (int64_t q = gid; q < n4; q += gstride) {
+      V4<T> vk[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        if (k < ncols) vk[k] = ld4(cols.p[k] + 4 * q);
+      V4<T> wv = ld4(w + 4 * q);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T r = vk[0].v[c] * h[0];
+#pragma unroll
+        for (int k = 1; k < NV; ++k)
+          if (k < ncols) r = fma_<T>(vk[k].v[c], h[k], r);
+        wv.v[c] = wv.v[c] - r;
+      }
+      st4(w + 4 * q, wv);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[0] = fma_<T>(wv.v[c], wv.v[c], v[0]);
+    }
+    i0 = 4 * n4;
+  }
+  for (int64_t i = i0 + gid; i < n; i += gstride) {
     T r = cols.p[0][i] * h[0];
 #pragma unroll
     for (int k = 1; k < NV; ++k)
@@ -227,13 +268,35 @@ __global__ void k_vscale(int64_t n, T *__restrict__ v, const T *__restrict__ inv
 
 // out = Z y (with_init) of an inner level; every block solves the small
 // triangular system itself (j <= NV)
-template <class T, class TO, int NV>
+template <class T, class TO, int NV, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<T, NV> Z, int j, const T *H, int ldh,
                                                      const T *e, TO *__restrict__ out) {
   __shared__ T y[NV];
   if (threadIdx.x == 0) hess_solve<T>(H, ldh, e, y, j);
   __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
+  int64_t i0 = 0;
+  if constexpr (VEC) {
+    const int64_t n4 = n / 4;
+    for (int64_t q = gid; q < n4; q += gstride) {
+      V4<T> zk[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        if (k < j) zk[k] = ld4(Z.p[k] + 4 * q);
+      V4<TO> o;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T r = zk[0].v[c] * y[0];
+#pragma unroll
+        for (int k = 1; k < NV; ++k)
+          if (k < j) r = fma_<T>(zk[k].v[c], y[k], r);
+        o.v[c] = cvt<TO>(r);
+      }
+      st4(out + 4 * q, o);
+    }
+    i0 = 4 * n4;
+  }
+  for (int64_t i = i0 + gid; i < n; i += gstride) {
     T r = Z.p[0][i] * y[0];
 #pragma unroll
     for (int k = 1; k < NV; ++k)
@@ -283,26 +346,53 @@ __global__ void k_scatter(int64_t n, const double *__restrict__ s, const int32_t
 // arithmetic): hin = fp16(in * s); with diagonal SDAINV factors also
 // hout = fp16(z * fp16(w * hin)) and, outside probe calls, hout *= ome_0.
 // `inv` (optional): `in` is a basis vector not yet normalised; it is scaled
-// by *inv in place first (the normalisation of the Arnoldi step that made it)
+// by *inv in place first (the normalisation of the Arnoldi step that made it).
+// VEC: 4 consecutive rows per lane (every array 4-element aligned).
+__device__ __forceinline__ void l4_entry_row(float &v, const float iv, bool has_inv, float s, half_t &h, half_t &o,
+                                             half_t dwi, half_t dzi, bool diag, bool scale_ome, float om) {
+  if (has_inv) v *= iv;
+  h = cvt<half_t>(v * s);
+  if (diag) {
+    const half_t t = cvt<half_t>(__builtin_fmaf((float)dwi, (float)h, 0.f));
+    o = cvt<half_t>(__builtin_fmaf((float)dzi, (float)t, 0.f));
+    if (scale_ome) o = cvt<half_t>((float)o * om);
+  }
+}
+template <bool VEC>
 __global__ void k_l4_entry(int64_t n, float *__restrict__ in, const float *__restrict__ inv, float s,
                            half_t *__restrict__ hin, half_t *__restrict__ hout, const half_t *__restrict__ dw,
                            const half_t *__restrict__ dz, const RichDev *st, int scale_ome) {
   const float om = scale_ome ? (float)st->ome[0] : 1.f;
   const float iv = inv ? *inv : 1.f;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+  const bool diag = dw != nullptr;
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
+  int64_t i0 = 0;
+  if constexpr (VEC) {
+    const int64_t n4 = n / 4;
+    for (int64_t q = gid; q < n4; q += gstride) {
+      V4<float> v = ld4(in + 4 * q);
+      V4<half_t> dwv{}, dzv{}, h, o{};
+      if (diag) {
+        dwv = ld4(dw + 4 * q);
+        dzv = ld4(dz + 4 * q);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        l4_entry_row(v.v[c], iv, inv != nullptr, s, h.v[c], o.v[c], dwv.v[c], dzv.v[c], diag, scale_ome != 0, om);
+      if (inv) st4(in + 4 * q, v);
+      st4(hin + 4 * q, h);
+      if (diag) st4(hout + 4 * q, o);
+    }
+    i0 = 4 * n4;
+  }
+  for (int64_t i = i0 + gid; i < n; i += gstride) {
     float v = in[i];
-    if (inv) {
-      v *= iv;
-      in[i] = v;
-    }
-    const half_t h = cvt<half_t>(v * s);
+    half_t h, o = (half_t)0;
+    l4_entry_row(v, iv, inv != nullptr, s, h, o, diag ? dw[i] : (half_t)0, diag ? dz[i] : (half_t)0, diag,
+                 scale_ome != 0, om);
+    if (inv) in[i] = v;
     hin[i] = h;
-    if (dw) {
-      const half_t t = cvt<half_t>(__builtin_fmaf((float)dw[i], (float)h, 0.f));
-      half_t o = cvt<half_t>(__builtin_fmaf((float)dz[i], (float)t, 0.f));
-      if (scale_ome) o = cvt<half_t>((float)o * om);
-      hout[i] = o;
-    }
+    if (diag) hout[i] = o;
   }
 }
 
@@ -467,7 +557,8 @@ struct Solver {
   // the next innermost entry kernel (its first reader) applies it
   const float *pending_inv = nullptr;
   int red_grid = 1;   // capped grid of the outer level's strided reductions
-  int dots_grid = 1;  // one row per thread for the fused reductions
+  int dots_grid = 1;  // blocks of the streaming kernels with a grid reduction
+  int spmv_grid = 0;  // blocks of the SpMV kernels with a fused grid reduction (0 = one row per thread)
   // capture of level-4 inputs (fp32, internal row order): every cap_every-th
   // call until cap_max vectors are held; level 2 runs eagerly meanwhile
   DevPtr cap;
@@ -507,12 +598,15 @@ struct Solver {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     EpiDots<T, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<T>(), counter(0)};
-    launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), dots_grid);
+    launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
   }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
-    hipLaunchKernelGGL((k_cgs_update<T, NV>), dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
+    bool vec = aligned4(L.V[j + 1]);
+    for (int k = 0; k <= j; ++k) vec = vec && aligned4(L.V[k]);
+    auto kern = vec ? k_cgs_update<T, NV, true> : k_cgs_update<T, NV, false>;
+    hipLaunchKernelGGL(kern, dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
                        (double *)nullptr, partial<T>(), counter(1));
@@ -550,7 +644,10 @@ struct Solver {
       constexpr int NV = decltype(nvtag)::value;
       Cols<T, NV> c{};
       for (int k = 0; k < j; ++k) c.p[k] = L.Z[k];
-      hipLaunchKernelGGL((k_solution<T, TO, NV>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
+      bool vec = aligned4(out);
+      for (int k = 0; k < j; ++k) vec = vec && aligned4(L.Z[k]);
+      auto kern = vec ? k_solution<T, TO, NV, true> : k_solution<T, TO, NV, false>;
+      hipLaunchKernelGGL(kern, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
                          (const T *)L.Hp(), L.ldh(), (const T *)L.e->template as<T>(), out);
     };
     if (j <= 4) go(std::integral_constant<int, 4>());
@@ -577,8 +674,13 @@ struct Solver {
     // entry (normalising `in` first when the Arnoldi step that made it did not)
     const float *pinv = pending_inv;
     pending_inv = nullptr;
-    hipLaunchKernelGGL(k_l4_entry, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw, dz,
-                       (const RichDev *)rs, (diag && !probe) ? 1 : 0);
+    const bool vec = aligned4(in) && aligned4(hi) && aligned4(ho) && (!diag || (aligned4(dw) && aligned4(dz)));
+    if (vec)
+      hipLaunchKernelGGL(k_l4_entry<true>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw,
+                         dz, (const RichDev *)rs, (diag && !probe) ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_l4_entry<false>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw,
+                         dz, (const RichDev *)rs, (diag && !probe) ? 1 : 0);
     if (!diag) {
       prec_apply_raw(P.get(), hi, ho, st());
       if (!probe) hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->ome[0]);
@@ -769,7 +871,7 @@ struct Solver {
     SellArgs a = args();
     EpiResidNorm<double> er{V, b_int->as<double>(), e1->as<double>(), inv1->as<double>(), stat,
                             partial<double>(), counter(3)};
-    launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), dots_grid);
+    launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), spmv_grid);
     hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V, (const double *)inv1->as<double>());
     double e = read_scalar(ctx.get(), stat);
     if (record) hist.push_back(e / nrm_b);
@@ -877,6 +979,11 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   // grid reduction costs an atomic round trip, so fewer, longer blocks)
   S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, std::min(kMaxReduceGrid, 8 * ctx->num_cu));
   if (p.grid > 0) S->dots_grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, std::min((int)p.grid, kMaxReduceGrid));
+  // the SpMV kernels with a fused reduction: one row per thread (the gathers
+  // need the waves; the arrivals of 8x more blocks cost less than that), up
+  // to the reduction's block limit; flags bit 1 caps them like the others
+  S->spmv_grid = (p.flags & 2) ? S->dots_grid
+                               : (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxReduceGrid);
   auto dd = [](int64_t cnt) { return dev_alloc(sizeof(double) * std::max<int64_t>(cnt, 1)); };
   S->V1 = dd(n * (p.m1 + 1));
   S->Z1 = dd(n * p.m1);
@@ -893,8 +1000,10 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
     using T = typename std::remove_reference<decltype(*L.V.data())>::type;
     using E = typename std::remove_pointer<T>::type;
     L.m = m;
-    L.Vbuf = dev_alloc(sizeof(E) * n * (m + 1));
-    L.Zbuf = dev_alloc(sizeof(E) * n * m);
+    // columns 64-element aligned, so the streaming kernels can use 16-byte accesses
+    const int64_t ld = (n + 63) / 64 * 64;
+    L.Vbuf = dev_alloc(sizeof(E) * ld * (m + 1));
+    L.Zbuf = dev_alloc(sizeof(E) * ld * m);
     L.H = dev_alloc(sizeof(E) * (m + 1) * m);
     L.cs = dev_alloc(sizeof(E) * m);
     L.sn = dev_alloc(sizeof(E) * m);
@@ -902,8 +1011,8 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
     L.inv = dev_alloc(sizeof(E) * 4);
     L.V.resize(m + 1);
     L.Z.resize(m);
-    for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->template as<E>() + (int64_t)k * n;
-    for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->template as<E>() + (int64_t)k * n;
+    for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->template as<E>() + (int64_t)k * ld;
+    for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->template as<E>() + (int64_t)k * ld;
   };
   if (p.variant == 2) {
     mk(S->L2d, p.m2);

@@ -190,7 +190,8 @@ typedef struct {
   int32_t C, sigma, K; /* SELL layout of the solver's matrices (0 = default: C 64, sigma 1,
                           K 8 for rows of 16 or more entries on average, else 4) */
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
-  int32_t flags;       /* bit 0: record the outer residual history */
+  int32_t flags;       /* bit 0: record the outer residual history; bit 1: run the SpMV
+                          kernels with fused reductions on the capped grid too */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */
