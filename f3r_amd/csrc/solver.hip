@@ -1228,7 +1228,9 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
 
 int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms, double *bytes) {
   F3R_API_BEGIN
-  F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 3, F3R_ERR_ARG, "time_kernel: bad argument");
+  F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 5, F3R_ERR_ARG, "time_kernel: bad argument");
+  F3R_CHECK(which < 4 || (S->s->p.variant == 0 && S->s->L3f.m <= 4), F3R_ERR_UNSUPPORTED,
+            "time_kernel: the Arnoldi-step kernels need the HALF variant with m3 <= 4");
   Solver *s = S->s;
   SellArgs a = s->args(), a16 = s->argsv<half_t>();
   F3R_CHECK(s->p.variant == 0 || which == 3, F3R_ERR_UNSUPPORTED, "time_kernel: fp16/fp32 kernels need the HALF variant");
@@ -1257,6 +1259,14 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
         launch_sell<double, double, double>(s->sh->K, a, s->A64->as<double>(), x64,
                                             EpiStore<double, double>{y64, nullptr}, s->st());
         by = s->sh->format_bytes(8, 8, 8);
+        break;
+      case 4:  // level-3 Arnoldi step j = m3 - 1: fp16 SpMV of Z_j + CGS dots with V_0..V_j
+        s->template spmv_dots<float, half_t, 4>(s->L3f, s->A16->as<half_t>(), s->L3f.m - 1);
+        by = s->sh16->format_bytes(2, 4, 4) + 4.0 * s->n * s->L3f.m;
+        break;
+      case 5:  // level-3 CGS update + norm + Givens of that step
+        s->template cgs_update<float, 4>(s->L3f, s->L3f.m - 1);
+        by = 4.0 * s->n * (s->L3f.m + 2);
         break;
     }
   };

@@ -226,8 +226,9 @@ int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len);
 /* one application of level 2, 3 or 4 as an operator on fp64 vectors, for the
  * per-level error study */
 int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out);
-/* mean device time per launch of one of the solver's SpMVs (which: 0 fp16
- * values and vectors, 1 fp16 values / fp32 vectors, 2 fp32, 3 fp64), measured
+/* mean device time per launch of one of the solver's kernels (which: SpMV
+ * with 0 fp16 values and vectors, 1 fp16 values / fp32 vectors, 2 fp32, 3 fp64;
+ * 4 the level-3 SpMV + CGS dots and 5 the CGS update of its last step), measured
  * with HIP events on the solver's stream over `reps` launches; `bytes` = the
  * bytes the layout must move per launch (values and int32 indices of the
  * nonzeros of int32-index chunks, every value slot of chunk-offset chunks, x, y) */

@@ -134,6 +134,43 @@ def test_graph_replay_identical_to_eager(ctx):
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[1][1], outs[2][1])
 
 
+@pytest.mark.parametrize("kind", ["27pt", "lognormal"])
+def test_layouts_give_identical_solves(ctx, kind):
+    # The chunk-offset format sums every row's nonzeros in CSR order (its
+    # extra slots add +0), and K only regroups the same fma chain.  The
+    # whole solve is therefore bit-identical across index formats and K.
+    import f3r_amd as F
+    A = _problem(kind, 20)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    bv = F.vector.from_numpy(ctx, F.uniform(A.n, -1.0, 1.0, seed=4))
+    outs = []
+    for index, K in ((0, 4), (1, 4), (1, 8), (0, 8), (1, 0)):
+        S = F.F3R(ctx, A, P, index=index, K=K, C=64)
+        xv = F.vector(ctx, "f64", A.n).fill(0.0)
+        r = S.solve(bv, xv)
+        print(f"{kind} index={index} K={K}: {r.res_iter} outer steps")
+        outs.append((r.res_iter, xv.download()))
+    for it, x in outs[1:]:
+        assert it == outs[0][0] and np.array_equal(x, outs[0][1])
+
+
+def test_reference_driver_csv_line():
+    # the C++ driver on the C ABI prints the reference's CSV line
+    # (work/src/f3r-gpu.cpp:97-102): name,F3R,__half,m2,m3,m4,c,SDAINV,acc,Time,Iter,ImplRes,ExplRes
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "f3r_amd", "f3r_gpu")
+    assert os.path.exists(exe), "f3r_amd/f3r_gpu is not built"
+    out = subprocess.run([exe, "gen:27:32:0.5", "1.0", "2", "8", "4", "2", "64", "20"], capture_output=True,
+                         text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    f = out.stdout.strip().split(",")
+    assert f[:8] == ["stencil27_32", "F3R", "__half", "8", "4", "2", "64", "SDAINV"]
+    t, it, impl, expl = float(f[9]), int(f[10]), float(f[11]), float(f[12])
+    assert t > 0 and it > 0 and impl <= 1e-8 and expl <= 1e-8
+
+
 def test_solve_edge_cases(ctx):
     import f3r_amd as F
     A = _problem("27pt", 9)    # n = 729: not a multiple of the chunk height
