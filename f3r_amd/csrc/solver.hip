@@ -114,12 +114,54 @@ template <class T, int NV> struct EpiDots {
     for (int k = 0; k < NV; ++k)
       if (k < ncols) s.v[k] = fma_<T>(acc, p.v[k], s.v[k]);
   }
+  // Block sums only, into partial[k * gridDim.x + block]; k_dots_finish adds
+  // them in the next launch.  A grid reduction finished in this kernel made
+  // every block wait for its stores (the w rows included) and its arrival
+  // atomic before retiring, which cost more than the SpMV's tail: 53 vs
+  // 34 us per launch at 128^3.
   __device__ void finish(State &s) const {
     __shared__ T lds[NV * kWavesPerBlock];
-    if (grid_sum<T, NV>(s.v, partial, counter, lds) && threadIdx.x == 0)
-      for (int k = 0; k < ncols; ++k) Hcol[k] = s.v[k];
+    block_sum<T, NV>(s.v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) partial[(int64_t)k * gridDim.x + blockIdx.x] = s.v[k];
   }
 };
+
+// Hcol[k] = sum over the G block partials of dot k, in block order per
+// thread, then a fixed tree over the 1024 threads
+template <class T, int NV>
+__global__ __launch_bounds__(1024) void k_dots_finish(const T *__restrict__ partial, int G, int ncols, T *Hcol) {
+  __shared__ T lds[NV][16];
+  // every load of a thread is issued before the first add (one memory
+  // latency, not G / 1024 of them); G <= kMaxReduceGrid < 16 * 1024
+  constexpr int kPer = 16;
+  T v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T t[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int b = threadIdx.x + q * 1024;
+      t[q] = (k < ncols && b < G) ? partial[(int64_t)k * G + b] : (T)0;
+    }
+    v[k] = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) v[k] += t[q];
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    v[k] = wave_sum(v[k]);
+    if (lane == 0) lds[k][wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV && threadIdx.x < ncols) {
+    T t = lds[threadIdx.x][0];
+    for (int w = 1; w < 16; ++w) t += lds[threadIdx.x][w];
+    Hcol[threadIdx.x] = t;
+  }
+}
 
 // r = b - A x into V_0 and ||r||: e_0 = ||r||, inv = 1/||r|| (FGMRES::solve +
 // the normalised start of FGMRES::kernel, gmres.hpp:84-102)
@@ -199,8 +241,31 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
     w[i] = wn;
     v[0] = fma_<T>(wn, wn, v[0]);
   }
-  if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
-    T nv = (T)sqrt(v[0]);
+  // block sums of ||w||^2 only; k_cgs_finish adds them (see EpiDots)
+  block_sum<T, 1>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// ||w|| from the G block sums (fixed order), then H(j, j+1), 1/h, Givens and
+// |e_{j+1}| for the host
+template <class T>
+__global__ __launch_bounds__(1024) void k_cgs_finish(const T *__restrict__ partial, int G, T *Hcol, T *cs, T *sn,
+                                                     T *e, int j, int e0_one, T *inv, double *status) {
+  __shared__ T lds[16];
+  constexpr int kPer = 16;  // G <= kMaxReduceGrid < 16 * 1024: all loads in flight at once
+  T t[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) t[q] = threadIdx.x + q * 1024 < G ? partial[threadIdx.x + q * 1024] : (T)0;
+  T v = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) v += t[q];
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = lds[0];
+    for (int w = 1; w < 16; ++w) t += lds[w];
+    T nv = (T)sqrt(t);
     Hcol[j + 1] = nv;
     *inv = (T)1 / nv;
     givens_step<T>(Hcol, cs, sn, e, j, e0_one != 0);
@@ -599,6 +664,10 @@ struct Solver {
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     EpiDots<T, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<T>(), counter(0)};
     launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
+    const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell used
+    const int G = spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
+    hipLaunchKernelGGL((k_dots_finish<T, NV>), dim3(1), dim3(1024), 0, st(), (const T *)partial<T>(), G, j + 1,
+                       L.Hp() + (int64_t)j * L.ldh());
   }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
@@ -610,6 +679,9 @@ struct Solver {
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
                        (double *)nullptr, partial<T>(), counter(1));
+    hipLaunchKernelGGL(k_cgs_finish<T>, dim3(1), dim3(1024), 0, st(), (const T *)partial<T>(), dots_grid,
+                       L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(), L.sn->template as<T>(),
+                       L.e->template as<T>(), j, 1, L.inv->template as<T>(), (double *)nullptr);
   }
   template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
