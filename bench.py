@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--blocks", type=int, default=0, help="blocks of the streaming reduction kernels (0 = auto)")
     ap.add_argument("--cap-spmv", action="store_true", help="run the fused SpMV+reduction kernels on that grid too")
+    ap.add_argument("--host-build", action="store_true", help="build the matrix layouts on the host (default: device)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--learned", type=int, default=4, metavar="N",
@@ -129,7 +130,8 @@ def main():
     ctx = F.Context(0)
     t0 = time.time()
     P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
-    S = F.F3R(ctx, A, P, use_graph=not args.no_graph, grid=args.blocks, cap_spmv=args.cap_spmv)
+    S = F.F3R(ctx, A, P, use_graph=not args.no_graph, grid=args.blocks, cap_spmv=args.cap_spmv,
+              device_build=not args.host_build)
     setup_s = time.time() - t0
     log(f"[bench] setup {setup_s:.1f} s")
     if args.kernel_only:
@@ -204,7 +206,8 @@ def main():
                                f"m=(100;8,4,2), c=64, restart 20, eps 1e-8",
                    "n": A.n, "nnz": A.nnz, "outer_iters": res.res_iter, "solved": bool(res.is_solved),
                    "true_rel_res": float(true_rel), "precond_calls": res.precond_calls,
-                   "setup_s": round(setup_s, 2), "graph": not args.no_graph},
+                   "setup_s": round(setup_s, 2), "layout_build": "host" if args.host_build else "device",
+                   "graph": not args.no_graph},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
                      "traffic_detail": None,

@@ -285,14 +285,15 @@ def sell_layout(A, C=32, sigma=1, K=1):
 class SELL:
     """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166)."""
 
-    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, _handle=None):
+    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, device=False, _handle=None):
         self.ctx = ctx
         self.A = A
         self.dtype = _dtype(dtype)
         self.C, self.sigma, self.K, self.offsets = C, sigma, K, offsets
         if _handle is None:
             h = ctypes.c_void_p()
-            check(lib.f3r_mat_to_sell_ex(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, 1 if offsets else 0,
+            flags = (1 if offsets else 0) | (2 if device else 0)
+            check(lib.f3r_mat_to_sell_ex(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, flags,
                                          ctypes.byref(h)), "mat_to_sell")
             self.handle = h
         else:
@@ -399,7 +400,7 @@ class F3R:
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
                  range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1,
-                 cap_spmv=False):
+                 cap_spmv=False, device_build=False):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -411,7 +412,7 @@ class F3R:
         p.range_scale = range_scale
         p.C, p.sigma, p.K = C, sigma, K
         p.use_graph = 1 if use_graph else 0
-        p.flags = (1 if history else 0) | (2 if cap_spmv else 0)
+        p.flags = (1 if history else 0) | (2 if cap_spmv else 0) | (4 if device_build else 0)
         p.grid = grid
         p.index = index
         self.params = p

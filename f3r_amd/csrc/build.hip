@@ -1,0 +1,313 @@
+// Device build of the SELL-C layout (sigma = 1) from a CSR matrix already on
+// the device, producing the same arrays as the host builder (build_sell):
+// chunk pointers and widths, and for the chunk-offset format the per-chunk
+// offset tables, the compacted int32 indices of the other chunks, and the
+// fp64 values in slot order.
+//
+//   pass 1 (one block per chunk): the widest row; for the offset format the
+//          sorted union of col - row over the chunk (bitonic sort in LDS) and
+//          the eligibility test of the host builder
+//   host:  prefix sums over the chunks (cptr, iptr, table offsets)
+//   pass 2 (one block per chunk): the offset tables
+//   pass 3 (one thread per row): values and indices into their slots
+//
+// A chunk whose entries do not fit the LDS sort buffer is stored explicitly;
+// the host builder has no such limit, so the build refuses the device path
+// when that happens and the caller builds on the host instead.
+#include <algorithm>
+#include <climits>
+#include <numeric>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace f3r {
+
+constexpr int kSortCap = 4096;  // entries of one chunk sorted in LDS (16 KB)
+
+struct ChunkInfo {
+  int32_t width;   // slots of the widest row, or |union| for an offset chunk
+  int32_t offset;  // 1: offset chunk
+  int32_t ulen;    // |union|
+  int32_t umin;    // smallest offset (offset chunks)
+  int32_t toobig;  // 1: the chunk's entries exceed kSortCap (host must build)
+};
+
+__device__ __forceinline__ int block_scan_excl(int v, int *lds, int &total) {
+  // exclusive prefix sum over the block (blockDim.x <= 1024), fixed order
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int a = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += a;
+    __syncthreads();
+  }
+  total = lds[blockDim.x - 1];
+  const int incl = lds[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_scan(int32_t n, int32_t C, const int32_t *__restrict__ rptr,
+                                                    const int32_t *__restrict__ col, int offsets,
+                                                    ChunkInfo *__restrict__ info, int32_t *__restrict__ ubuf) {
+  __shared__ int keys[kSortCap];
+  __shared__ int scan[256];
+  __shared__ int s_mx, s_bad;
+  const int s = blockIdx.x;
+  const int64_t r0 = (int64_t)s * C;
+  const int tc = (int)min<int64_t>(C, n - r0);
+  if (threadIdx.x == 0) {
+    s_mx = 0;
+    s_bad = 0;
+  }
+  __syncthreads();
+  // rows of this thread: j = threadIdx.x + k * 256 (C <= 256 in practice)
+  int cnt = 0;
+  for (int j = threadIdx.x; j < tc; j += blockDim.x) {
+    const int r = (int)(r0 + j);
+    const int len = rptr[r + 1] - rptr[r];
+    atomicMax(&s_mx, len);
+    cnt += len;
+  }
+  int total = 0;
+  const int base = block_scan_excl(cnt, scan, total);
+  const int mx = s_mx;
+  ChunkInfo ci{mx, 0, 0, 0, 0};
+  if (offsets && mx > 0 && total > kSortCap) ci.toobig = 1;
+  if (offsets && mx > 0 && total <= kSortCap) {
+    // (col - row) of every entry, and the column order of every row
+    int p = base;
+    for (int j = threadIdx.x; j < tc; j += blockDim.x) {
+      const int r = (int)(r0 + j);
+      int prev = INT_MIN;
+      for (int k = rptr[r]; k < rptr[r + 1]; ++k) {
+        const int c = col[k];
+        if (c <= prev) s_bad = 1;
+        prev = c;
+        keys[p++] = c - r;
+      }
+    }
+    int m = 1;
+    while (m < total) m <<= 1;
+    for (int i = total + threadIdx.x; i < m; i += blockDim.x) keys[i] = INT_MAX;
+    __syncthreads();
+    // bitonic sort of keys[0, m)
+    for (int k = 2; k <= m; k <<= 1)
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+          const int l = i ^ jj;
+          if (l > i) {
+            const int a = keys[i], b = keys[l];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              keys[i] = b;
+              keys[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    // unique: first occurrences, compacted in order (one pass per thread range)
+    const int per = (total + blockDim.x - 1) / blockDim.x;
+    const int i0 = min(total, (int)threadIdx.x * per), i1 = min(total, i0 + per);
+    int firsts = 0;
+    for (int i = i0; i < i1; ++i) firsts += (i == 0 || keys[i] != keys[i - 1]);
+    int u = 0;
+    const int ub = block_scan_excl(firsts, scan, u);
+    int q = ub;
+    int32_t *out = ubuf + (int64_t)s * kSortCap;
+    for (int i = i0; i < i1; ++i)
+      if (i == 0 || keys[i] != keys[i - 1]) out[q++] = keys[i];
+    const int umin = keys[0], umax = keys[total - 1];
+    const bool ok = !s_bad && (int64_t)u * 4 <= (int64_t)mx * 5 && r0 + umin >= 0 && r0 + tc - 1 + umax < n;
+    if (ok) {
+      ci.width = u;
+      ci.offset = 1;
+      ci.ulen = u;
+      ci.umin = umin;
+    }
+  }
+  if (threadIdx.x == 0) info[s] = ci;
+}
+
+// offset tables: coff[tpos[s] + t] = union[t] (0 past |union|, up to the K-padded width)
+__global__ void k_coff_fill(const ChunkInfo *__restrict__ info, const int32_t *__restrict__ ubuf,
+                            const int32_t *__restrict__ tpos, const int32_t *__restrict__ cw, int K,
+                            int32_t *__restrict__ coff) {
+  const int s = blockIdx.x;
+  const ChunkInfo ci = info[s];
+  if (!ci.offset) return;
+  const int w = cw[s] * K;
+  for (int t = threadIdx.x; t < w; t += blockDim.x)
+    coff[tpos[s] + t] = t < ci.ulen ? ubuf[(int64_t)s * kSortCap + t] : 0;
+}
+
+// values and indices of row r into their slots (val zeroed beforehand)
+__global__ void k_sell_fill(int32_t n, int32_t C, int K, const int32_t *__restrict__ rptr,
+                            const int32_t *__restrict__ col, const double *__restrict__ a,
+                            const ChunkInfo *__restrict__ info, const int32_t *__restrict__ ubuf,
+                            const int32_t *__restrict__ cptr, const int32_t *__restrict__ cw,
+                            const int32_t *__restrict__ iptr, int32_t *__restrict__ idx, double *__restrict__ val) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = r / C, j = r - s * C;
+  const int tc = min(C, n - s * C);
+  const ChunkInfo ci = info[s];
+  const int64_t c0 = cptr[s];
+  if (ci.offset) {
+    const int32_t *u = ubuf + (int64_t)s * kSortCap;
+    int t = 0;
+    for (int k = rptr[r]; k < rptr[r + 1]; ++k) {
+      const int d = col[k] - r;
+      while (u[t] < d) ++t;  // columns ascend: the slot only moves forward
+      val[c0 + ((int64_t)(t / K) * tc + j) * K + (t % K)] = a[k];
+    }
+    return;
+  }
+  const int width = cw[s] * K;
+  const int len = rptr[r + 1] - rptr[r];
+  const int64_t ib = (int64_t)iptr[s] - c0;
+  int last = 0;
+  for (int q = 0; q < width; ++q) {
+    const int64_t pos = c0 + ((int64_t)(q / K) * tc + j) * K + (q % K);
+    if (q < len) {
+      last = col[rptr[r] + q];
+      val[pos] = a[rptr[r] + q];
+    } else {
+      val[pos] = 0.0;
+    }
+    idx[pos + ib] = last;
+  }
+}
+
+__global__ void k_zero_f64(int64_t n, double *p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+template <class D> __global__ void k_vals_cast(int64_t n, const double *__restrict__ s, D *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = cvt<D>(s[i]);
+}
+
+DevPtr cast_values_device(Context *c, const DevPtr &v64, int64_t n, int dtype) {
+  if (dtype == F3R_F64) {
+    DevPtr d = dev_alloc(sizeof(double) * std::max<int64_t>(1, n));
+    if (n) F3R_HIP(hipMemcpyAsync(d->p, v64->p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    return d;
+  }
+  DevPtr d = dev_alloc(dtype_size(dtype) * std::max<int64_t>(1, n));
+  if (dtype == F3R_F32)
+    hipLaunchKernelGGL(k_vals_cast<float>, dim3(2048), dim3(256), 0, c->stream, n, (const double *)v64->as<double>(),
+                       d->as<float>());
+  else
+    hipLaunchKernelGGL(k_vals_cast<half_t>, dim3(2048), dim3(256), 0, c->stream, n,
+                       (const double *)v64->as<double>(), d->as<half_t>());
+  F3R_HIP(hipGetLastError());
+  return d;
+}
+
+static DevPtr upload_vec(Context *c, const std::vector<int32_t> &v) {
+  DevPtr d = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, v.size()));
+  if (!v.empty()) F3R_HIP(hipMemcpyAsync(d->p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice, c->stream));
+  return d;
+}
+
+// Builds the layout of the device CSR (d_rptr, d_col, d_val; host rptr for
+// the row counts) on the device.  Returns false, building nothing, when a
+// chunk's entries exceed the sort buffer.  Values come back as fp64 in slot
+// order (sval); the caller converts them.
+bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col,
+                       const double *d_val, int32_t C, int32_t K, bool offsets, std::shared_ptr<SellShared> &sh,
+                       DevPtr &sval) {
+  hipStream_t st = c->stream;
+  const int32_t n = A.n;
+  offsets = offsets && C >= 64 && n < (1 << 27);
+  const int32_t nch = (n + C - 1) / C;
+  DevPtr dinfo = dev_alloc(sizeof(ChunkInfo) * std::max(1, nch));
+  DevPtr ubuf = dev_alloc(offsets ? sizeof(int32_t) * (size_t)kSortCap * nch : 4);
+  if (nch > 0)
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nch), dim3(256), 0, st, n, C, d_rptr, d_col, offsets ? 1 : 0,
+                       dinfo->as<ChunkInfo>(), ubuf->as<int32_t>());
+  F3R_HIP(hipGetLastError());
+  std::vector<ChunkInfo> info(nch);
+  if (nch) F3R_HIP(hipMemcpyAsync(info.data(), dinfo->p, sizeof(ChunkInfo) * nch, hipMemcpyDeviceToHost, st));
+  F3R_HIP(hipStreamSynchronize(st));
+  for (const ChunkInfo &ci : info)
+    if (ci.toobig) return false;
+  auto round_up = [](int32_t v, int32_t m) { return (v + m - 1) / m * m; };
+  std::vector<int32_t> cptr(nch + 1, 0), cw(nch), iptr(offsets ? nch : 0), tpos(nch, 0);
+  int64_t tot = 0, itot = 0, ctot = 0, off_slots = 0, expl_nnz = 0;
+  int32_t off_chunks = 0, off_bias = 0;
+  for (int32_t s = 0; s < nch; ++s) {
+    const int32_t tc = std::min<int32_t>(C, n - s * C);
+    cw[s] = round_up(info[s].width, K) / K;
+    tot += (int64_t)cw[s] * K * tc;
+    F3R_CHECK(tot < INT32_MAX, F3R_ERR_RANGE, "SELL: more than 2^31 slots");
+    cptr[s + 1] = (int32_t)tot;
+    if (offsets) {
+      if (info[s].offset) {
+        iptr[s] = -(int32_t)ctot - 1;
+        tpos[s] = (int32_t)ctot;
+        ctot += (int64_t)cw[s] * K;
+        off_slots += (int64_t)cw[s] * K * tc;
+        off_bias = std::max(off_bias, -info[s].umin);
+        ++off_chunks;
+      } else {
+        iptr[s] = (int32_t)itot;
+        itot += cptr[s + 1] - cptr[s];
+        expl_nnz += A.rptr[std::min<int64_t>(n, (int64_t)(s + 1) * C)] - A.rptr[(int64_t)s * C];
+      }
+    } else {
+      expl_nnz += A.rptr[std::min<int64_t>(n, (int64_t)(s + 1) * C)] - A.rptr[(int64_t)s * C];
+    }
+  }
+  if (!offsets) itot = tot;
+  sh = std::make_shared<SellShared>();
+  sh->n = n;
+  sh->C = C;
+  sh->K = K;
+  sh->nchunks = nch;
+  sh->nslots = tot;
+  sh->internal = true;
+  sh->offset_chunks = off_chunks;
+  sh->offset_slots = off_slots;
+  sh->explicit_nnz = expl_nnz;
+  sh->off_bias = off_bias;
+  sh->cptr = upload_vec(c, cptr);
+  sh->cwidth = upload_vec(c, cw);
+  sh->idx = dev_alloc(sizeof(int32_t) * std::max<int64_t>(1, itot));
+  sh->h_perm.resize(n);
+  std::iota(sh->h_perm.begin(), sh->h_perm.end(), 0);
+  DevPtr d_iptr;
+  if (offsets) {
+    sh->iptr = upload_vec(c, iptr);
+    d_iptr = sh->iptr;
+    sh->coff = dev_alloc(sizeof(int32_t) * std::max<int64_t>(1, ctot));
+    DevPtr d_tpos = upload_vec(c, tpos);
+    if (nch)
+      hipLaunchKernelGGL(k_coff_fill, dim3(nch), dim3(64), 0, st, (const ChunkInfo *)dinfo->as<ChunkInfo>(),
+                         (const int32_t *)ubuf->as<int32_t>(), (const int32_t *)d_tpos->as<int32_t>(),
+                         (const int32_t *)sh->cwidth->as<int32_t>(), K, sh->coff->as<int32_t>());
+    F3R_HIP(hipGetLastError());
+    F3R_HIP(hipStreamSynchronize(st));  // d_tpos is released at scope end
+  } else {
+    d_iptr = sh->cptr;  // explicit everywhere: indices at the value offsets
+  }
+  sval = dev_alloc(sizeof(double) * std::max<int64_t>(1, tot));
+  hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, st, std::max<int64_t>(1, tot), sval->as<double>());
+  if (n > 0)
+    hipLaunchKernelGGL(k_sell_fill, dim3((n + 255) / 256), dim3(256), 0, st, n, C, K, d_rptr, d_col, d_val,
+                       (const ChunkInfo *)dinfo->as<ChunkInfo>(), (const int32_t *)ubuf->as<int32_t>(),
+                       (const int32_t *)sh->cptr->as<int32_t>(), (const int32_t *)sh->cwidth->as<int32_t>(),
+                       (const int32_t *)d_iptr->as<int32_t>(), sh->idx->as<int32_t>(), sval->as<double>());
+  F3R_HIP(hipGetLastError());
+  F3R_HIP(hipStreamSynchronize(st));
+  return true;
+}
+
+}  // namespace f3r

@@ -130,6 +130,7 @@ struct Mat {
   std::shared_ptr<SellShared> sh;
   DevPtr s_val;
   std::shared_ptr<std::vector<double>> sell_host_val;  // fp64 slot values, for casts
+  DevPtr sell_dev_val64;  // fp64 slot values on the device (layouts built there)
 };
 
 // ----------------------------------------------------- host set-up API ----

@@ -769,7 +769,27 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
   F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
   *out = nullptr;
-  F3R_CHECK((flags & ~1) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  F3R_CHECK((flags & ~3) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  if ((flags & 2) && sigma == 1 && csr->m->d_rptr) {  // build on the device
+    std::shared_ptr<SellShared> sh;
+    DevPtr v64;
+    if (build_sell_device(ctx->c.get(), *csr->m->host, csr->m->d_rptr->as<int32_t>(), csr->m->d_idx->as<int32_t>(),
+                          csr->m->d_val->as<double>(), C, K, (flags & 1) != 0, sh, v64)) {
+      auto m = std::make_shared<Mat>();
+      m->ctx = ctx->c;
+      m->format = 1;
+      m->dtype = dtype;
+      m->host = csr->m->host;
+      sh->internal = false;
+      sh->nnz = csr->m->host->nnz();
+      m->sh = sh;
+      m->s_val = cast_values_device(ctx->c.get(), v64, sh->nslots, dtype);
+      m->sell_dev_val64 = v64;
+      F3R_HIP(hipStreamSynchronize(ctx->c->stream));
+      *out = new f3r_mat_s{m};
+      return F3R_OK;
+    }
+  }
   HostSell hs = build_sell(*csr->m->host, C, sigma, K, nullptr, false, (flags & 1) != 0);
   auto m = std::make_shared<Mat>();
   m->ctx = ctx->c;
@@ -832,11 +852,14 @@ int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, c
 
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
   F3R_API_BEGIN
-  F3R_CHECK(sell && out && sell->m->format == 1 && sell->m->sell_host_val, F3R_ERR_ARG,
+  F3R_CHECK(sell && out && sell->m->format == 1 && (sell->m->sell_host_val || sell->m->sell_dev_val64), F3R_ERR_ARG,
             "mat_cast: needs a SELL matrix");
   auto m = std::make_shared<Mat>(*sell->m);
   m->dtype = dtype;
-  m->s_val = upload_values(m->ctx.get(), *m->sell_host_val, dtype);
+  if (m->sell_host_val)
+    m->s_val = upload_values(m->ctx.get(), *m->sell_host_val, dtype);
+  else
+    m->s_val = cast_values_device(m->ctx.get(), m->sell_dev_val64, m->sh->nslots, dtype);
   *out = new f3r_mat_s{m};
   F3R_API_END
 }

@@ -66,4 +66,11 @@ Vec *vec_of(f3r_vec v);
 std::shared_ptr<Mat> mat_of(f3r_mat m);
 std::shared_ptr<Prec> prec_of(f3r_prec p);
 f3r_prec wrap_prec(std::shared_ptr<Prec> p);  // new handle owning a reference
+// device build of a sigma = 1 SELL layout from a device CSR (build.hip);
+// false (nothing built) when a chunk is too long for the on-chip sort
+bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col,
+                       const double *d_val, int32_t C, int32_t K, bool offsets, std::shared_ptr<SellShared> &sh,
+                       DevPtr &sval);
+// fp64 slot values on the device -> dtype (correctly rounded)
+DevPtr cast_values_device(Context *c, const DevPtr &v64, int64_t n, int dtype);
 }  // namespace f3r

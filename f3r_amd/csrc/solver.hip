@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <numeric>
 
 #include "common.hpp"
 #include "internal.hpp"
@@ -987,7 +988,8 @@ struct Solver {
 // ----------------------------------------------------------- set-up ------
 static int ilog2_floor(double v) { return v < 1 ? 0 : (int)std::floor(std::log2(v)); }
 
-Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p) {
+Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p,
+                      const Mat *dev = nullptr) {
   F3R_CHECK(p.variant >= 0 && p.variant <= 2, F3R_ERR_ARG, "solver: variant must be 0 (HALF), 1 (FLOAT) or 2 (DOUBLE)");
   F3R_CHECK(p.m1 >= 1 && p.m1 <= 120, F3R_ERR_ARG, "solver: m1 must be in [1, 120]");
   F3R_CHECK(p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16, F3R_ERR_ARG, "solver: m2, m3 in [1, 16]");
@@ -1008,28 +1010,65 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   const int K16 = p.K > 0 ? p.K : (A.n > 0 && A.nnz() >= (int64_t)16 * A.n ? 8 : 4);
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
   F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "solver: K must be 1, 2, 4 or 8");
-  HostSell hs = build_sell(A, C, sigma, K, nullptr, true, p.index != 0);
-  bool ident = true;
-  for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
-  S->sh = upload_sell_shared(ctx.get(), hs, true);
-  S->sh->nnz = A.nnz();
-  S->sh->sigma = sigma;
-  S->sh16 = S->sh;
-  // the level matrices: fp64 always (outer level; Richardson of FLOAT and
-  // DOUBLE), fp32 for levels 2-3 of HALF / FLOAT, fp16 for HALF
-  S->A64 = upload_sell_values(ctx.get(), hs, F3R_F64);
-  if (p.variant != 2) S->A32 = upload_sell_values(ctx.get(), hs, F3R_F32);
-  if (p.variant == 0) {
-    if (K16 == K) {
-      S->A16 = upload_sell_values(ctx.get(), hs, F3R_F16);
-    } else {
-      HostSell h16 = build_sell(A, C, sigma, K16, nullptr, true, p.index != 0);
-      S->sh16 = upload_sell_shared(ctx.get(), h16, true);
-      S->sh16->nnz = A.nnz();
-      S->sh16->sigma = sigma;
-      S->A16 = upload_sell_values(ctx.get(), h16, F3R_F16);
+  // flags bit 2: build the layouts on the device from the CSR's device copy
+  // (sigma = 1; the same arrays as the host build)
+  const bool on_dev = (p.flags & 4) && sigma == 1 && dev && dev->d_rptr && dev->ctx == ctx;
+  HostSell hs;
+  bool built = false;
+  if (on_dev) {
+    std::shared_ptr<SellShared> sh, sh16;
+    DevPtr v64, v64_16;
+    built = build_sell_device(ctx.get(), A, dev->d_rptr->as<int32_t>(), dev->d_idx->as<int32_t>(),
+                              dev->d_val->as<double>(), C, K, p.index != 0, sh, v64);
+    if (built && p.variant == 0 && K16 != K)
+      built = build_sell_device(ctx.get(), A, dev->d_rptr->as<int32_t>(), dev->d_idx->as<int32_t>(),
+                                dev->d_val->as<double>(), C, K16, p.index != 0, sh16, v64_16);
+    if (built) {
+      S->sh = sh;
+      S->sh->nnz = A.nnz();
+      S->sh->sigma = 1;
+      S->sh16 = S->sh;
+      S->A64 = cast_values_device(ctx.get(), v64, sh->nslots, F3R_F64);
+      if (p.variant != 2) S->A32 = cast_values_device(ctx.get(), v64, sh->nslots, F3R_F32);
+      if (p.variant == 0) {
+        if (K16 == K) {
+          S->A16 = cast_values_device(ctx.get(), v64, sh->nslots, F3R_F16);
+        } else {
+          S->sh16 = sh16;
+          S->sh16->nnz = A.nnz();
+          S->sh16->sigma = 1;
+          S->A16 = cast_values_device(ctx.get(), v64_16, sh16->nslots, F3R_F16);
+        }
+      }
+      F3R_HIP(hipStreamSynchronize(ctx->stream));
+      hs.perm.resize(A.n);
+      std::iota(hs.perm.begin(), hs.perm.end(), 0);
     }
   }
+  if (!built) {
+    hs = build_sell(A, C, sigma, K, nullptr, true, p.index != 0);
+    S->sh = upload_sell_shared(ctx.get(), hs, true);
+    S->sh->nnz = A.nnz();
+    S->sh->sigma = sigma;
+    S->sh16 = S->sh;
+    // the level matrices: fp64 always (outer level; Richardson of FLOAT and
+    // DOUBLE), fp32 for levels 2-3 of HALF / FLOAT, fp16 for HALF
+    S->A64 = upload_sell_values(ctx.get(), hs, F3R_F64);
+    if (p.variant != 2) S->A32 = upload_sell_values(ctx.get(), hs, F3R_F32);
+    if (p.variant == 0) {
+      if (K16 == K) {
+        S->A16 = upload_sell_values(ctx.get(), hs, F3R_F16);
+      } else {
+        HostSell h16 = build_sell(A, C, sigma, K16, nullptr, true, p.index != 0);
+        S->sh16 = upload_sell_shared(ctx.get(), h16, true);
+        S->sh16->nnz = A.nnz();
+        S->sh16->sigma = sigma;
+        S->A16 = upload_sell_values(ctx.get(), h16, F3R_F16);
+      }
+    }
+  }
+  bool ident = true;
+  for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
   // innermost preconditioner in the internal order and the innermost precision
   const int pdt = p.variant == 0 ? F3R_F16 : p.variant == 1 ? F3R_F32 : F3R_F64;
   auto P = std::make_shared<Prec>();
@@ -1200,7 +1239,7 @@ int f3r_solver_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const f3r_params *para
   auto m = mat_of(A);
   F3R_CHECK(m->host, F3R_ERR_ARG, "solver_create: A must carry its CSR");
   *S = nullptr;
-  Solver *s = solver_create(ctx_of(ctx), *m->host, prec_of(P).get(), *params);
+  Solver *s = solver_create(ctx_of(ctx), *m->host, prec_of(P).get(), *params, m.get());
   *S = new f3r_solver_s{s};
   F3R_API_END
 }

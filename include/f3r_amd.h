@@ -125,6 +125,9 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
  * them once per chunk instead of a column index per slot.  The sums keep the
  * CSR order of every row.  No counterpart in the reference. */
 #define F3R_SELL_OFFSETS 1
+/* flag bit 1: build the layout on the device from the CSR's device copy
+ * (sigma = 1 only; identical arrays to the host build) */
+#define F3R_SELL_DEVICE 2
 int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
                        int32_t K, int32_t flags, f3r_mat *sell);
 /* host-only SELL-C-sigma layout of a CSR matrix (csr_to_sell /
@@ -191,7 +194,9 @@ typedef struct {
                           K 8 for rows of 16 or more entries on average, else 4) */
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history; bit 1: run the SpMV
-                          kernels with fused reductions on the capped grid too */
+                          kernels with fused reductions on the capped grid too; bit 2:
+                          build the matrix layouts on the device (sigma 1; needs A created
+                          with a context) */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */

@@ -154,6 +154,20 @@ def test_layouts_give_identical_solves(ctx, kind):
         assert it == outs[0][0] and np.array_equal(x, outs[0][1])
 
 
+def test_device_built_layouts_give_identical_solves(ctx):
+    import f3r_amd as F
+    A = _problem("lognormal", 18)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    bv = F.vector.from_numpy(ctx, F.uniform(A.n, -1.0, 1.0, seed=8))
+    outs = []
+    for dev in (False, True):
+        S = F.F3R(ctx, A, P, device_build=dev)
+        xv = F.vector(ctx, "f64", A.n).fill(0.0)
+        r = S.solve(bv, xv)
+        outs.append((r.res_iter, xv.download()))
+    assert outs[0][0] == outs[1][0] and np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_reference_driver_csv_line():
     # the C++ driver on the C ABI prints the reference's CSV line
     # (work/src/f3r-gpu.cpp:97-102): name,F3R,__half,m2,m3,m4,c,SDAINV,acc,Time,Iter,ImplRes,ExplRes

@@ -194,3 +194,23 @@ def test_ilu0_apply_matches_oracle(ctx):
         bad = np.flatnonzero(got != o16)
         print(kind, "ilu16 mismatches", bad.size, bad[:8], got[bad[:8]].view(np.float16), o16[bad[:8]].view(np.float16))
         assert bad.size == 0, kind
+
+
+@pytest.mark.parametrize("C,K,offsets", [(64, 4, True), (128, 8, True), (64, 4, False), (256, 2, True)])
+def test_device_sell_build_matches_host(ctx, C, K, offsets):
+    # the device-built layout equals the host-built one: same slot count and
+    # chunk split, and bit-identical SpMV results in fp64 and fp16
+    import f3r_amd as F
+    for name, A in _mats():
+        print(f"device build {name} C={C} K={K} offsets={offsets}", flush=True)
+        Hs = F.SELL(ctx, A, "f64", C=C, sigma=1, K=K, offsets=offsets)
+        Ds = F.SELL(ctx, A, "f64", C=C, sigma=1, K=K, offsets=offsets, device=True)
+        assert Ds.nslots() == Hs.nslots() and Ds.chunks() == Hs.chunks(), name
+        x = np.random.default_rng(3).uniform(-1, 1, A.n)
+        for dt, np_t in (("f64", np.float64), ("f16", np.float16)):
+            Hd, Dd = (Hs, Ds) if dt == "f64" else (Hs.cast(dt), Ds.cast(dt))
+            xv = F.vector.from_numpy(ctx, x.astype(np_t))
+            y1, y2 = F.vector(ctx, dt, A.n), F.vector(ctx, dt, A.n)
+            Hd.operate(xv, y1)
+            Dd.operate(xv, y2)
+            assert np.array_equal(y1.download().view(np.uint8), y2.download().view(np.uint8)), (name, dt)
