@@ -171,6 +171,14 @@ class CSR:
         d[rows[on]] = self.val[on]
         return d
 
+    def factor_levels(self, both=True):
+        """(level of every row, number of levels) of the device factorisations:
+        row i waits for the rows j < i with a_ij != 0 and, when `both`, a_ji != 0"""
+        lev = np.empty(self.n, np.int32)
+        nl = ctypes.c_int32()
+        check(lib.f3r_factor_levels(self.handle(), 1 if both else 0, _p(lev), ctypes.byref(nl)), "factor_levels")
+        return lev, nl.value
+
     def handle(self, ctx=None):
         """C-ABI matrix handle on ctx (None: host-only)."""
         key = id(ctx) if ctx is not None else None
@@ -337,6 +345,16 @@ class SELL:
 
 
 class _Precond:
+    SETUP_WHERE = {0: "host", 1: "device", 2: "host (device capacity exceeded)"}
+
+    @property
+    def where(self):
+        """where the factors were built: "host", "device", or "host (device
+        capacity exceeded)" when the device set-up handed the rows back"""
+        w = ctypes.c_int()
+        check(lib.f3r_prec_setup_where(self.handle, ctypes.byref(w)), "prec_setup_where")
+        return self.SETUP_WHERE[w.value]
+
     def factors(self):
         out = []
         for w in (0, 1):
@@ -363,15 +381,16 @@ class _Precond:
 
 class SDAINV(_Precond):
     """SDAINV<T, device>(A, {tol, acc}) (sparse/preconditioner/ainv.hpp:129-147).
-    factors() -> (Z, W^T) as CSR triples.  ctx=None builds the host factors only."""
+    factors() -> (Z, W^T) as CSR triples.  ctx=None builds the host factors only;
+    device=True factorises on the GPU (bit-identical factors)."""
 
-    def __init__(self, A, tol=0.1, acc=1.0, dtype="f16", ctx=None):
+    def __init__(self, A, tol=0.1, acc=1.0, dtype="f16", ctx=None, device=False):
         self.n = A.n
         self.kind = 0
         self.dtype = _dtype(dtype)
         h = ctypes.c_void_p()
-        check(lib.f3r_sdainv_create(ctx.handle if ctx else None, A.handle(ctx), tol, acc, self.dtype,
-                                    ctypes.byref(h)), "sdainv_create")
+        check(lib.f3r_sdainv_create_ex(ctx.handle if ctx else None, A.handle(ctx), tol, acc, self.dtype,
+                                       1 if device else 0, ctypes.byref(h)), "sdainv_create")
         self.handle = h
 
 
@@ -379,13 +398,13 @@ class ILU0(_Precond):
     """ILU(0) on `blocks` diagonal blocks + split (ilu.hpp:128-164,
     algorithm.hpp:8-43).  factors() -> (L, U)."""
 
-    def __init__(self, A, blocks=1, acc=1.0, dtype="f16", ctx=None):
+    def __init__(self, A, blocks=1, acc=1.0, dtype="f16", ctx=None, device=False):
         self.n = A.n
         self.kind = 1
         self.dtype = _dtype(dtype)
         h = ctypes.c_void_p()
-        check(lib.f3r_ilu0_create(ctx.handle if ctx else None, A.handle(ctx), blocks, acc, self.dtype,
-                                  ctypes.byref(h)), "ilu0_create")
+        check(lib.f3r_ilu0_create_ex(ctx.handle if ctx else None, A.handle(ctx), blocks, acc, self.dtype,
+                                     1 if device else 0, ctypes.byref(h)), "ilu0_create")
         self.handle = h
 
 

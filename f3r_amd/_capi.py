@@ -77,6 +77,12 @@ SIGNATURES = [
     ("f3r_spmv_time", ctypes.c_int, [c_vp, c_vp, c_vp, c_i32, P_dbl]),
     ("f3r_sdainv_create", ctypes.c_int, [c_vp, c_vp, c_dbl, c_dbl, ctypes.c_int, ctypes.POINTER(c_vp)]),
     ("f3r_ilu0_create", ctypes.c_int, [c_vp, c_vp, c_i32, c_dbl, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    ("f3r_sdainv_create_ex", ctypes.c_int, [c_vp, c_vp, c_dbl, c_dbl, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(c_vp)]),
+    ("f3r_ilu0_create_ex", ctypes.c_int, [c_vp, c_vp, c_i32, c_dbl, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(c_vp)]),
+    ("f3r_prec_setup_where", ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int)]),
+    ("f3r_factor_levels", ctypes.c_int, [c_vp, ctypes.c_int, c_vp, P_i32]),
     ("f3r_prec_apply", ctypes.c_int, [c_vp, c_vp, c_vp]),
     ("f3r_prec_factor_nnz", ctypes.c_int, [c_vp, ctypes.c_int, P_i64]),
     ("f3r_prec_factor_get", ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp]),

@@ -162,6 +162,17 @@ struct HostFactors {
 };
 HostFactors sdainv_host(const HostCSR &A, double tol, double acc);
 HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc);
+HostCSR csr_transpose(const HostCSR &A);
+// pieces of ilu0_host: the block-diagonal part of A, the position of each
+// row's diagonal entry in it, and the split of the eliminated rows into L, U
+HostCSR ilu0_restrict(const HostCSR &A, int32_t blocks);
+std::vector<int32_t> ilu0_diag_pos(const HostCSR &R);
+HostFactors ilu0_split(const HostCSR &R);
+// Level sets of a row-by-row factorisation: row i waits for every row j < i
+// with A[i][j] != 0 and, when `both`, with A[j][i] != 0.  rows[lptr[l] ..
+// lptr[l + 1]) lists the rows of level l in increasing order; returns the
+// number of levels.
+int32_t factor_levels(const HostCSR &A, bool both, std::vector<int32_t> &lptr, std::vector<int32_t> &rows);
 // fp64 -> fp16 bits, correctly rounded (round to nearest even)
 uint16_t d2h_bits(double d);
 

@@ -928,7 +928,8 @@ int f3r_spmv_time(f3r_mat A, f3r_vec x, f3r_vec y, int32_t reps, double *ms) {
   F3R_API_END
 }
 
-static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc, int dtype, f3r_prec *P) {
+static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc, int dtype, int flags,
+                       f3r_prec *P) {
   F3R_API_BEGIN
   F3R_CHECK(csr && P && csr->m->host, F3R_ERR_ARG, "precond: needs a CSR matrix");
   F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16, F3R_ERR_ARG, "precond: dtype");
@@ -939,10 +940,22 @@ static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc
   p->dtype = dtype;
   p->tol = p1;
   p->acc = acc;
-  if (kind == 0)
-    p->host = sdainv_host(*csr->m->host, p1, acc);
-  else
-    p->host = ilu0_host(*csr->m->host, (int32_t)p1, acc);
+  const bool dev = (flags & F3R_SETUP_DEVICE) != 0;
+  F3R_CHECK(!dev || (p->ctx && csr->m->d_rptr), F3R_ERR_ARG, "precond: a device set-up needs a context");
+  bool built = false;
+  if (dev) {
+    const Mat &m = *csr->m;
+    built = kind == 0 ? sdainv_device(p->ctx.get(), *m.host, m.d_rptr->as<int32_t>(), m.d_idx->as<int32_t>(),
+                                      m.d_val->as<double>(), p1, acc, p->host)
+                      : ilu0_device(p->ctx.get(), *m.host, (int32_t)p1, acc, p->host);
+    p->where = built ? 1 : 2;
+  }
+  if (!built) {
+    if (kind == 0)
+      p->host = sdainv_host(*csr->m->host, p1, acc);
+    else
+      p->host = ilu0_host(*csr->m->host, (int32_t)p1, acc);
+  }
   p->n = csr->m->host->n;
   if (p->ctx) prec_build_device(p.get(), nullptr, 64, 1);
   *P = new f3r_prec_s{p};
@@ -950,10 +963,31 @@ static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc
 }
 
 int f3r_sdainv_create(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtype, f3r_prec *P) {
-  return prec_create(ctx, csr, 0, tol, acc, dtype, P);
+  return prec_create(ctx, csr, 0, tol, acc, dtype, 0, P);
 }
 int f3r_ilu0_create(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype, f3r_prec *P) {
-  return prec_create(ctx, csr, 1, (double)blocks, acc, dtype, P);
+  return prec_create(ctx, csr, 1, (double)blocks, acc, dtype, 0, P);
+}
+int f3r_sdainv_create_ex(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtype, int flags, f3r_prec *P) {
+  return prec_create(ctx, csr, 0, tol, acc, dtype, flags, P);
+}
+int f3r_ilu0_create_ex(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype, int flags, f3r_prec *P) {
+  return prec_create(ctx, csr, 1, (double)blocks, acc, dtype, flags, P);
+}
+int f3r_factor_levels(f3r_mat csr, int both, int32_t *level, int32_t *nlevels) {
+  F3R_API_BEGIN
+  F3R_CHECK(csr && csr->m->host && level && nlevels, F3R_ERR_ARG, "factor_levels: needs a CSR matrix");
+  std::vector<int32_t> lptr, rows;
+  *nlevels = factor_levels(*csr->m->host, both != 0, lptr, rows);
+  for (size_t l = 0; l + 1 < lptr.size(); ++l)
+    for (int32_t k = lptr[l]; k < lptr[l + 1]; ++k) level[rows[k]] = (int32_t)l;
+  F3R_API_END
+}
+int f3r_prec_setup_where(f3r_prec P, int *where) {
+  F3R_API_BEGIN
+  F3R_CHECK(P && where, F3R_ERR_ARG, "null argument");
+  *where = P->p->where;
+  F3R_API_END
 }
 int f3r_prec_apply(f3r_prec P, f3r_vec in, f3r_vec out) {
   F3R_API_BEGIN

@@ -288,7 +288,9 @@ void merge_axpy(double alpha, const SparseRow &x, const SparseRow &y, double tol
   }
 }
 
-HostCSR transpose(const HostCSR &A) {
+}  // namespace
+
+HostCSR csr_transpose(const HostCSR &A) {
   HostCSR T;
   T.n = A.ncols ? A.ncols : A.n;
   T.ncols = A.n;
@@ -307,6 +309,7 @@ HostCSR transpose(const HostCSR &A) {
   return T;
 }
 
+namespace {
 HostCSR rows_to_csr(const std::vector<SparseRow> &rows) {
   HostCSR R;
   R.n = R.ncols = (int32_t)rows.size();
@@ -324,7 +327,7 @@ HostCSR rows_to_csr(const std::vector<SparseRow> &rows) {
 
 HostFactors sdainv_host(const HostCSR &A, double tol, double acc) {
   const int32_t N = A.n;
-  HostCSR At = transpose(A);
+  HostCSR At = csr_transpose(A);
   std::vector<double> d(N, 0.0);
   std::vector<SparseRow> z(N), w(N);
   for (int32_t i = 0; i < N; ++i) {
@@ -368,7 +371,7 @@ HostFactors sdainv_host(const HostCSR &A, double tol, double acc) {
   // Z = (rows z_i / d_i)^T, W^T = rows w_i (ainv.hpp:57-64)
   for (int32_t i = 0; i < N; ++i)
     for (double &v : z[i].v) v /= d[i];
-  F.f[0] = transpose(rows_to_csr(z));
+  F.f[0] = csr_transpose(rows_to_csr(z));
   F.f[1] = rows_to_csr(w);
   return F;
 }
@@ -378,7 +381,7 @@ HostFactors sdainv_host(const HostCSR &A, double tol, double acc) {
 // multiplied by acc before elimination (ilu.hpp:128-164,
 // csr.hpp:109-155), then split into unit L (1 stored last in each row) and
 // U with its diagonal first (algorithm.hpp:8-43).
-HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
+HostCSR ilu0_restrict(const HostCSR &A, int32_t blocks) {
   const int32_t n = A.n;
   F3R_CHECK(blocks >= 1 && blocks <= n, F3R_ERR_ARG, "ILU(0): blocks must be in [1, n]");
   const int32_t bs = (n + blocks - 1) / blocks;
@@ -387,7 +390,6 @@ HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
   std::vector<int32_t> bnd(blocks + 1);
   for (int32_t b = 0; b < blocks; ++b) bnd[b] = b * bs;
   bnd[blocks] = n;
-  // block-diagonal part
   HostCSR R;
   R.n = R.ncols = n;
   R.rptr.assign(n + 1, 0);
@@ -400,35 +402,24 @@ HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
         }
       R.rptr[i + 1] = (int32_t)R.idx.size();
     }
-  auto &v = R.val;
+  return R;
+}
+
+std::vector<int32_t> ilu0_diag_pos(const HostCSR &R) {
+  std::vector<int32_t> dpos(R.n, -1);
+  for (int32_t i = 0; i < R.n; ++i)
+    for (int32_t k = R.rptr[i]; k < R.rptr[i + 1]; ++k)
+      if (R.idx[k] == i) dpos[i] = k;
+  for (int32_t i = 0; i < R.n; ++i)
+    F3R_CHECK(dpos[i] >= 0, F3R_ERR_ARG, "ILU(0): a row has no diagonal entry");
+  return dpos;
+}
+
+HostFactors ilu0_split(const HostCSR &R) {
+  const int32_t n = R.n;
+  const auto &v = R.val;
   const auto &c = R.idx;
   const auto &p = R.rptr;
-  std::vector<int32_t> dpos(n, -1);
-  for (int32_t i = 0; i < n; ++i)
-    for (int32_t k = p[i]; k < p[i + 1]; ++k)
-      if (c[k] == i) dpos[i] = k;
-  for (int32_t i = 0; i < n; ++i)
-    F3R_CHECK(dpos[i] >= 0, F3R_ERR_ARG, "ILU(0): a row has no diagonal entry");
-  for (int32_t i = 0; i < n; ++i) {
-    v[dpos[i]] *= acc;
-    for (int32_t q = p[i]; q < p[i + 1] && c[q] < i; ++q) {
-      const int32_t k = c[q];
-      int32_t r = dpos[k];
-      v[q] = v[q] / v[r];
-      int32_t t = q + 1;
-      while (t < p[i + 1] && r < p[k + 1]) {
-        if (c[r] < c[t]) {
-          ++r;
-        } else if (c[r] > c[t]) {
-          ++t;
-        } else {
-          v[t] -= v[q] * v[r];
-          ++t;
-          ++r;
-        }
-      }
-    }
-  }
   HostFactors F;
   HostCSR &L = F.f[0], &U = F.f[1];
   L.n = U.n = L.ncols = U.ncols = n;
@@ -452,6 +443,64 @@ HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
     U.rptr[i + 1] = (int32_t)U.idx.size();
   }
   return F;
+}
+
+HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
+  HostCSR R = ilu0_restrict(A, blocks);
+  const int32_t n = R.n;
+  auto &v = R.val;
+  const auto &c = R.idx;
+  const auto &p = R.rptr;
+  const std::vector<int32_t> dpos = ilu0_diag_pos(R);
+  for (int32_t i = 0; i < n; ++i) {
+    v[dpos[i]] *= acc;
+    for (int32_t q = p[i]; q < p[i + 1] && c[q] < i; ++q) {
+      const int32_t k = c[q];
+      int32_t r = dpos[k];
+      v[q] = v[q] / v[r];
+      int32_t t = q + 1;
+      while (t < p[i + 1] && r < p[k + 1]) {
+        if (c[r] < c[t]) {
+          ++r;
+        } else if (c[r] > c[t]) {
+          ++t;
+        } else {
+          v[t] -= v[q] * v[r];
+          ++t;
+          ++r;
+        }
+      }
+    }
+  }
+  return ilu0_split(R);
+}
+
+int32_t factor_levels(const HostCSR &A, bool both, std::vector<int32_t> &lptr, std::vector<int32_t> &rows) {
+  const int32_t n = A.n;
+  std::vector<int32_t> lev(n), need(both ? n : 0, 0);
+  int32_t top = -1;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t l = both ? need[i] : 0;
+    for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k) {
+      const int32_t j = A.idx[k];
+      if (j < i) l = std::max(l, lev[j] + 1);
+    }
+    lev[i] = l;
+    if (both)
+      for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k) {
+        const int32_t j = A.idx[k];
+        if (j > i && j < n) need[j] = std::max(need[j], l + 1);
+      }
+    top = std::max(top, l);
+  }
+  const int32_t nl = top + 1;
+  lptr.assign(nl + 1, 0);
+  for (int32_t i = 0; i < n; ++i) lptr[lev[i] + 1]++;
+  for (int32_t l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
+  rows.resize(n);
+  std::vector<int32_t> fill(lptr.begin(), lptr.end() - 1);
+  for (int32_t i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+  return nl;
 }
 
 uint16_t d2h_bits(double d) {

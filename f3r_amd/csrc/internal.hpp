@@ -15,6 +15,7 @@ struct Prec {
   int dtype = F3R_F16;
   double tol = 0.1, acc = 1.0;
   int32_t n = 0;
+  int where = 0;  // factors built: 0 host, 1 device, 2 device asked, host ran
   HostFactors host;
   DevPtr tmp;
   // SDAINV
@@ -71,6 +72,11 @@ f3r_prec wrap_prec(std::shared_ptr<Prec> p);  // new handle owning a reference
 bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col,
                        const double *d_val, int32_t C, int32_t K, bool offsets, std::shared_ptr<SellShared> &sh,
                        DevPtr &sval);
+// device factorisations (factor.hip), bit-identical to sdainv_host /
+// ilu0_host; false (F untouched) when a row outgrows the device capacity
+bool sdainv_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col, const double *d_val,
+                   double tol, double acc, HostFactors &F);
+bool ilu0_device(Context *c, const HostCSR &A, int32_t blocks, double acc, HostFactors &F);
 // fp64 slot values on the device -> dtype (correctly rounded)
 DevPtr cast_values_device(Context *c, const DevPtr &v64, int64_t n, int dtype);
 }  // namespace f3r

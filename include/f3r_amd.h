@@ -173,6 +173,24 @@ int f3r_sdainv_create(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtyp
  * solves of trsv/bj.hpp:58-98, same per-row order). */
 int f3r_ilu0_create(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype,
                     f3r_prec *P);
+/* The same with set-up flags.  F3R_SETUP_DEVICE: factorise on the device
+ * (needs ctx): rows in level sets, one wavefront per row, factors bit-identical
+ * to the host set-up; when a row outgrows the device's row capacity (SDAINV:
+ * 1024 entries of z_i or w_i; ILU(0): 512 entries per row) the host set-up
+ * runs instead.  f3r_prec_setup_where tells which ran. */
+#define F3R_SETUP_DEVICE 1
+int f3r_sdainv_create_ex(f3r_ctx ctx, f3r_mat csr, double tol, double acc, int dtype, int flags,
+                         f3r_prec *P);
+int f3r_ilu0_create_ex(f3r_ctx ctx, f3r_mat csr, int32_t blocks, double acc, int dtype, int flags,
+                       f3r_prec *P);
+/* The level sets the device factorisations run in: level[i] for every row
+ * (row i waits for the rows j < i with A[i][j] != 0 and, when `both`, with
+ * A[j][i] != 0; SDAINV uses both, ILU(0) its lower triangle) and the number
+ * of levels (= kernel launches).  Host only. */
+int f3r_factor_levels(f3r_mat csr, int both, int32_t *level, int32_t *nlevels);
+/* *where = 0: factors built on the host; 1: on the device; 2: the device
+ * build was asked for and handed the rows back to the host. */
+int f3r_prec_setup_where(f3r_prec P, int *where);
 /* out = M in (IZW::operate ainv.hpp:110-114; ILU::operate ilu.hpp:57-61) */
 int f3r_prec_apply(f3r_prec P, f3r_vec in, f3r_vec out);
 /* factor sizes and a host copy of the factors (for tests):
