@@ -168,6 +168,7 @@ HostCSR csr_transpose(const HostCSR &A);
 HostCSR ilu0_restrict(const HostCSR &A, int32_t blocks);
 std::vector<int32_t> ilu0_diag_pos(const HostCSR &R);
 HostFactors ilu0_split(const HostCSR &R);
+HostFactors ilu0_split(const HostCSR &R, const double *val);  // R's pattern, these values
 // Level sets of a row-by-row factorisation: row i waits for every row j < i
 // with A[i][j] != 0 and, when `both`, with A[j][i] != 0.  rows[lptr[l] ..
 // lptr[l + 1]) lists the rows of level l in increasing order; returns the

@@ -947,7 +947,8 @@ static int prec_create(f3r_ctx ctx, f3r_mat csr, int kind, double p1, double acc
     const Mat &m = *csr->m;
     built = kind == 0 ? sdainv_device(p->ctx.get(), *m.host, m.d_rptr->as<int32_t>(), m.d_idx->as<int32_t>(),
                                       m.d_val->as<double>(), p1, acc, p->host)
-                      : ilu0_device(p->ctx.get(), *m.host, (int32_t)p1, acc, p->host);
+                      : ilu0_device(p->ctx.get(), *m.host, m.d_rptr->as<int32_t>(), m.d_idx->as<int32_t>(),
+                                    m.d_val->as<double>(), (int32_t)p1, acc, p->host);
     p->where = built ? 1 : 2;
   }
   if (!built) {

@@ -21,7 +21,7 @@
 // The transpose of A that SDAINV's z-side reads is built here too (column
 // counts by atomics, host prefix sum, atomic fill, per-column sort).
 #include <algorithm>
-#include <chrono>
+#include <climits>
 #include <cmath>
 #include <numeric>
 
@@ -55,23 +55,37 @@ __global__ void k_t_fill(int32_t n, const int32_t *__restrict__ rptr, const int3
   }
 }
 
-// the atomic fill leaves each column's rows in arrival order: sort them
-__global__ void k_t_sort(int32_t n, const int32_t *__restrict__ trptr, int32_t *__restrict__ tcol,
-                         double *__restrict__ tval) {
-  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// the atomic fill leaves each column's rows in arrival order: sort them, one
+// wave per column (rank of each row among the column's rows; rows are
+// distinct), lane 0 alone for a column of more than 64 entries
+__global__ __launch_bounds__(256) void k_t_sort(int32_t n, const int32_t *__restrict__ trptr, int32_t *__restrict__ tcol,
+                                                double *__restrict__ tval) {
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+  const int lane = threadIdx.x % kLanes;
   if (c >= n) return;
-  const int32_t b = trptr[c], e = trptr[c + 1];
-  for (int32_t p = b + 1; p < e; ++p) {
-    const int32_t key = tcol[p];
-    const double v = tval[p];
-    int32_t q = p - 1;
-    while (q >= b && tcol[q] > key) {
-      tcol[q + 1] = tcol[q];
-      tval[q + 1] = tval[q];
-      --q;
+  const int32_t b = trptr[c], e = trptr[c + 1], len = e - b;
+  if (len <= kLanes) {
+    const int32_t key = lane < len ? tcol[b + lane] : INT32_MAX;
+    const double v = lane < len ? tval[b + lane] : 0.0;
+    int rank = 0;
+    for (int q = 0; q < len; ++q) rank += __shfl(key, q, kLanes) < key;
+    if (lane < len) {
+      tcol[b + rank] = key;
+      tval[b + rank] = v;
     }
-    tcol[q + 1] = key;
-    tval[q + 1] = v;
+  } else if (lane == 0) {
+    for (int32_t p = b + 1; p < e; ++p) {
+      const int32_t key = tcol[p];
+      const double v = tval[p];
+      int32_t q = p - 1;
+      while (q >= b && tcol[q] > key) {
+        tcol[q + 1] = tcol[q];
+        tval[q + 1] = tval[q];
+        --q;
+      }
+      tcol[q + 1] = key;
+      tval[q + 1] = v;
+    }
   }
 }
 
@@ -97,6 +111,12 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 
 __device__ __forceinline__ int wave_scan_excl(int v, int &total) {
   const int lane = threadIdx.x;
+  if (__builtin_amdgcn_readfirstlane(__ballot(v > 1) == 0)) {
+    // every lane holds 0 or 1: a mask count
+    const uint64_t m = __ballot(v != 0);
+    total = __builtin_popcountll(m);
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+  }
   int x = v;
   for (int off = 1; off < kLanes; off <<= 1) {
     const int y = __shfl_up(x, off, kLanes);
@@ -129,6 +149,27 @@ __device__ double sparse_dot(const int32_t *xc, const double *xv, int lx, const 
       mask &= mask - 1;
       s += readlane_f64(p, l);
     }
+  }
+  return s;
+}
+
+// The same with the CSR row held in registers: lane b has entry b (rlen <= 64)
+__device__ double sparse_dot_reg(const int32_t *xc, const double *xv, int lx, int32_t cb, double vb, int rlen) {
+  double s = 0.0;
+  double p = 0.0;
+  bool m = false;
+  if ((int)threadIdx.x < rlen) {
+    const int pos = lower_bound_lds(xc, lx, cb);
+    if (pos < lx && xc[pos] == cb) {
+      m = true;
+      p = xv[pos] * vb;
+    }
+  }
+  uint64_t mask = __ballot(m);
+  while (mask) {
+    const int l = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    s += readlane_f64(p, l);
   }
   return s;
 }
@@ -213,24 +254,6 @@ __device__ int sparse_axpy(double alpha, const int32_t *xc, const double *xv, in
   return total;
 }
 
-// row j's slot of a pool -> LDS (the length read with the entries)
-template <int E>
-__device__ __forceinline__ int load_row(int32_t j, const int32_t *__restrict__ len, const int32_t *__restrict__ pc,
-                                        const double *__restrict__ pv, int32_t *c, double *v) {
-  constexpr int CAP = kLanes * E;
-  const int64_t base = (int64_t)j * CAP;
-  const int l = min(max(len[j], 0), CAP);
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = q * kLanes + (int)threadIdx.x;
-    if (e < l) {
-      c[e] = pc[base + e];
-      v[e] = pv[base + e];
-    }
-  }
-  return l;
-}
-
 template <int E>
 __device__ __forceinline__ void store_row(int32_t i, int l, const int32_t *c, const double *v,
                                           int32_t *__restrict__ len, int32_t *__restrict__ pc,
@@ -250,6 +273,7 @@ __device__ __forceinline__ void store_row(int32_t i, int l, const int32_t *c, co
 
 struct SdainvArgs {
   const int32_t *rows;  // rows of this level
+  int32_t cnt;          // rows in this level
   const int32_t *rptr, *col;
   const double *val;
   const int32_t *trptr, *tcol;  // A^T
@@ -260,87 +284,123 @@ struct SdainvArgs {
   int32_t *overflow;
 };
 
-// One row of the biconjugation per 64-lane block (ainv.hpp:280-325).
-template <int E> __global__ __launch_bounds__(64) void k_sdainv_row(SdainvArgs a) {
+// One side of one row of the biconjugation (ainv.hpp:280-325) per 64-lane
+// block: blocks [0, cnt) build w_i against the z_j of the lower part of row
+// i, then the pivot d_i; blocks [cnt, 2 cnt) build z_i against the w_j of
+// the upper part of column i.  The two sides of a row read only rows of
+// earlier levels, so they run side by side.  The slots of NB neighbours j
+// (the dot operand, the axpy source and d_j) are fetched together, whole
+// slots without waiting for their lengths, so a batch costs one round trip.
+template <int E, int NB> __global__ __launch_bounds__(64) void k_sdainv_side(SdainvArgs a) {
   constexpr int CAP = kLanes * E;
-  // 0: dot operand, 1: axpy source, 2 / 3: the row being built (ping-pong)
-  __shared__ int32_t sc[4][CAP];
-  __shared__ double sv[4][CAP];
+  __shared__ int32_t nc[NB][2][CAP];  // neighbour slots: 0 dot operand, 1 axpy source
+  __shared__ double nv[NB][2][CAP];
+  __shared__ int32_t yc[2][CAP];      // the row being built (ping-pong)
+  __shared__ double yv[2][CAP];
   __shared__ int32_t spx[CAP + 1], spy[CAP + 1];
   const int lane = threadIdx.x;
-  const int32_t i = a.rows[blockIdx.x];
-  const int32_t a0 = a.rptr[i], alen = a.rptr[i + 1] - a0;
-  const int32_t t0 = a.trptr[i], clen = a.trptr[i + 1] - t0;
-  const int32_t *arow = a.col + a0, *acol = a.tcol + t0;
-  const double *aval = a.val + a0, *acv = a.tval + t0;
+  const bool zside = (int32_t)blockIdx.x >= a.cnt;
+  const int32_t i = a.rows[zside ? blockIdx.x - a.cnt : blockIdx.x];
+  // w-side: row i of A, dot operands z_j, sources w_j; z-side: row i of A^T,
+  // dot operands w_j, sources z_j
+  const int32_t r0 = zside ? a.trptr[i] : a.rptr[i];
+  const int32_t rlen = (zside ? a.trptr[i + 1] : a.rptr[i + 1]) - r0;
+  const int32_t *rcol = (zside ? a.tcol : a.col) + r0;
+  const double *rval = (zside ? a.tval : a.val) + r0;
+  const int32_t *dlen = zside ? a.wlen : a.zlen, *dc = zside ? a.wc : a.zc;
+  const double *dv = zside ? a.wv : a.zv;
+  int32_t *slen = zside ? a.zlen : a.wlen, *sc = zside ? a.zc : a.wc;
+  double *sv = zside ? a.zv : a.wv;
+  // entries of the row before column i (sorted row)
+  const bool short_row = rlen <= kLanes;
+  int32_t cb = INT32_MAX;
+  double vb = 0.0;
+  if ((int)threadIdx.x < rlen) {
+    cb = rcol[threadIdx.x];
+    vb = rval[threadIdx.x];
+  }
+  int nl;
+  if (short_row) {
+    nl = __builtin_popcountll(__ballot(cb < i));
+  } else {
+    nl = 0;
+    while (nl < rlen && rcol[nl] < i) ++nl;
+  }
 
-  // w_i: against the z_j of the lower part of row i
-  int cur = 2, ly = 1;
+  int cur = 0, ly = 1;
   if (lane == 0) {
-    sc[2][0] = i;
-    sv[2][0] = 1.0;
+    yc[0][0] = i;
+    yv[0][0] = 1.0;
   }
-  for (int k = 0; k < alen; ++k) {
-    const int32_t j = arow[k];
-    if (j >= i) break;
+  for (int base = 0; base < nl; base += NB) {
+    const int nb = min(NB, nl - base);
+    int32_t lc[NB][2][E];
+    double lv[NB][2][E];
+    int ll[NB][2];
+    double ld[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < nb) {
+        const int32_t j = rcol[base + q];
+        const int64_t off = (int64_t)j * CAP;
+        ll[q][0] = dlen[j];
+        ll[q][1] = slen[j];
+        ld[q] = a.d[j];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          lc[q][0][e] = dc[off + e * kLanes + lane];
+          lv[q][0][e] = dv[off + e * kLanes + lane];
+          lc[q][1][e] = sc[off + e * kLanes + lane];
+          lv[q][1][e] = sv[off + e * kLanes + lane];
+        }
+      }
     __syncthreads();
-    const int lz = load_row<E>(j, a.zlen, a.zc, a.zv, sc[0], sv[0]);
-    const int lw = load_row<E>(j, a.wlen, a.wc, a.wv, sc[1], sv[1]);
-    const double dj = a.d[j];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < nb) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          nc[q][0][e * kLanes + lane] = lc[q][0][e];
+          nv[q][0][e * kLanes + lane] = lv[q][0][e];
+          nc[q][1][e * kLanes + lane] = lc[q][1][e];
+          nv[q][1][e * kLanes + lane] = lv[q][1][e];
+        }
+      }
     __syncthreads();
-    const double alpha = -sparse_dot(sc[0], sv[0], lz, arow, aval, alen) / dj;
-    const int lo = sparse_axpy<E>(alpha, sc[1], sv[1], lw, sc[cur], sv[cur], ly, a.tol, sc[5 - cur], sv[5 - cur],
-                                  spx, spy);
-    if (lo < 0) {
-      if (lane == 0) a.overflow[0] = 1;
-      return;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < nb) {
+        const int l0 = min(max(ll[q][0], 0), CAP), l1 = min(max(ll[q][1], 0), CAP);
+        const double dot = short_row ? sparse_dot_reg(nc[q][0], nv[q][0], l0, cb, vb, rlen)
+                                     : sparse_dot(nc[q][0], nv[q][0], l0, rcol, rval, rlen);
+        const double alpha = -dot / ld[q];
+        const int lo = sparse_axpy<E>(alpha, nc[q][1], nv[q][1], l1, yc[cur], yv[cur], ly, a.tol, yc[1 - cur],
+                                      yv[1 - cur], spx, spy);
+        if (lo < 0) {
+          if (lane == 0) a.overflow[0] = 1;
+          return;
+        }
+        ly = lo;
+        cur = 1 - cur;
+      }
+  }
+  __syncthreads();
+  if (!zside) {
+    // pivot with the weight acc on w_i's last entry (ainv.hpp:310-316)
+    const int32_t t0 = a.trptr[i], clen = a.trptr[i + 1] - t0;
+    double di = 0.0;
+    if (ly > 0) {
+      if (lane == 0) yv[cur][ly - 1] = yv[cur][ly - 1] * a.acc;
+      __syncthreads();
+      di = sparse_dot(yc[cur], yv[cur], ly, a.tcol + t0, a.tval + t0, clen);
+      __syncthreads();
+      if (lane == 0) yv[cur][ly - 1] = yv[cur][ly - 1] / a.acc;
+      __syncthreads();
     }
-    ly = lo;
-    cur = 5 - cur;
+    if (di == 0.0) di = 1.0;
+    if (lane == 0) a.d[i] = di;
   }
-  __syncthreads();
-  // pivot with the weight acc on w_i's last entry (ainv.hpp:310-316)
-  double di = 0.0;
-  if (ly > 0) {
-    if (lane == 0) sv[cur][ly - 1] = sv[cur][ly - 1] * a.acc;
-    __syncthreads();
-    di = sparse_dot(sc[cur], sv[cur], ly, acol, acv, clen);
-    __syncthreads();
-    if (lane == 0) sv[cur][ly - 1] = sv[cur][ly - 1] / a.acc;
-    __syncthreads();
-  }
-  if (di == 0.0) di = 1.0;
-  store_row<E>(i, ly, sc[cur], sv[cur], a.wlen, a.wc, a.wv);
-  if (lane == 0) a.d[i] = di;
-
-  // z_i: against the w_j of the upper part of column i
-  __syncthreads();
-  cur = 2;
-  ly = 1;
-  if (lane == 0) {
-    sc[2][0] = i;
-    sv[2][0] = 1.0;
-  }
-  for (int k = 0; k < clen; ++k) {
-    const int32_t j = acol[k];
-    if (j >= i) break;
-    __syncthreads();
-    const int lw = load_row<E>(j, a.wlen, a.wc, a.wv, sc[0], sv[0]);
-    const int lz = load_row<E>(j, a.zlen, a.zc, a.zv, sc[1], sv[1]);
-    const double dj = a.d[j];
-    __syncthreads();
-    const double alpha = -sparse_dot(sc[0], sv[0], lw, acol, acv, clen) / dj;
-    const int lo = sparse_axpy<E>(alpha, sc[1], sv[1], lz, sc[cur], sv[cur], ly, a.tol, sc[5 - cur], sv[5 - cur],
-                                  spx, spy);
-    if (lo < 0) {
-      if (lane == 0) a.overflow[0] = 1;
-      return;
-    }
-    ly = lo;
-    cur = 5 - cur;
-  }
-  __syncthreads();
-  store_row<E>(i, ly, sc[cur], sv[cur], a.zlen, a.zc, a.zv);
+  store_row<E>(i, ly, yc[cur], yv[cur], slen, sc, sv);
 }
 
 __global__ void k_sdainv_init(int32_t n, int32_t *wlen, int32_t *zlen, double *d) {
@@ -440,10 +500,11 @@ bool sdainv_pass(Context *c, const HostCSR &A, const std::vector<int32_t> &lptr,
   a.zv = zv->as<double>();
   a.d = d->as<double>();
   a.overflow = flag->as<int32_t>();
+  constexpr int NB = E == 1 ? 8 : E == 4 ? 2 : 1;
   for (size_t l = 0; l + 1 < lptr.size(); ++l) {
-    const int32_t cnt = lptr[l + 1] - lptr[l];
+    a.cnt = lptr[l + 1] - lptr[l];
     a.rows = d_rows->as<int32_t>() + lptr[l];
-    hipLaunchKernelGGL(k_sdainv_row<E>, dim3(cnt), dim3(kLanes), 0, st, a);
+    hipLaunchKernelGGL((k_sdainv_side<E, NB>), dim3(2 * a.cnt), dim3(kLanes), 0, st, a);
   }
   F3R_HIP(hipGetLastError());
   int32_t over = 0;
@@ -520,8 +581,8 @@ bool sdainv_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const in
   DevPtr tval = dev_alloc(sizeof(double) * std::max<int64_t>(1, nnz));
   hipLaunchKernelGGL(k_t_fill, dim3((n + 255) / 256), dim3(256), 0, st, n, d_rptr, d_col, d_val,
                      fill->as<int32_t>(), tcol->as<int32_t>(), tval->as<double>());
-  hipLaunchKernelGGL(k_t_sort, dim3((n + 255) / 256), dim3(256), 0, st, n, (const int32_t *)d_trptr->as<int32_t>(),
-                     tcol->as<int32_t>(), tval->as<double>());
+  hipLaunchKernelGGL(k_t_sort, dim3((unsigned)(((int64_t)n * kLanes + 255) / 256)), dim3(256), 0, st, n,
+                     (const int32_t *)d_trptr->as<int32_t>(), tcol->as<int32_t>(), tval->as<double>());
   F3R_HIP(hipGetLastError());
   SdainvArgs a{};
   a.rptr = d_rptr;
@@ -542,9 +603,13 @@ bool sdainv_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const in
   return false;
 }
 
-bool ilu0_device(Context *c, const HostCSR &A, int32_t blocks, double acc, HostFactors &F) {
+bool ilu0_device(Context *c, const HostCSR &A, const int32_t *d_arptr, const int32_t *d_acol, const double *d_aval,
+                 int32_t blocks, double acc, HostFactors &F) {
   hipStream_t st = c->stream;
-  HostCSR R = ilu0_restrict(A, blocks);
+  // one block: the whole matrix, already on the device
+  HostCSR Rb;
+  if (blocks != 1) Rb = ilu0_restrict(A, blocks);
+  const HostCSR &R = blocks == 1 ? A : Rb;
   const int32_t n = R.n;
   if (n == 0) return false;
   for (int32_t i = 0; i < n; ++i)
@@ -552,22 +617,29 @@ bool ilu0_device(Context *c, const HostCSR &A, int32_t blocks, double acc, HostF
   const std::vector<int32_t> dpos = ilu0_diag_pos(R);
   std::vector<int32_t> lptr, rows;
   factor_levels(R, false, lptr, rows);
+  const size_t nnz = R.idx.size();
   DevPtr d_rows = upload_i32(c, rows.data(), rows.size());
-  DevPtr d_rptr = upload_i32(c, R.rptr.data(), n + 1);
-  DevPtr d_col = upload_i32(c, R.idx.data(), R.idx.size());
-  DevPtr d_val = upload_f64(c, R.val.data(), R.val.size());
+  DevPtr d_rptr, d_col, d_val = dev_alloc(sizeof(double) * std::max<size_t>(1, nnz));
+  if (blocks == 1) {
+    if (nnz) F3R_HIP(hipMemcpyAsync(d_val->p, d_aval, sizeof(double) * nnz, hipMemcpyDeviceToDevice, st));
+  } else {
+    d_rptr = upload_i32(c, R.rptr.data(), n + 1);
+    d_col = upload_i32(c, R.idx.data(), nnz);
+    if (nnz) F3R_HIP(hipMemcpyAsync(d_val->p, R.val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+  }
+  const int32_t *rp = blocks == 1 ? d_arptr : d_rptr->as<int32_t>();
+  const int32_t *cl = blocks == 1 ? d_acol : d_col->as<int32_t>();
   DevPtr d_dpos = upload_i32(c, dpos.data(), n);
   for (size_t l = 0; l + 1 < lptr.size(); ++l) {
     const int32_t cnt = lptr[l + 1] - lptr[l];
     hipLaunchKernelGGL(k_ilu0_row, dim3(cnt), dim3(kLanes), 0, st, (const int32_t *)d_rows->as<int32_t>() + lptr[l],
-                       (const int32_t *)d_rptr->as<int32_t>(), (const int32_t *)d_col->as<int32_t>(),
-                       d_val->as<double>(), (const int32_t *)d_dpos->as<int32_t>(), acc);
+                       rp, cl, d_val->as<double>(), (const int32_t *)d_dpos->as<int32_t>(), acc);
   }
   F3R_HIP(hipGetLastError());
-  if (!R.val.empty())
-    F3R_HIP(hipMemcpyAsync(R.val.data(), d_val->p, sizeof(double) * R.val.size(), hipMemcpyDeviceToHost, st));
+  std::vector<double> v(nnz);
+  if (nnz) F3R_HIP(hipMemcpyAsync(v.data(), d_val->p, sizeof(double) * nnz, hipMemcpyDeviceToHost, st));
   F3R_HIP(hipStreamSynchronize(st));
-  F = ilu0_split(R);
+  F = ilu0_split(R, v.data());
   return true;
 }
 

@@ -415,9 +415,10 @@ std::vector<int32_t> ilu0_diag_pos(const HostCSR &R) {
   return dpos;
 }
 
-HostFactors ilu0_split(const HostCSR &R) {
+HostFactors ilu0_split(const HostCSR &R) { return ilu0_split(R, R.val.data()); }
+
+HostFactors ilu0_split(const HostCSR &R, const double *v) {
   const int32_t n = R.n;
-  const auto &v = R.val;
   const auto &c = R.idx;
   const auto &p = R.rptr;
   HostFactors F;
@@ -425,6 +426,11 @@ HostFactors ilu0_split(const HostCSR &R) {
   L.n = U.n = L.ncols = U.ncols = n;
   L.rptr.assign(n + 1, 0);
   U.rptr.assign(n + 1, 0);
+  const size_t half = c.size() / 2 + (size_t)n;
+  L.idx.reserve(half);
+  L.val.reserve(half);
+  U.idx.reserve(half);
+  U.val.reserve(half);
   for (int32_t i = 0; i < n; ++i) {
     for (int32_t k = p[i]; k < p[i + 1]; ++k) {
       if (c[k] < i) {
@@ -476,22 +482,27 @@ HostFactors ilu0_host(const HostCSR &A, int32_t blocks, double acc) {
 }
 
 int32_t factor_levels(const HostCSR &A, bool both, std::vector<int32_t> &lptr, std::vector<int32_t> &rows) {
+  // rows are sorted: the entries before column i, then (diagonal and) after
   const int32_t n = A.n;
   std::vector<int32_t> lev(n), need(both ? n : 0, 0);
+  const int32_t *rp = A.rptr.data(), *ix = A.idx.data();
   int32_t top = -1;
   for (int32_t i = 0; i < n; ++i) {
     int32_t l = both ? need[i] : 0;
-    for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k) {
-      const int32_t j = A.idx[k];
-      if (j < i) l = std::max(l, lev[j] + 1);
+    const int32_t *p = ix + rp[i], *e = ix + rp[i + 1];
+    for (; p < e && *p < i; ++p) {
+      const int32_t c = lev[*p] + 1;
+      l = c > l ? c : l;
     }
     lev[i] = l;
-    if (both)
-      for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k) {
-        const int32_t j = A.idx[k];
-        if (j > i && j < n) need[j] = std::max(need[j], l + 1);
+    top = l > top ? l : top;
+    if (both) {
+      const int32_t c = l + 1;
+      for (; p < e; ++p) {
+        int32_t &d = need[*p];
+        d = (*p != i && c > d) ? c : d;
       }
-    top = std::max(top, l);
+    }
   }
   const int32_t nl = top + 1;
   lptr.assign(nl + 1, 0);

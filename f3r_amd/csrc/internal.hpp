@@ -76,7 +76,8 @@ bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, cons
 // ilu0_host; false (F untouched) when a row outgrows the device capacity
 bool sdainv_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col, const double *d_val,
                    double tol, double acc, HostFactors &F);
-bool ilu0_device(Context *c, const HostCSR &A, int32_t blocks, double acc, HostFactors &F);
+bool ilu0_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col, const double *d_val,
+                 int32_t blocks, double acc, HostFactors &F);
 // fp64 slot values on the device -> dtype (correctly rounded)
 DevPtr cast_values_device(Context *c, const DevPtr &v64, int64_t n, int dtype);
 }  // namespace f3r
