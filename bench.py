@@ -96,6 +96,44 @@ def learned_vs_plain(F, ctx, count, nx=96, reps=3):
             "systems": rows}
 
 
+def setup_timings(F, ctx, A, m):
+    """wall time of each set-up step on the device and on the host (same
+    factors: checked), for the bench matrix"""
+    import numpy as np
+    out = {"workload": f"set-up of the 27-point nonsymmetric {m}^3 matrix (n={A.n}, nnz={A.nnz})"}
+    t = time.time()
+    A.handle(ctx)
+    ctx.sync()
+    out["csr_upload_s"] = round(time.time() - t, 3)
+    t = time.time()
+    _, nl = A.factor_levels(True)
+    out["levels"] = nl
+    out["level_sets_s"] = round(time.time() - t, 3)
+    for name, make in (("sdainv", lambda dev: F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx, device=dev)),
+                       ("ilu0", lambda dev: F.ILU0(A, 1, 1.0, "f16", ctx=ctx, device=dev))):
+        facs = {}
+        for dev in (True, False):
+            t = time.time()
+            P = make(dev)
+            ctx.sync()
+            out[f"{name}_{'device' if dev else 'host'}_s"] = round(time.time() - t, 3)
+            out[f"{name}_{'device' if dev else 'host'}_where"] = P.where
+            facs[dev] = P.factors()
+            del P
+        out[f"{name}_factors_identical"] = all(np.array_equal(a, b) for fa, fb in zip(facs[True], facs[False])
+                                               for a, b in zip(fa, fb))
+        log(f"[bench] {name}: device {out[f'{name}_device_s']} s, host {out[f'{name}_host_s']} s, "
+            f"identical {out[f'{name}_factors_identical']}")
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx, device=True)
+    for dev in (True, False):
+        t = time.time()
+        S = F.F3R(ctx, A, P, device_build=dev)
+        ctx.sync()
+        out[f"solver_layouts_{'device' if dev else 'host'}_s"] = round(time.time() - t, 3)
+        del S
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +148,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--learned", type=int, default=4, metavar="N",
                     help="held-out 96^3 lognormal systems for learned vs plain F3R (0: skip)")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="no solves: time the set-up steps (SDAINV and ILU(0) factors, solver layouts) on the "
+                         "device and on the host, check the factors agree, print one JSON line and exit")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="REPS",
                     help="no solves: launch the dominant fp16 SpMV REPS times and a plain copy of a known "
                          "size REPS times (for hardware-counter runs), print their timings and exit")
@@ -129,6 +170,9 @@ def main():
     b = F.uniform(A.n, -1.0, 1.0, seed=2024)
     log(f"[bench] matrix {m}^3: n={A.n} nnz={A.nnz} ({time.time() - t0:.1f} s)")
     ctx = F.Context(0)
+    if args.setup_only:
+        setup_timings(F, ctx, A, m)
+        return
     t0 = time.time()
     P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx, device=not args.host_build)
     factor_s = time.time() - t0
