@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <stdexcept>
 #include <limits>
 #include <memory>
 #include <utility>
@@ -612,18 +614,23 @@ template <class T1, class T2, class V> struct LevelOp : OpT<T1> {
 
 // ---------------------------------------------------- adaptive Richardson --
 // Shared state of the adaptive weights (work/src/f3r-gpu.cpp:106-107,224-227)
+// With a static weight w > 0 there are no probes and every step uses w
+// (work/src/static.cpp:99-119).
 struct RichState {
   int c = 64;
   int cnt = 64;
   int den = 0;
+  bool fixed = false;
   std::vector<double> sum, ome;
-  void reset(int m4, int c_) {
+  void reset(int m4, int c_, double w = 0.0) {
     c = c_;
     cnt = c_;
     den = 0;
+    fixed = w > 0;
     sum.assign(m4, 1.0);
-    ome.assign(m4, 1.0);
+    ome.assign(m4, fixed ? w : 1.0);
   }
+  bool probe() const { return !fixed && cnt % c == 0; }
 };
 
 // Preconditioner of the innermost level in storage type S.
@@ -668,7 +675,7 @@ template <class T> struct RichT : OpT<T> {
     int64_t n = A->n;
     ++calls;
     ++st->cnt;
-    bool probe = st->cnt % st->c == 0;
+    bool probe = st->probe();
     P->template apply<T, false>(in, out);
     if (probe) {
       spmv_t<double>(*A, out, tmp.data());
@@ -733,7 +740,7 @@ template <bool ref16> struct RichH : OpT<float> {
     int64_t n = A->n;
     ++calls;
     ++st->cnt;
-    bool probe = st->cnt % st->c == 0;
+    bool probe = st->probe();
     const float sc = ref16 ? 1.f : scale;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) hin[i] = f2h(in[i] * sc);
@@ -858,7 +865,12 @@ template <class T2, class V3, class S, bool ref16> struct F3RImpl : F3R {
     L2op.reset(new LevelOp<double, T2, T2>(L2.get()));
     L1.reset(new FGMRES<double, double>(&A64, L2op.get(), o.m1, true));
   }
-  void reset() override { st.reset(o.m4, o.c); }
+  void reset() override {
+    // the reference keeps the static weight in fp16 (static.cpp:84-85)
+    double w = o.static_weight;
+    if (w > 0 && ref16) w = h2f(f2h((float)w));
+    st.reset(o.m4, o.c, w);
+  }
   int64_t calls() const override {
     if constexpr (std::is_same_v<S, h16>)
       return static_cast<RichH<ref16> *>(R4.get())->calls;
@@ -895,7 +907,189 @@ template <class T2, class V3, class S, bool ref16> struct F3RImpl : F3R {
   }
 };
 
+// ------------------------------------------------------ depth ablations --
+// The nests of work/src/depth.cpp:122-167: FGMRES levels (gmres2.hpp:56-150,
+// normalize = false inside) down to the preconditioner applied directly.
+// This build's arithmetic: an fp16 level stores V and Z in fp16 and computes
+// in fp32 (sums, Hessenberg, Givens), with its vectors kept scaled by the
+// power-of-two range factor s (stored = s * value; dots and norms take
+// 1/s^2, the output 1/s).  The reference computes such a level in host
+// _Float16 arithmetic.
+
+// FGMRES level with fp16 storage (F2H, F3H, the innermost level of F4)
+template <class V> struct FGMRESH {
+  const CSRv<V> *A;
+  OpT<h16> *M;
+  int m;
+  int64_t n;
+  float s, inv_s, inv_s2;
+  std::vector<h16> Vb, Zb;
+  std::vector<float> H, c, sn, y, e;
+  FGMRESH(const CSRv<V> *A_, OpT<h16> *M_, int m_, float s_)
+      : A(A_), M(M_), m(m_), n(A_->n), s(s_), inv_s(1.f / s_), inv_s2(1.f / (s_ * s_)),
+        Vb((size_t)n * (m_ + 1)), Zb((size_t)n * m_), H((size_t)(m_ + 1) * m_), c(m_), sn(m_), y(m_),
+        e(m_ + 1) {}
+  h16 *Vc(int j) { return Vb.data() + (size_t)n * j; }
+  h16 *Zc(int j) { return Zb.data() + (size_t)n * j; }
+  template <class TI, class TO> void operate(const TI *in, TO *out) {
+    h16 *v0 = Vc(0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) v0[i] = cvt<h16>(in[i] * (TI)s);
+    e[0] = 1;
+    for (int j = 0; j < m; j++) {
+      M->apply(Vc(j), Zc(j));
+      spmv_t<float>(*A, Zc(j), Vc(j + 1));
+      h16 *w = Vc(j + 1);
+      float *Hj = H.data() + (size_t)j * (m + 1);
+      for (int k = 0; k <= j; k++) {
+        const h16 *vk = Vc(k);
+        Hj[k] = reduce_sum<float>(n, [&](int64_t i) { return h2f(w[i]) * h2f(vk[i]); }) * inv_s2;
+      }
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) {
+        float r = h2f(Vb[i]) * Hj[0];
+        for (int k = 1; k <= j; k++) r = madd(h2f(Vb[(size_t)n * k + i]), Hj[k], r);
+        w[i] = f2h(h2f(w[i]) - r);
+      }
+      const float nv =
+          std::sqrt(reduce_sum<float>(n, [&](int64_t i) { float t = h2f(w[i]); return t * t; }) * inv_s2);
+      Hj[j + 1] = nv;
+      const float inv = (float)(1. / nv);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) w[i] = f2h(h2f(w[i]) * inv);
+      full_grot(c.data(), sn.data(), Hj, j);
+      final_grot(c.data(), sn.data(), e.data(), j);
+    }
+    hess_trsv(H.data(), e.data(), y.data(), m + 1, m);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      float r = h2f(Zb[i]) * y[0];
+      for (int k = 1; k < m; k++) r = madd(h2f(Zb[(size_t)n * k + i]), y[k], r);
+      out[i] = cvt<TO>(r * inv_s);
+    }
+  }
+};
+template <class T1, class V> struct LevelOpH : OpT<T1> {
+  FGMRESH<V> *F;
+  explicit LevelOpH(FGMRESH<V> *f) : F(f) {}
+  void apply(const T1 *in, T1 *out) override { F->operate(in, out); }
+};
+
+// M applied directly, from an fp32 level: fp16 copy scaled by s, fp16
+// factors with fp32 sums, back to fp32
+struct MDirectF : OpT<float> {
+  Prec<h16> *P;
+  int64_t n;
+  float s;
+  std::vector<h16> hin, hout;
+  int64_t calls = 0;
+  MDirectF(Prec<h16> *P_, int64_t n_, float s_) : P(P_), n(n_), s(s_), hin(n_), hout(n_) {}
+  void apply(const float *in, float *out) override {
+    ++calls;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) hin[i] = f2h(in[i] * s);
+    P->template apply<float, false>(hin.data(), hout.data());
+    const float inv = 1.f / s;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = h2f(hout[i]) * inv;
+  }
+};
+// from an fp16 level: its stored (scaled) vectors go straight in
+struct MDirectH : OpT<h16> {
+  Prec<h16> *P;
+  int64_t calls = 0;
+  explicit MDirectH(Prec<h16> *P_) : P(P_) {}
+  void apply(const h16 *in, h16 *out) override {
+    ++calls;
+    P->template apply<float, false>(in, out);
+  }
+};
+
+struct DepthImpl : F3R {
+  CSRv<double> A64;
+  CSRv<float> A32;
+  CSRv<h16> A16;
+  Prec<h16> P;
+  orc_f3r_opts o;
+  std::unique_ptr<MDirectF> mf;
+  std::unique_ptr<MDirectH> mh;
+  std::vector<std::shared_ptr<void>> own;  // the levels and their adapters
+  OpT<double> *top = nullptr;
+  std::unique_ptr<FGMRES<double, double>> L1;
+  std::vector<double> r;
+
+  template <class X> X *keep(X *p) {
+    own.emplace_back(std::shared_ptr<X>(p));
+    return p;
+  }
+  DepthImpl(const CSRd &A, const orc_f3r_opts &opts) : A64(A), A32(A), A16(A), o(opts), r(A.n) {
+    setup_prec(P, A, o);
+    const float s = (float)o.range_scale;
+    const int64_t n = A.n;
+    switch (o.depth) {
+      case 1: {  // F2: FGMRES<float>(A32, M, 64)
+        mf.reset(new MDirectF(&P, n, s));
+        auto *l2 = keep(new FGMRES<float, float>(&A32, mf.get(), 64, false));
+        top = keep(new LevelOp<double, float, float>(l2));
+        break;
+      }
+      case 2: {  // F2H: FGMRES<half>(A16, M, 64)
+        mh.reset(new MDirectH(&P));
+        auto *l2 = keep(new FGMRESH<h16>(&A16, mh.get(), 64, s));
+        top = keep(new LevelOpH<double, h16>(l2));
+        break;
+      }
+      case 3: {  // F3: FGMRES<float>(A32, FGMRES<float>(A16, M, 8), 8)
+        mf.reset(new MDirectF(&P, n, s));
+        auto *l3 = keep(new FGMRES<float, h16>(&A16, mf.get(), 8, false));
+        auto *l3op = keep(new LevelOp<float, float, h16>(l3));
+        auto *l2 = keep(new FGMRES<float, float>(&A32, l3op, 8, false));
+        top = keep(new LevelOp<double, float, float>(l2));
+        break;
+      }
+      case 4: {  // F3H: FGMRES<float>(A32, FGMRES<half>(A16, M, 8), 8)
+        mh.reset(new MDirectH(&P));
+        auto *l3 = keep(new FGMRESH<h16>(&A16, mh.get(), 8, s));
+        auto *l3op = keep(new LevelOpH<float, h16>(l3));
+        auto *l2 = keep(new FGMRES<float, float>(&A32, l3op, 8, false));
+        top = keep(new LevelOp<double, float, float>(l2));
+        break;
+      }
+      case 5: {  // F4: FGMRES<float>(A32, FGMRES<float>(A16, FGMRES<half>(A16, M, 2), 4), 8)
+        mh.reset(new MDirectH(&P));
+        auto *l4 = keep(new FGMRESH<h16>(&A16, mh.get(), 2, s));
+        auto *l4op = keep(new LevelOpH<float, h16>(l4));
+        auto *l3 = keep(new FGMRES<float, h16>(&A16, l4op, 4, false));
+        auto *l3op = keep(new LevelOp<float, float, h16>(l3));
+        auto *l2 = keep(new FGMRES<float, float>(&A32, l3op, 8, false));
+        top = keep(new LevelOp<double, float, float>(l2));
+        break;
+      }
+      default:
+        throw std::runtime_error("depth must be 1..5");
+    }
+    L1.reset(new FGMRES<double, double>(&A64, top, o.m1, true));
+  }
+  void reset() override {}
+  int64_t calls() const override { return mf ? mf->calls : mh->calls; }
+  SolveRes solve(const double *b, double *x, const Cond &cond) override {
+    auto one = [&]() {
+      resid_t<double>(A64, b, x, r.data());
+      return L1->template kernel<false>(r.data(), x, cond);
+    };
+    SolveRes flag = one();
+    int cnt = 1;
+    while (!flag.is_solved && cnt < o.restart) {
+      flag += one();
+      cnt++;
+    }
+    return flag;
+  }
+  void level_apply(int, const double *in, double *out) override { top->apply(in, out); }
+};
+
 std::unique_ptr<F3R> make_f3r(const CSRd &A, const orc_f3r_opts &o) {
+  if (o.depth != 0) return std::unique_ptr<F3R>(new DepthImpl(A, o));
   if (o.variant == 2) return std::unique_ptr<F3R>(new F3RImpl<double, double, double, false>(A, o));
   if (o.variant == 1) return std::unique_ptr<F3R>(new F3RImpl<float, float, float, false>(A, o));
   if (o.ref_arith == 1) return std::unique_ptr<F3R>(new F3RImpl<float, h16, h16, true>(A, o));
@@ -1089,6 +1283,7 @@ void orc_bj_solve16(void *ilu, const uint16_t *in, uint16_t *out, uint16_t *tmp,
 int32_t orc_f3r_solve(int32_t n, const int32_t *rptr, const int32_t *idx, const double *val,
                       const double *b, double *x, const orc_f3r_opts *opts,
                       orc_f3r_result *res, double *hist) {
+  if (opts->depth < 0 || opts->depth > 5 || (opts->depth != 0 && opts->ref_arith != 0)) return -1;
   ThreadScope ts(opts->nthreads);
   const bool saved_fma = g_fma;
   g_fma = opts->ref_arith == 1 ? false : true;
@@ -1130,6 +1325,7 @@ int32_t orc_f3r_solve(int32_t n, const int32_t *rptr, const int32_t *idx, const 
 
 int32_t orc_level_apply(int32_t n, const int32_t *rptr, const int32_t *idx, const double *val,
                         const orc_f3r_opts *opts, int32_t level, const double *in, double *out) {
+  if (opts->depth < 0 || opts->depth > 5 || (opts->depth != 0 && opts->ref_arith != 0)) return -1;
   ThreadScope ts(opts->nthreads);
   const bool saved_fma = g_fma;
   g_fma = opts->ref_arith == 1 ? false : true;

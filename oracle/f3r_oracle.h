@@ -18,7 +18,9 @@
  *   FGMRES + Givens        sparse/solver/gmres.hpp:12-135
  *   Restarted              sparse/solver/restart.hpp:8-37
  *   adaptive Richardson    work/src/f3r-gpu.cpp:229-260, work/src/f3r.cpp:212-243
+ *   static Richardson      work/src/static.cpp:99-119
  *   nesting / driver       work/src/f3r-gpu.cpp:77-104,210-266
+ *   depth ablations        work/src/depth.cpp:122-167, solver/gmres2.hpp:56-150
  *
  * Precision codes used throughout: 0 = fp64, 1 = fp32, 2 = fp16.
  */
@@ -98,6 +100,12 @@ typedef struct {
   int32_t bj_blocks;       /* ILU(0) diagonal blocks */
   int32_t max_outer;       /* >0: stop after this many outer steps (timing samples) */
   int32_t nthreads;        /* OpenMP threads, 0 = default */
+  double static_weight;    /* >0: Richardson with this fixed weight for every
+                              step and no probes (work/src/static.cpp:99-119) */
+  int32_t depth;           /* 0: the F3R nest above; 1..5: the nesting-depth
+                              ablations F2, F2H, F3, F3H, F4 of
+                              work/src/depth.cpp:122-167 (this build's
+                              arithmetic only) */
 } orc_f3r_opts;
 
 typedef struct {
@@ -113,6 +121,7 @@ typedef struct {
 } orc_f3r_result;
 
 /* Solves A x = b with the matrix as given (the caller applies scaling).
+ * Returns 0, or -1 for a depth outside 0..5 or a depth with ref_arith = 1.
  * hist (optional, length >= m1*restart+1) receives |e_j|/||b|| per outer step.*/
 int32_t orc_f3r_solve(int32_t n, const int32_t *rptr, const int32_t *idx,
                       const double *val, const double *b, double *x,

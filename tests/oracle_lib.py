@@ -23,7 +23,8 @@ class Opts(ctypes.Structure):
                 ("ref_arith", ctypes.c_int32), ("range_scale", ctypes.c_double),
                 ("precond", ctypes.c_int32), ("tol", ctypes.c_double), ("acc", ctypes.c_double),
                 ("bj_blocks", ctypes.c_int32), ("max_outer", ctypes.c_int32),
-                ("nthreads", ctypes.c_int32)]
+                ("nthreads", ctypes.c_int32), ("static_weight", ctypes.c_double),
+                ("depth", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -130,7 +131,7 @@ def ilu0(n, rptr, idx, val, blocks=1, acc=1.0):
 def f3r_opts(**kw):
     d = dict(m1=100, m2=8, m3=4, m4=2, c=64, restart=20, eps=1e-8, variant=0, ref_arith=0,
              range_scale=1.0, precond=0, tol=0.1, acc=1.0, bj_blocks=112, max_outer=0,
-             nthreads=0)
+             nthreads=0, static_weight=0.0, depth=0)
     d.update(kw)
     return Opts(**d)
 
