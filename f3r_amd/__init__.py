@@ -411,15 +411,22 @@ class ILU0(_Precond):
 solve_res_t = namedtuple("solve_res_t", "is_solved res_iter res_nrm2 nrm_b solve_s precond_calls outer_cycles")
 
 VARIANTS = {"half": 0, "float": 1, "double": 2}
+# nesting-depth ablations of work/src/depth.cpp:122-167 (0: the F3R nest)
+DEPTHS = {"F3R": 0, "F2": 1, "F2H": 2, "F3": 3, "F3H": 4, "F4": 5}
 
 
 class F3R:
     """Restarted<double>(A, FGMRES<double>(A, FGMRES<float>(A32, FGMRES<float>(A16,
-    Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU."""
+    Richardson(M)))) of work/src/f3r-gpu.cpp:210-266 on one GPU.
+
+    static_weight > 0: the Richardson level with that fixed weight and no probes
+    (work/src/static.cpp:99-119).  depth = "F2", "F2H", "F3", "F3H" or "F4": the
+    nesting-depth ablations of work/src/depth.cpp:122-167 instead of the F3R
+    nest (M applied directly under fp32 / fp16-basis FGMRES levels)."""
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
                  range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1,
-                 cap_spmv=False, device_build=False):
+                 cap_spmv=False, device_build=False, depth=0, static_weight=0.0):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -434,6 +441,8 @@ class F3R:
         p.flags = (1 if history else 0) | (2 if cap_spmv else 0) | (4 if device_build else 0)
         p.grid = grid
         p.index = index
+        p.depth = DEPTHS[depth] if isinstance(depth, str) else int(depth)
+        p.static_weight = float(static_weight)
         self.params = p
         h = ctypes.c_void_p()
         check(lib.f3r_solver_create(ctx.handle, A.handle(ctx), M.handle, ctypes.byref(p), ctypes.byref(h)),

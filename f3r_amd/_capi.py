@@ -26,7 +26,7 @@ class Params(ctypes.Structure):
     _fields_ = [("m1", c_i32), ("m2", c_i32), ("m3", c_i32), ("m4", c_i32), ("c", c_i32),
                 ("restart", c_i32), ("eps", c_dbl), ("variant", c_i32), ("range_scale", c_dbl),
                 ("C", c_i32), ("sigma", c_i32), ("K", c_i32), ("use_graph", c_i32), ("flags", c_i32),
-                ("grid", c_i32), ("index", c_i32)]
+                ("grid", c_i32), ("index", c_i32), ("static_weight", c_dbl), ("depth", c_i32)]
 
 
 class Result(ctypes.Structure):

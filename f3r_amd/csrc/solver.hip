@@ -543,11 +543,12 @@ __global__ __launch_bounds__(kBlock) void k_probe(int64_t n, const half_t *__res
   }
 }
 
-__global__ void k_rich_reset(RichDev *st) {
+// weights 1 (adaptive start), or the static weight w > 0 for every step
+__global__ void k_rich_reset(RichDev *st, double w) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     for (int k = 0; k < 16; ++k) {
       st->sum[k] = 1.0;
-      st->ome[k] = 1.0;
+      st->ome[k] = w > 0 ? w : 1.0;
       st->alpha[k] = 0.0;
     }
     st->nprobe = 0;
@@ -589,6 +590,99 @@ __global__ __launch_bounds__(kBlock) void k_probe_t(int64_t n, const T *__restri
   }
 }
 
+// ------------------------------------------ generic levels (depth study) ---
+// FGMRES levels of the nesting-depth ablations (work/src/depth.cpp:122-167):
+// basis storage S (fp32 or fp16), fp32 arithmetic.  An fp16 level keeps its
+// vectors scaled by a power of two s (stored = s * value) so unit vectors
+// stay clear of fp16 subnormals; dots and norms take 1/s^2, the output 1/s.
+
+// out = SO(in * factor) (conversion into a level's storage)
+template <class SI, class SO>
+__global__ void k_g_entry(int64_t n, const SI *__restrict__ in, SO *__restrict__ out, double factor) {
+  using L = decltype(to_acc(SI{}));
+  const L f = (L)factor;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    out[i] = cvt<SO>(to_acc(in[i]) * f);
+}
+
+// block sums of <w, V_k> for the 16 columns of blockIdx.y (k < ncols) into
+// partial[k * gridDim.x + block]
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_g_mdot(int64_t n, const S *__restrict__ w, const S *__restrict__ V,
+                                                   int64_t ld, int ncols, float *__restrict__ partial) {
+  constexpr int NV = 16;
+  __shared__ float lds[NV * kWavesPerBlock];
+  const int k0 = blockIdx.y * NV;
+  float v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float wi = to_acc(w[i]);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k0 + k < ncols) v[k] = fma_<float>(wi, to_acc(V[(int64_t)(k0 + k) * ld + i]), v[k]);
+  }
+  block_sum<float, NV>(v, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV && k0 + k < ncols; ++k) partial[(int64_t)(k0 + k) * gridDim.x + blockIdx.x] = v[k];
+}
+
+// Hcol[k] = scale * the G block sums of column k (one block per column)
+__global__ __launch_bounds__(kBlock) void k_g_dots_finish(const float *__restrict__ partial, int G, float *Hcol,
+                                                          float scale) {
+  __shared__ float lds[kWavesPerBlock];
+  const int k = blockIdx.x;
+  float v[1] = {0.f};
+  for (int b = threadIdx.x; b < G; b += kBlock) v[0] += partial[(int64_t)k * G + b];
+  block_sum<float, 1>(v, lds);
+  if (threadIdx.x == 0) Hcol[k] = v[0] * scale;
+}
+
+// w = S(w - V h) over ncols <= 128 columns; block sums of scale * w^2 of the
+// stored w into partial[block] (k_cgs_finish adds them)
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_g_update(int64_t n, S *__restrict__ w, const S *__restrict__ V, int64_t ld,
+                                                     int ncols, const float *Hcol, float scale,
+                                                     float *__restrict__ partial) {
+  __shared__ float h[128];
+  __shared__ float lds[kWavesPerBlock];
+  for (int k = threadIdx.x; k < ncols; k += kBlock) h[k] = Hcol[k];
+  __syncthreads();
+  float v[1] = {0.f};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    float r = to_acc(V[i]) * h[0];
+    for (int k = 1; k < ncols; ++k) r = fma_<float>(to_acc(V[(int64_t)k * ld + i]), h[k], r);
+    const S ws = cvt<S>(to_acc(w[i]) - r);
+    w[i] = ws;
+    const float wf = to_acc(ws);
+    v[0] = fma_<float>(wf * scale, wf, v[0]);
+  }
+  block_sum<float, 1>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+template <class S> __global__ void k_g_vscale(int64_t n, S *__restrict__ v, const float *inv) {
+  const float a = *inv;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    v[i] = cvt<S>(to_acc(v[i]) * a);
+}
+
+// out = TO(factor * Z y), y from the level's m x m triangle (every block
+// solves it; m <= 128)
+template <class S, class TO>
+__global__ __launch_bounds__(kBlock) void k_g_combine(int64_t n, const S *__restrict__ Z, int64_t ld, int m,
+                                                      const float *H, int ldh, const float *e, TO *__restrict__ out,
+                                                      float factor) {
+  __shared__ float y[128];
+  if (threadIdx.x == 0) hess_solve<float>(H, ldh, e, y, m);
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    float r = to_acc(Z[i]) * y[0];
+    for (int k = 1; k < m; ++k) r = fma_<float>(to_acc(Z[(int64_t)k * ld + i]), y[k], r);
+    out[i] = cvt<TO>(r * factor);
+  }
+}
+
 // ------------------------------------------------------------ the solver ---
 template <class T> struct InnerLevel {
   int m = 0;
@@ -613,6 +707,19 @@ struct Solver {
   // inner levels: fp32 basis (HALF, FLOAT) or fp64 basis (DOUBLE)
   InnerLevel<float> L2f, L3f;
   InnerLevel<double> L2d, L3d;
+  // the nesting-depth ablations: FGMRES levels from the outermost inner one
+  // down, then the preconditioner applied directly
+  struct GLevel {
+    int m = 0;
+    int store = F3R_F32;  // basis storage: F3R_F32 or F3R_F16 (fp32 arithmetic)
+    int mat = F3R_F32;    // the level's matrix values: F3R_F32 or F3R_F16
+    float s = 1.f;        // storage scale (stored = s * value), fp16 levels
+    int64_t ld = 0;
+    DevPtr V, Z, H, cs, sn, e, inv;
+    template <class S> S *Vc(int k) const { return V->as<S>() + (int64_t)k * ld; }
+    template <class S> S *Zc(int k) const { return Z->as<S>() + (int64_t)k * ld; }
+  };
+  std::vector<GLevel> glev;
   // Richardson work vectors (fp16 in HALF, else the basis type)
   DevPtr hin, hout, hout2, hr, tmp2, tt, rich;
   int64_t cnt = 0, calls = 0;
@@ -732,7 +839,7 @@ struct Solver {
   void level4_half(const float *in, float *out) {
     ++cnt;
     ++calls;
-    const bool probe = (cnt % p.c) == 0;
+    const bool probe = p.static_weight <= 0 && (cnt % p.c) == 0;
     const int64_t nn = n;
     const dim3 g(grid()), b(kBlock);
     RichDev *rs = rich->as<RichDev>();
@@ -801,7 +908,7 @@ struct Solver {
   template <class T> void level4_t(const T *in, T *out) {
     ++cnt;
     ++calls;
-    const bool probe = (cnt % p.c) == 0;
+    const bool probe = p.static_weight <= 0 && (cnt % p.c) == 0;
     const int64_t nn = n;
     const dim3 g(grid()), b(kBlock);
     RichDev *rs = rich->as<RichDev>();
@@ -884,7 +991,7 @@ struct Solver {
     const int64_t calls_per = (int64_t)m2() * m3();
     if (calls_per > 63) return UINT64_MAX;
     uint64_t mask = 0;
-    for (int64_t q = 0; q < calls_per; ++q)
+    for (int64_t q = 0; q < calls_per && p.static_weight <= 0; ++q)
       if (((cnt + q + 1) % p.c) == 0) mask |= 1ULL << q;
     return mask | (1ULL << 63);
   }
@@ -922,7 +1029,73 @@ struct Solver {
     solution<T, double>(L, out);
   }
 
+  // ------------------------------------ generic levels (depth ablations) ---
+  // M applied directly to a vector of the caller's storage SI, scaled by s_in
+  template <class SI> void mdirect(const SI *in, SI *out, float s_in) {
+    ++cnt;
+    ++calls;
+    if constexpr (std::is_same<SI, half_t>::value) {
+      prec_apply_raw(P.get(), in, out, st());  // M is linear: the scale carries through
+    } else {
+      half_t *hi = hin->as<half_t>(), *ho = hout->as<half_t>();
+      hipLaunchKernelGGL((k_g_entry<SI, half_t>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, in, hi,
+                         (double)range_s / s_in);
+      prec_apply_raw(P.get(), hi, ho, st());
+      hipLaunchKernelGGL((k_g_entry<half_t, SI>), dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n,
+                         (const half_t *)ho, out, (double)s_in / range_s);
+    }
+  }
+  // generic level l applied to `in` (storage SI, scaled by s_in), out likewise
+  template <class SI> void glevel(int l, const SI *in, SI *out, float s_in) {
+    if (l == (int)glev.size()) {
+      mdirect<SI>(in, out, s_in);
+      return;
+    }
+    if (glev[l].store == F3R_F16)
+      glevel_t<SI, half_t>(l, in, out, s_in);
+    else
+      glevel_t<SI, float>(l, in, out, s_in);
+  }
+  // FGMRES(m) with normalize = false (solver/gmres2.hpp:84-145): V_0 = in,
+  // e_0 = 1; CGS with the dots, the update and the norm as separate passes
+  template <class SI, class S> void glevel_t(int l, const SI *in, SI *out, float s_in) {
+    GLevel &L = glev[l];
+    const int64_t nn = n;
+    const dim3 g(grid()), b(kBlock);
+    const float sc = 1.f / (L.s * L.s);
+    const int G = dots_grid;
+    float *part = partial<float>();
+    hipLaunchKernelGGL((k_g_entry<SI, S>), g, b, 0, st(), nn, in, L.Vc<S>(0), (double)L.s / s_in);
+    for (int j = 0; j < L.m; ++j) {
+      glevel<S>(l + 1, L.Vc<S>(j), L.Zc<S>(j), L.s);
+      if (L.mat == F3R_F16)
+        launch_sell<half_t, S, float>(sh16->K, argsv<half_t>(), A16->as<half_t>(), L.Zc<S>(j),
+                                      EpiStore<S, float>{L.Vc<S>(j + 1), nullptr}, st());
+      else
+        launch_sell<float, S, float>(sh->K, args(), A32->as<float>(), L.Zc<S>(j),
+                                     EpiStore<S, float>{L.Vc<S>(j + 1), nullptr}, st());
+      const int ncols = j + 1;
+      float *Hcol = L.H->as<float>() + (int64_t)j * (L.m + 1);
+      hipLaunchKernelGGL(k_g_mdot<S>, dim3(G, (ncols + 15) / 16), b, 0, st(), nn, (const S *)L.Vc<S>(j + 1),
+                         (const S *)L.Vc<S>(0), L.ld, ncols, part);
+      hipLaunchKernelGGL(k_g_dots_finish, dim3(ncols), b, 0, st(), (const float *)part, G, Hcol, sc);
+      hipLaunchKernelGGL(k_g_update<S>, dim3(G), b, 0, st(), nn, L.Vc<S>(j + 1), (const S *)L.Vc<S>(0), L.ld,
+                         ncols, (const float *)Hcol, sc, part);
+      hipLaunchKernelGGL(k_cgs_finish<float>, dim3(1), dim3(1024), 0, st(), (const float *)part, G, Hcol,
+                         L.cs->as<float>(), L.sn->as<float>(), L.e->as<float>(), j, 1, L.inv->as<float>(),
+                         (double *)nullptr);
+      hipLaunchKernelGGL(k_g_vscale<S>, g, b, 0, st(), nn, L.Vc<S>(j + 1), (const float *)L.inv->as<float>());
+    }
+    hipLaunchKernelGGL((k_g_combine<S, SI>), g, b, 0, st(), nn, (const S *)L.Zc<S>(0), L.ld, L.m,
+                       (const float *)L.H->as<float>(), L.m + 1, (const float *)L.e->as<float>(), out,
+                       s_in / L.s);
+  }
+
   void level2_any(const double *in, double *out) {
+    if (!glev.empty()) {
+      glevel<double>(0, in, out, 1.f);
+      return;
+    }
     if (p.variant == 2)
       level2<double>(in, out);
     else
@@ -995,6 +1168,9 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   F3R_CHECK(p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16, F3R_ERR_ARG, "solver: m2, m3 in [1, 16]");
   F3R_CHECK(p.m4 >= 1 && p.m4 <= 16 && p.c >= 1 && p.restart >= 1, F3R_ERR_ARG, "solver: m4 in [1, 16], c, restart >= 1");
   F3R_CHECK(A.n == A.ncols && A.n > 0, F3R_ERR_ARG, "solver: square non-empty matrix");
+  F3R_CHECK(p.depth >= 0 && p.depth <= 5, F3R_ERR_ARG, "solver: depth must be 0 (F3R) or 1..5 (F2 F2H F3 F3H F4)");
+  F3R_CHECK(p.depth == 0 || p.variant == 0, F3R_ERR_ARG, "solver: the depth ablations run in the HALF build (variant 0)");
+  F3R_CHECK(p.static_weight >= 0, F3R_ERR_ARG, "solver: static_weight must be >= 0 (0: adaptive weights)");
   auto S = std::make_unique<Solver>();
   S->ctx = ctx;
   S->p = p;
@@ -1125,7 +1301,38 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
     for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->template as<E>() + (int64_t)k * ld;
     for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->template as<E>() + (int64_t)k * ld;
   };
-  if (p.variant == 2) {
+  if (p.depth != 0) {
+    // the nests of work/src/depth.cpp:122-167: {m, basis storage, matrix values}
+    struct Spec {
+      int m, store, mat;
+    };
+    std::vector<Spec> spec;
+    switch (p.depth) {
+      case 1: spec = {{64, F3R_F32, F3R_F32}}; break;                                           // F2
+      case 2: spec = {{64, F3R_F16, F3R_F16}}; break;                                           // F2H
+      case 3: spec = {{8, F3R_F32, F3R_F32}, {8, F3R_F32, F3R_F16}}; break;                     // F3
+      case 4: spec = {{8, F3R_F32, F3R_F32}, {8, F3R_F16, F3R_F16}}; break;                     // F3H
+      default: spec = {{8, F3R_F32, F3R_F32}, {4, F3R_F32, F3R_F16}, {2, F3R_F16, F3R_F16}};  // F4
+    }
+    const int64_t ld = (n + 63) / 64 * 64;
+    for (const Spec &q : spec) {
+      Solver::GLevel L;
+      L.m = q.m;
+      L.store = q.store;
+      L.mat = q.mat;
+      L.s = q.store == F3R_F16 ? S->range_s : 1.f;
+      L.ld = ld;
+      const size_t es = q.store == F3R_F16 ? sizeof(half_t) : sizeof(float);
+      L.V = dev_alloc(es * ld * (q.m + 1));
+      L.Z = dev_alloc(es * ld * q.m);
+      L.H = dev_alloc(sizeof(float) * (q.m + 1) * q.m);
+      L.cs = dev_alloc(sizeof(float) * q.m);
+      L.sn = dev_alloc(sizeof(float) * q.m);
+      L.e = dev_alloc(sizeof(float) * (q.m + 1));
+      L.inv = dev_alloc(sizeof(float) * 4);
+      S->glev.push_back(L);
+    }
+  } else if (p.variant == 2) {
     mk(S->L2d, p.m2);
     mk(S->L3d, p.m3);
   } else {
@@ -1141,7 +1348,7 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   S->tmp2 = dh();
   S->tt = dh();
   S->rich = dev_alloc(sizeof(RichDev));
-  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, ctx->stream, S->rich->as<RichDev>());
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, ctx->stream, S->rich->as<RichDev>(), p.static_weight);
   F3R_HIP(hipStreamSynchronize(ctx->stream));
   return S.release();
 }
@@ -1167,7 +1374,7 @@ void solver_solve(Solver *S, Vec *b, Vec *x, f3r_result *res) {
   // reset the adaptive weights (src/f3r-gpu.cpp:277-281)
   S->cnt = S->p.c;
   S->calls = 0;
-  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, c->stream, S->rich->as<RichDev>());
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, c->stream, S->rich->as<RichDev>(), S->p.static_weight);
   F3R_HIP(hipStreamSynchronize(c->stream));
   S->hist.clear();
   const bool record = (S->p.flags & 1) != 0;
@@ -1309,10 +1516,12 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
   F3R_CHECK(vi->dtype == F3R_F64 && vo->dtype == F3R_F64 && vi->n == s->n && vo->n == s->n, F3R_ERR_ARG,
             "level_apply: fp64 vectors of the matrix size");
   F3R_CHECK(!s->sh->perm, F3R_ERR_UNSUPPORTED, "level_apply: sorted layouts");
+  F3R_CHECK(s->glev.empty() || level == 2, F3R_ERR_UNSUPPORTED,
+            "level_apply: a depth ablation applies its whole nest as level 2 only");
   const int64_t n = s->n;
   const dim3 g(s->grid()), b(kBlock);
   s->cnt = s->p.c;
-  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, s->st(), s->rich->as<RichDev>());
+  hipLaunchKernelGGL(k_rich_reset, dim3(1), dim3(64), 0, s->st(), s->rich->as<RichDev>(), s->p.static_weight);
   if (level == 2) {
     const int32_t keep = s->p.use_graph;
     s->p.use_graph = 0;

@@ -218,6 +218,18 @@ typedef struct {
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */
+  double static_weight; /* > 0: Richardson with this fixed weight in every step and no probes
+                           (work/src/static.cpp:99-119); 0 = adaptive weights (default) */
+  int32_t depth;       /* 0: the F3R nest (default).  1..5: the nesting-depth ablations of
+                          work/src/depth.cpp:122-167, in the HALF build, M applied directly
+                          under FGMRES levels (basis storage / matrix values):
+                          1 F2  fp32/fp32(64)
+                          2 F2H fp16/fp16(64)
+                          3 F3  fp32/fp32(8) > fp32/fp16(8)
+                          4 F3H fp32/fp32(8) > fp16/fp16(8)
+                          5 F4  fp32/fp32(8) > fp32/fp16(4) > fp16/fp16(2)
+                          fp16 levels compute in fp32 with their vectors scaled by the
+                          range factor; m2, m3, m4, c are not used */
 } f3r_params;
 
 typedef struct {
