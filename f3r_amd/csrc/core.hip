@@ -110,6 +110,47 @@ __global__ void k_trsv_u(int32_t cnt, const int32_t *__restrict__ rows, const in
   out[r] = cvt<S>(s * to_acc(dinv[r]));
 }
 
+// The same solves for a block-Jacobi split: one workgroup per diagonal block
+// walks that block's levels with a barrier between them (the blocks never
+// read each other's rows), so a solve is one launch.  brows lists the rows
+// block by block, level by level; block b's levels start at
+// blptr[bstart[b] .. bstart[b + 1] - 1] (the last entry ends the block).
+template <class S>
+__global__ __launch_bounds__(1024) void k_bj_trsv_l(const int32_t *__restrict__ bstart, const int32_t *__restrict__ blptr,
+                                                    const int32_t *__restrict__ brows, const int32_t *__restrict__ rptr,
+                                                    const int32_t *__restrict__ idx, const S *__restrict__ val,
+                                                    const S *__restrict__ in, S *__restrict__ out) {
+  using A = typename Acc<S>::type;
+  const int32_t q0 = bstart[blockIdx.x], q1 = bstart[blockIdx.x + 1];
+  for (int32_t q = q0; q + 1 < q1; ++q) {
+    for (int32_t t = blptr[q] + threadIdx.x; t < blptr[q + 1]; t += blockDim.x) {
+      const int i = brows[t];
+      A s = to_acc(in[i]);
+      for (int k = rptr[i]; k < rptr[i + 1] - 1; ++k) s = fma_<A>(-to_acc(val[k]), to_acc(out[idx[k]]), s);
+      out[i] = cvt<S>(s);
+    }
+    __syncthreads();
+  }
+}
+template <class S>
+__global__ __launch_bounds__(1024) void k_bj_trsv_u(const int32_t *__restrict__ bstart, const int32_t *__restrict__ blptr,
+                                                    const int32_t *__restrict__ brows, const int32_t *__restrict__ rptr,
+                                                    const int32_t *__restrict__ idx, const S *__restrict__ val,
+                                                    const S *__restrict__ dinv, const S *__restrict__ in,
+                                                    S *__restrict__ out) {
+  using A = typename Acc<S>::type;
+  const int32_t q0 = bstart[blockIdx.x], q1 = bstart[blockIdx.x + 1];
+  for (int32_t q = q0; q + 1 < q1; ++q) {
+    for (int32_t t = blptr[q] + threadIdx.x; t < blptr[q + 1]; t += blockDim.x) {
+      const int r = brows[t];
+      A s = to_acc(in[r]);
+      for (int k = rptr[r + 1] - 1; k > rptr[r]; --k) s = fma_<A>(-to_acc(val[k]), to_acc(out[idx[k]]), s);
+      out[r] = cvt<S>(s * to_acc(dinv[r]));
+    }
+    __syncthreads();
+  }
+}
+
 static int grid_for(int64_t n, int cu) {
   int64_t g = (n + kBlock - 1) / kBlock;
   int64_t cap = (int64_t)cu * 16;
@@ -402,6 +443,35 @@ static std::vector<int32_t> levels_lower(const HostCSR &L, std::vector<int32_t> 
   for (int32_t i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
   return rows;
 }
+// Level-ordered rows regrouped by diagonal block of `bsz` rows (dependencies
+// stay inside a block): brows block by block, each block level by level;
+// blptr[bstart[b] + l] starts level l of block b, blptr[bstart[b + 1] - 1]
+// ends the block.
+static void group_by_block(const std::vector<int32_t> &rows, const std::vector<int32_t> &lptr, int32_t n,
+                           int32_t bsz, std::vector<int32_t> &brows, std::vector<int32_t> &blptr,
+                           std::vector<int32_t> &bstart) {
+  const int32_t nb = (n + bsz - 1) / bsz;
+  std::vector<int32_t> lev(n);
+  for (size_t l = 0; l + 1 < lptr.size(); ++l)
+    for (int32_t k = lptr[l]; k < lptr[l + 1]; ++k) lev[rows[k]] = (int32_t)l;
+  // stable by block: rows keep their level order inside a block
+  std::vector<int32_t> cnt(nb + 1, 0);
+  for (int32_t i = 0; i < n; ++i) cnt[i / bsz + 1]++;
+  for (int32_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+  brows.resize(n);
+  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int32_t r : rows) brows[fill[r / bsz]++] = r;
+  blptr.clear();
+  bstart.assign(nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    bstart[b] = (int32_t)blptr.size();
+    for (int32_t t = cnt[b]; t < cnt[b + 1]; ++t)
+      if (t == cnt[b] || lev[brows[t]] != lev[brows[t - 1]]) blptr.push_back(t);
+    blptr.push_back(cnt[b + 1]);
+  }
+  bstart[nb] = (int32_t)blptr.size();
+}
+
 static std::vector<int32_t> levels_upper(const HostCSR &U, std::vector<int32_t> &lptr) {
   const int32_t n = U.n;
   std::vector<int32_t> lev(n, 0);
@@ -496,8 +566,25 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
     HostCSR U = perm ? permute_csr(P->host.f[1], *perm) : P->host.f[1];
     // a permutation would break the triangular structure: only the identity
     F3R_CHECK(!perm, F3R_ERR_UNSUPPORTED, "ILU(0) with a sorted (sigma > 1) layout");
-    P->l_rows = upload_i32(c, levels_lower(L, P->l_lptr));
-    P->u_rows = upload_i32(c, levels_upper(U, P->u_lptr));
+    const std::vector<int32_t> lrows = levels_lower(L, P->l_lptr), urows = levels_upper(U, P->u_lptr);
+    P->l_rows = upload_i32(c, lrows);
+    P->u_rows = upload_i32(c, urows);
+    // many diagonal blocks: one workgroup per block, one launch per solve
+    const int32_t blocks = (int32_t)P->tol;
+    P->bj_blocks = blocks >= kBjBlockKernel ? blocks : 0;
+    if (P->bj_blocks) {
+      const int32_t bsz = (n + blocks - 1) / blocks;
+      std::vector<int32_t> br, bl, bs;
+      group_by_block(lrows, P->l_lptr, n, bsz, br, bl, bs);
+      P->lb_rows = upload_i32(c, br);
+      P->lb_ptr = upload_i32(c, bl);
+      P->lb_start = upload_i32(c, bs);
+      group_by_block(urows, P->u_lptr, n, bsz, br, bl, bs);
+      P->ub_rows = upload_i32(c, br);
+      P->ub_ptr = upload_i32(c, bl);
+      P->ub_start = upload_i32(c, bs);
+      P->bj_groups = (int32_t)bs.size() - 1;
+    }
     P->l_rptr = upload_i32(c, L.rptr);
     P->l_idx = upload_i32(c, L.idx);
     P->l_val = upload_values(c, L.val, P->dtype);
@@ -512,6 +599,17 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
 
 template <class S> static void ilu_apply_t(Prec *P, const S *in, S *out, hipStream_t st) {
   S *tmp = P->tmp->as<S>();
+  if (P->bj_blocks) {
+    hipLaunchKernelGGL(k_bj_trsv_l<S>, dim3(P->bj_groups), dim3(1024), 0, st, (const int32_t *)P->lb_start->as<int32_t>(),
+                       (const int32_t *)P->lb_ptr->as<int32_t>(), (const int32_t *)P->lb_rows->as<int32_t>(),
+                       (const int32_t *)P->l_rptr->as<int32_t>(), (const int32_t *)P->l_idx->as<int32_t>(),
+                       (const S *)P->l_val->as<S>(), in, tmp);
+    hipLaunchKernelGGL(k_bj_trsv_u<S>, dim3(P->bj_groups), dim3(1024), 0, st, (const int32_t *)P->ub_start->as<int32_t>(),
+                       (const int32_t *)P->ub_ptr->as<int32_t>(), (const int32_t *)P->ub_rows->as<int32_t>(),
+                       (const int32_t *)P->u_rptr->as<int32_t>(), (const int32_t *)P->u_idx->as<int32_t>(),
+                       (const S *)P->u_val->as<S>(), (const S *)P->u_dinv->as<S>(), (const S *)tmp, out);
+    return;
+  }
   for (size_t l = 0; l + 1 < P->l_lptr.size(); ++l) {
     int32_t cnt = P->l_lptr[l + 1] - P->l_lptr[l];
     hipLaunchKernelGGL(k_trsv_l<S>, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,

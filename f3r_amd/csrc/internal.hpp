@@ -25,7 +25,14 @@ struct Prec {
   // ILU(0)
   std::vector<int32_t> l_lptr, u_lptr;  // level pointers into *_rows
   DevPtr l_rows, u_rows, l_rptr, l_idx, l_val, u_rptr, u_idx, u_val, u_dinv;
+  // block-Jacobi split with at least kBjBlockKernel blocks: rows grouped by
+  // block, one workgroup per block (bj_groups of them)
+  int32_t bj_blocks = 0, bj_groups = 0;
+  DevPtr lb_rows, lb_ptr, lb_start, ub_rows, ub_ptr, ub_start;
 };
+// fewer diagonal blocks than this: one launch per level instead (a block
+// per workgroup would leave most of the GPU idle)
+constexpr int32_t kBjBlockKernel = 32;
 
 std::shared_ptr<Context> ctx_create(int device);
 double read_scalar(Context *c, const double *dev);

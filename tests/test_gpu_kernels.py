@@ -196,6 +196,36 @@ def test_ilu0_apply_matches_oracle(ctx):
         assert bad.size == 0, kind
 
 
+@pytest.mark.parametrize("blocks", [31, 32, 64, 100, 128])
+@pytest.mark.parametrize("dtype", ["f64", "f16"])
+def test_bjilu0_block_per_workgroup_matches_oracle(ctx, blocks, dtype):
+    """32 or more diagonal blocks: one workgroup per block walks its levels"""
+    import f3r_amd as F
+    A = F.CSR.convdiff7(16, sigma=1.5, corr=4.0, peclet=10.0, seed=3)
+    F.algorithm.scaling(A)
+    print(f"bjilu0 blocks={blocks} {dtype}", flush=True)
+    P = F.ILU0(A, blocks=blocks, acc=1.0, dtype=dtype, ctx=ctx)
+    x = np.random.default_rng(7).uniform(-1, 1, A.n)
+    h = O.lib().orc_ilu0(A.n, O.P(A.rptr), O.P(A.idx), O.P(A.val), blocks, 1.0)
+    if dtype == "f64":
+        xv = F.vector.from_numpy(ctx, x)
+        yv = F.vector(ctx, "f64", A.n)
+        P.operate(xv, yv)
+        out, tmp = np.zeros(A.n), np.zeros(A.n)
+        O.lib().orc_bj_solve(h, O.P(x), O.P(out), O.P(tmp))
+        got = yv.download()
+    else:
+        xh = x.astype(np.float16)
+        xv = F.vector.from_numpy(ctx, xh)
+        yv = F.vector(ctx, "f16", A.n)
+        P.operate(xv, yv)
+        out, tmp = np.zeros(A.n, np.uint16), np.zeros(A.n, np.uint16)
+        O.lib().orc_bj_solve16(h, O.P(xh.view(np.uint16)), O.P(out), O.P(tmp), 0)
+        got = yv.download().view(np.uint16)
+    O.lib().orc_factor_free(h)
+    assert np.array_equal(got, out)
+
+
 @pytest.mark.parametrize("C,K,offsets", [(64, 4, True), (128, 8, True), (64, 4, False), (256, 2, True)])
 def test_device_sell_build_matches_host(ctx, C, K, offsets):
     # the device-built layout equals the host-built one: same slot count and
