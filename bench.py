@@ -146,7 +146,7 @@ def main():
                     help="factorise SDAINV and build the matrix layouts on the host (default: device)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--learned", type=int, default=4, metavar="N",
+    ap.add_argument("--learned", type=int, default=16, metavar="N",
                     help="held-out 96^3 lognormal systems for learned vs plain F3R (0: skip)")
     ap.add_argument("--setup-only", action="store_true",
                     help="no solves: time the set-up steps (SDAINV and ILU(0) factors, solver layouts) on the "

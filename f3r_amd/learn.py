@@ -301,6 +301,8 @@ def main(argv=None):
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--out", default="learned_sdainv.npz")
     ap.add_argument("--report", default=None)
+    ap.add_argument("--evaluate", default=None, metavar="PARAMS.npz",
+                    help="no training: evaluate these parameters on the held-out systems")
     a = ap.parse_args(argv)
     ctx = Context(0)
     t0 = time.time()
@@ -308,8 +310,17 @@ def main(argv=None):
     def log(m):
         print(f"[learn {time.time() - t0:7.1f}s] {m}", flush=True)
 
-    systems = []
     chunk = max(a.threads * 4, 16)
+    if a.evaluate:
+        d = np.load(a.evaluate, allow_pickle=False)
+        mlp = MLP(ctx, params=d["params"])
+        a.kind = int(d["kind"]) if "kind" in d.files else 1
+        a.train = int(d["train"]) if "train" in d.files else 0
+        base, hist = float(d["base_loss"]) if "base_loss" in d.files else float("nan"), \
+            [float(v) for v in d["loss"]] if "loss" in d.files else [float("nan")]
+        log(f"evaluating {a.evaluate} (loss kind {a.kind}, trained on {a.train} systems)")
+        return _held_out(ctx, mlp, a, base, hist, chunk, log)
+    systems = []
     for c0 in range(0, a.train, chunk):
         for hs in host_systems(a.nx, range(c0, min(a.train, c0 + chunk)), a.threads):
             L = LearnSystem(ctx, hs["A"], hs["P"], hs["feat"])
@@ -324,6 +335,11 @@ def main(argv=None):
              probes=a.probes, epochs=a.epochs, kind=a.kind, lr=a.lr)
     log(f"parameters saved to {a.out}")
     del systems
+    return _held_out(ctx, mlp, a, base, hist, chunk, log)
+
+
+def _held_out(ctx, mlp, a, base, hist, chunk, log):
+    """plain vs learned F3R on the held-out seeds 100000..; summary printed"""
     rep = []
     test_seeds = range(100000, 100000 + a.test)
     for c0 in range(0, a.test, chunk):
