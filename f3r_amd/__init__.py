@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi
 from ._capi import F16, F32, F64, I32, F3RError, check, lib
 
-__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "algorithm", "solve_res_t",
+__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "solve_res_t",
            "F3RError", "F64", "F32", "F16", "device_count"]
 
 _NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32}
@@ -492,6 +492,56 @@ class F3R:
         try:
             if self.handle:
                 lib.f3r_solver_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent systems solved in lockstep on one GPU (f3r_batch_*): segment s
+    is (As[s], Ps[s]) with its own right-hand side; a system with k right-hand
+    sides is listed k times.  Every launch covers all segments; reductions,
+    Hessenberg/Givens updates and Richardson weights are per segment.  Ps are
+    SDAINV preconditioners (fp16).  Solves the HALF nest of F3R."""
+
+    def __init__(self, ctx, As, Ps, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, C=0, K=0, index=1,
+                 range_scale=0.0):
+        self.ctx = ctx
+        self.n = [A.n for A in As]
+        p = _capi.Params()
+        check(lib.f3r_params_default(ctypes.byref(p)), "params_default")
+        p.m1, p.m2, p.m3, p.m4 = m
+        p.c, p.restart, p.eps = c, restart, eps
+        p.C, p.K, p.index, p.range_scale = C, K, index, range_scale
+        ns = len(As)
+        ah = (ctypes.c_void_p * ns)(*[A.handle(ctx) for A in As])
+        ph = (ctypes.c_void_p * ns)(*[P.handle for P in Ps])
+        self._keep = (As, Ps)
+        h = ctypes.c_void_p()
+        check(lib.f3r_batch_create(ctx.handle, ns, ah, ph, ctypes.byref(p), ctypes.byref(h)), "batch_create")
+        self.handle = h
+
+    def info(self):
+        """(padded rows of all segments, segments)"""
+        rows, segs = ctypes.c_int64(), ctypes.c_int32()
+        check(lib.f3r_batch_info(self.handle, ctypes.byref(rows), ctypes.byref(segs)), "batch_info")
+        return rows.value, segs.value
+
+    def solve(self, bs, xs):
+        """solves every segment: bs, xs lists of fp64 vectors; returns one
+        solve_res_t per segment (solve_s: the whole batch)"""
+        ns = len(bs)
+        bh = (ctypes.c_void_p * ns)(*[b.handle for b in bs])
+        xh = (ctypes.c_void_p * ns)(*[x.handle for x in xs])
+        res = (_capi.Result * ns)()
+        check(lib.f3r_batch_solve(self.handle, bh, xh, res), "batch_solve")
+        return [solve_res_t(bool(r.is_solved), r.res_iter, r.res_nrm2, r.nrm_b, r.solve_s, r.precond_calls,
+                            r.outer_cycles) for r in res]
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_batch_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

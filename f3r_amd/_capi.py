@@ -83,6 +83,10 @@ SIGNATURES = [
                                           ctypes.POINTER(c_vp)]),
     ("f3r_prec_setup_where", ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int)]),
     ("f3r_factor_levels", ctypes.c_int, [c_vp, ctypes.c_int, c_vp, P_i32]),
+    ("f3r_batch_create", ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, ctypes.POINTER(Params), ctypes.POINTER(c_vp)]),
+    ("f3r_batch_solve", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(Result)]),
+    ("f3r_batch_info", ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), P_i32]),
+    ("f3r_batch_destroy", ctypes.c_int, [c_vp]),
     ("f3r_prec_apply", ctypes.c_int, [c_vp, c_vp, c_vp]),
     ("f3r_prec_factor_nnz", ctypes.c_int, [c_vp, ctypes.c_int, P_i64]),
     ("f3r_prec_factor_get", ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp]),

@@ -1,0 +1,845 @@
+// Batched F3R: many independent systems (and right-hand sides) solved in
+// lockstep, every launch covering all of them.
+//
+// Each segment (one system with one right-hand side) occupies a contiguous
+// row range of one block-diagonal matrix, padded to a whole number of
+// 2048-row tiles, so a workgroup never straddles two segments.  The SpMVs are
+// the single solver's SELL kernels on the concatenated matrices (fp64, fp32,
+// fp16 values; the SDAINV factors likewise).  Every reduction is per
+// segment: each block writes its partial sums, and one block per segment
+// adds its blocks' partials in a fixed order and does that segment's
+// Hessenberg / Givens / Richardson-weight work.  The host reads one array of
+// residual norms per outer step for all segments.  A segment that has
+// converged stops counting steps and keeps its solution; its vectors may run
+// on with the others, and the breakdown guards below (a zero norm, a zero
+// rotation, a zero pivot) keep them finite.
+//
+// The nest is the HALF build's: fp64 FGMRES(m1) with restarts > fp32
+// FGMRES(m2) on fp32 values > fp32-basis FGMRES(m3) on fp16 values >
+// adaptive Richardson(m4, c) in fp16 with SDAINV (work/src/f3r-gpu.cpp:
+// 210-266), each segment with its own probes' weights.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "internal.hpp"
+#include "sell_kernel.hpp"
+
+namespace f3r {
+namespace {
+
+// ---------------------------------------------------- device helpers ---
+template <class T> __device__ void givens_guarded(T *H, T *cs, T *sn, T *e, int j, bool e0_one) {
+  for (int i = 0; i < j; ++i) {
+    T t = H[i], h1 = H[i + 1];
+    H[i] = fma_<T>(cs[i], t, sn[i] * h1);
+    H[i + 1] = fma_<T>(-sn[i], t, cs[i] * h1);
+  }
+  T t = (T)sqrt(fma_<T>(H[j], H[j], H[j + 1] * H[j + 1]));
+  if (t > (T)0) {
+    cs[j] = H[j] / t;
+    sn[j] = H[j + 1] / t;
+  } else {  // a zero column (a segment that has run dry): identity rotation
+    cs[j] = 1;
+    sn[j] = 0;
+  }
+  H[j] = t;
+  H[j + 1] = 0;
+  T ej = (j == 0 && e0_one) ? (T)1 : e[j];
+  e[j + 1] = -sn[j] * ej;
+  e[j] = cs[j] * ej;
+}
+
+template <class T> __device__ void hess_solve_guarded(const T *H, int ldh, const T *e, T *y, int j) {
+  for (int i = j - 1; i >= 0; --i) {
+    T t = e[i];
+    for (int q = j - 1; q > i; --q) t = fma_<T>(-H[q * ldh + i], y[q], t);
+    const T d = H[i * ldh + i];
+    y[i] = d != (T)0 ? t / d : (T)0;
+  }
+}
+
+// Per-segment Richardson state (work/src/f3r-gpu.cpp:224-260)
+struct RichSeg {
+  double sum[16];
+  double ome[16];
+  double alpha[16];
+  int nprobe;
+};
+
+__global__ void k_b_rich_reset(int nseg, RichSeg *st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  for (int k = 0; k < 16; ++k) {
+    st[s].sum[k] = 1.0;
+    st[s].ome[k] = 1.0;
+    st[s].alpha[k] = 0.0;
+  }
+  st[s].nprobe = 0;
+}
+
+// ----------------------------------------------------------- kernels ---
+// Block b covers the tile of rows [b * kTile, (b + 1) * kTile), all in
+// segment bseg[b]; thread t takes rows b * kTile + t + q * kBlock, q < kRows
+// (coalesced).  Segments are padded to whole tiles, so the grid is exactly
+// the row count / kTile.
+constexpr int kRows = 8;
+constexpr int64_t kTile = (int64_t)kBlock * kRows;
+
+__device__ __forceinline__ int64_t tile_row(int q) { return (int64_t)blockIdx.x * kTile + threadIdx.x + (int64_t)q * kBlock; }
+
+// block partial sums of <w, V_k> for the NV columns of blockIdx.y (k < ncols):
+// partial[k * gridDim.x + block]
+template <class S, class A, int NV>
+__global__ __launch_bounds__(kBlock) void k_b_mdot(const S *__restrict__ w, const S *__restrict__ V, int64_t ld,
+                                                   int ncols, A *__restrict__ partial) {
+  __shared__ A lds[NV * kWavesPerBlock];
+  const int k0 = blockIdx.y * NV;
+  A v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = (A)0;
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    const A wi = (A)to_acc(w[i]);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k0 + k < ncols) v[k] = fma_<A>(wi, (A)to_acc(V[(int64_t)(k0 + k) * ld + i]), v[k]);
+  }
+  block_sum<A, NV>(v, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV && k0 + k < ncols; ++k) partial[(int64_t)(k0 + k) * gridDim.x + blockIdx.x] = v[k];
+}
+
+// H[seg * hs + k] = the partials of segment seg's blocks, added in block
+// order; grid (segments, columns)
+template <class A>
+__global__ __launch_bounds__(kBlock) void k_b_dots_finish(const A *__restrict__ partial, int G,
+                                                          const int32_t *__restrict__ sb0,
+                                                          const int32_t *__restrict__ snb, A *H, int64_t hs) {
+  __shared__ A lds[kWavesPerBlock];
+  const int seg = blockIdx.x, k = blockIdx.y;
+  const int32_t b0 = sb0[seg], nb = snb[seg];
+  A v[1] = {(A)0};
+  for (int32_t b = threadIdx.x; b < nb; b += kBlock) v[0] += partial[(int64_t)k * G + b0 + b];
+  block_sum<A, 1>(v, lds);
+  if (threadIdx.x == 0) H[(int64_t)seg * hs + k] = v[0];
+}
+
+// w = S(w - V h) with h of the block's segment (MULMV order); block partial
+// of ||w||^2 of the stored w
+template <class S, class A>
+__global__ __launch_bounds__(kBlock) void k_b_update(S *__restrict__ w, const S *__restrict__ V, int64_t ld, int ncols,
+                                                     const A *H, int64_t hs, const int32_t *__restrict__ bseg,
+                                                     A *__restrict__ partial) {
+  __shared__ A h[128];
+  __shared__ A lds[kWavesPerBlock];
+  const int seg = bseg[blockIdx.x];
+  for (int k = threadIdx.x; k < ncols; k += kBlock) h[k] = H[(int64_t)seg * hs + k];
+  __syncthreads();
+  A v[1] = {(A)0};
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    A r = (A)to_acc(V[i]) * h[0];
+    for (int k = 1; k < ncols; ++k) r = fma_<A>((A)to_acc(V[(int64_t)k * ld + i]), h[k], r);
+    const S ws = cvt<S>((A)to_acc(w[i]) - r);
+    w[i] = ws;
+    const A wf = (A)to_acc(ws);
+    v[0] = fma_<A>(wf, wf, v[0]);
+  }
+  block_sum<A, 1>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// block partial of ||v||^2 (the outer level's residual norm)
+template <class A>
+__global__ __launch_bounds__(kBlock) void k_b_norm2(const A *__restrict__ v, A *__restrict__ partial) {
+  __shared__ A lds[kWavesPerBlock];
+  A s[1] = {(A)0};
+  for (int q = 0; q < kRows; ++q) {
+    const A x = v[tile_row(q)];
+    s[0] = fma_<A>(x, x, s[0]);
+  }
+  block_sum<A, 1>(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// One block per segment: ||w|| from its blocks' partials, then H(j+1, j),
+// 1/||w|| (0 for a zero vector), the guarded Givens step and |e_{j+1}|.
+// H, e, cs, sn point at segment 0's column j / arrays; strides per segment.
+template <class A>
+__global__ __launch_bounds__(kBlock) void k_b_norm_finish(const A *__restrict__ partial, const int32_t *__restrict__ sb0,
+                                                          const int32_t *__restrict__ snb, A *H, int64_t hs, A *cs,
+                                                          A *sn, int64_t ms, A *e, int64_t es, int j, int e0_one,
+                                                          A *inv, double *status) {
+  __shared__ A lds[kWavesPerBlock];
+  const int seg = blockIdx.x;
+  const int32_t b0 = sb0[seg], nb = snb[seg];
+  A v[1] = {(A)0};
+  for (int32_t b = threadIdx.x; b < nb; b += kBlock) v[0] += partial[b0 + b];
+  block_sum<A, 1>(v, lds);
+  if (threadIdx.x == 0) {
+    A *Hc = H + (int64_t)seg * hs;
+    const A nv = (A)sqrt(v[0]);
+    Hc[j + 1] = nv;
+    inv[seg] = nv > (A)0 ? (A)1 / nv : (A)0;
+    if (cs) {
+      A *ee = e + (int64_t)seg * es;
+      givens_guarded<A>(Hc, cs + (int64_t)seg * ms, sn + (int64_t)seg * ms, ee, j, e0_one != 0);
+      if (status) status[seg] = fabs((double)ee[j + 1]);
+    } else if (status) {
+      status[seg] = (double)nv;  // the residual norm itself (start of a cycle)
+    }
+  }
+}
+
+// v *= inv[segment of the tile]
+template <class S, class A>
+__global__ __launch_bounds__(kBlock) void k_b_vscale(S *__restrict__ v, const A *__restrict__ inv,
+                                                     const int32_t *__restrict__ bseg) {
+  const A a = inv[bseg[blockIdx.x]];
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    v[i] = cvt<S>((A)to_acc(v[i]) * a);
+  }
+}
+
+// out = TO(Z y) (accumulate: out += Z y) with y solved from the block's
+// segment's j x j triangle; j = jseg[segment] or jall (jseg null)
+template <class S, class A, class TO>
+__global__ __launch_bounds__(kBlock) void k_b_combine(const S *__restrict__ Z, int64_t ld, const A *H, int64_t hs,
+                                                      int ldh, const A *e, int64_t es, const int32_t *jseg, int jall,
+                                                      const int32_t *__restrict__ bseg, TO *__restrict__ out,
+                                                      int accumulate) {
+  __shared__ A y[128];
+  const int seg = bseg[blockIdx.x];
+  const int j = jseg ? jseg[seg] : jall;
+  if (j == 0) {
+    if (!accumulate)
+      for (int q = 0; q < kRows; ++q) out[tile_row(q)] = cvt<TO>((A)0);
+    return;
+  }
+  if (threadIdx.x == 0) hess_solve_guarded<A>(H + (int64_t)seg * hs, ldh, e + (int64_t)seg * es, y, j);
+  __syncthreads();
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    A r = (A)to_acc(Z[i]) * y[0];
+    for (int k = 1; k < j; ++k) r = fma_<A>((A)to_acc(Z[(int64_t)k * ld + i]), y[k], r);
+    if (accumulate)
+      out[i] = cvt<TO>((A)to_acc(out[i]) + r);
+    else
+      out[i] = cvt<TO>(r);
+  }
+}
+
+template <class SI, class SO>
+__global__ __launch_bounds__(kBlock) void k_b_cast(const SI *__restrict__ in, SO *__restrict__ out) {
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    out[i] = cvt<SO>(to_acc(in[i]));
+  }
+}
+
+// out = SO(in * f[segment]) (conversions into and out of the fp16 level)
+template <class SI, class SO>
+__global__ __launch_bounds__(kBlock) void k_b_entry(const SI *__restrict__ in, SO *__restrict__ out,
+                                                    const float *__restrict__ f, const int32_t *__restrict__ bseg) {
+  const float a = f[bseg[blockIdx.x]];
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    out[i] = cvt<SO>(to_acc(in[i]) * a);
+  }
+}
+
+// block partials of <t, t> and <t, r> (Richardson probe): partial[q * G + b]
+__global__ __launch_bounds__(kBlock) void k_b_probe_dots(const half_t *__restrict__ t, const half_t *__restrict__ r,
+                                                         float *__restrict__ partial) {
+  __shared__ float lds[2 * kWavesPerBlock];
+  float v[2] = {0.f, 0.f};
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    const float ti = to_acc(t[i]);
+    v[0] = fma_<float>(ti, ti, v[0]);
+    v[1] = fma_<float>(ti, to_acc(r[i]), v[1]);
+  }
+  block_sum<float, 2>(v, lds);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = v[0];
+    partial[(int64_t)gridDim.x + blockIdx.x] = v[1];
+  }
+}
+
+// one block per segment: alpha_k = <t, r> / <t, t>, then the running mean of
+// the weights (probe count den = probes + 1: the initial weight 1 counts)
+__global__ __launch_bounds__(kBlock) void k_b_probe_finish(const float *__restrict__ partial, int G,
+                                                           const int32_t *__restrict__ sb0,
+                                                           const int32_t *__restrict__ snb, RichSeg *st, int k) {
+  __shared__ float lds[2 * kWavesPerBlock];
+  const int seg = blockIdx.x;
+  const int32_t b0 = sb0[seg], nb = snb[seg];
+  float v[2] = {0.f, 0.f};
+  for (int32_t b = threadIdx.x; b < nb; b += kBlock) {
+    v[0] += partial[b0 + b];
+    v[1] += partial[(int64_t)G + b0 + b];
+  }
+  block_sum<float, 2>(v, lds);
+  if (threadIdx.x == 0) {
+    RichSeg &s = st[seg];
+    const float al = v[0] > 0.f ? v[1] / v[0] : 0.f;
+    if (k == 0) s.nprobe += 1;
+    const float den = (float)(s.nprobe + 1);
+    const float sm = (float)s.sum[k] + al;
+    s.sum[k] = sm;
+    s.ome[k] = sm / den;
+    s.alpha[k] = al;
+  }
+}
+
+// ho = fp16(ho * w) (step 0) or ho2 = fp16(w * t2 + ho) (step k), w the
+// segment's weight (alpha on a probe call, else the running mean)
+__global__ __launch_bounds__(kBlock) void k_b_l4_scale(half_t *__restrict__ ho, const RichSeg *__restrict__ st,
+                                                       int probe, const int32_t *__restrict__ bseg) {
+  const RichSeg &s = st[bseg[blockIdx.x]];
+  const float w = (float)(probe ? s.alpha[0] : s.ome[0]);
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    ho[i] = cvt<half_t>(to_acc(ho[i]) * w);
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_b_l4_update(const half_t *__restrict__ t2, const half_t *__restrict__ ho,
+                                                        half_t *__restrict__ ho2, const RichSeg *__restrict__ st, int k,
+                                                        int probe, const int32_t *__restrict__ bseg) {
+  const RichSeg &s = st[bseg[blockIdx.x]];
+  const float w = (float)(probe ? s.alpha[k] : s.ome[k]);
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    ho2[i] = cvt<half_t>(fma_<float>(w, to_acc(t2[i]), to_acc(ho[i])));
+  }
+}
+
+// An SDAINV factor split by row kind: a row whose only entry is its diagonal
+// is an elementwise product (dval), any other row is a CSR row summed in
+// column order with fp32 accumulation, as the SELL SpMV sums it.
+__global__ __launch_bounds__(kBlock) void k_b_fdiag(const half_t *__restrict__ x, const half_t *__restrict__ d,
+                                                    half_t *__restrict__ y) {
+  for (int q = 0; q < kRows; ++q) {
+    const int64_t i = tile_row(q);
+    y[i] = cvt<half_t>(to_acc(d[i]) * to_acc(x[i]));
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_b_frows(int32_t nr, const int32_t *__restrict__ rows,
+                                                    const int32_t *__restrict__ rptr, const int32_t *__restrict__ col,
+                                                    const half_t *__restrict__ val, const half_t *__restrict__ x,
+                                                    half_t *__restrict__ y) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nr) return;
+  float acc = 0.f;
+  for (int32_t k = rptr[t]; k < rptr[t + 1]; ++k) acc = fma_<float>(to_acc(val[k]), to_acc(x[col[k]]), acc);
+  y[rows[t]] = cvt<half_t>(acc);
+}
+
+struct Factor16 {
+  DevPtr dval, rows, rptr, col, val;
+  int32_t nr = 0;
+};
+
+// ----------------------------------------------------------- solver ---
+template <class T> struct BLevel {
+  int m = 0;
+  DevPtr V, Z, H, cs, sn, e, inv;  // V, Z column-major (ld = N); per segment H, cs, sn, e, inv
+  T *Vc(int64_t N, int k) const { return V->as<T>() + (int64_t)k * N; }
+  T *Zc(int64_t N, int k) const { return Z->as<T>() + (int64_t)k * N; }
+  int64_t hs() const { return (int64_t)(m + 1) * m; }
+};
+
+HostCSR concat(const std::vector<const HostCSR *> &parts, const std::vector<int64_t> &off, int64_t N) {
+  HostCSR R;
+  int64_t nnz = 0;
+  for (const HostCSR *p : parts) nnz += p->nnz();
+  F3R_CHECK(N < INT32_MAX && nnz < INT32_MAX, F3R_ERR_RANGE, "batch: more than 2^31 rows or nonzeros in all");
+  R.n = R.ncols = (int32_t)N;
+  R.rptr.assign(N + 1, 0);
+  R.idx.resize(nnz);
+  R.val.resize(nnz);
+  int64_t q = 0;
+  for (size_t s = 0; s < parts.size(); ++s) {
+    const HostCSR &A = *parts[s];
+    const int32_t o = (int32_t)off[s];
+    for (int32_t i = 0; i < A.n; ++i) {
+      for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k, ++q) {
+        R.idx[q] = A.idx[k] + o;
+        R.val[q] = A.val[k];
+      }
+      R.rptr[o + i + 1] = (int32_t)q;
+    }
+    for (int64_t r = o + A.n; r < off[s + 1]; ++r) R.rptr[r + 1] = (int32_t)q;  // padding rows: empty
+  }
+  return R;
+}
+
+}  // namespace
+
+struct BatchSolver {
+  std::shared_ptr<Context> ctx;
+  f3r_params p{};
+  int nseg = 0;
+  int64_t N = 0;
+  int G = 0;  // blocks of kBlock rows
+  std::vector<int64_t> off;
+  std::vector<int32_t> nrow;
+  DevPtr bseg, sb0, snb, scale, iscale;
+  std::shared_ptr<SellShared> sh, sh16;
+  DevPtr A64, A32, A16;
+  Factor16 W, Z;
+  DevPtr partial, status, jseg, rich;
+  DevPtr b_int, x_int, r_int;
+  // outer level (fp64)
+  BLevel<double> L1;
+  BLevel<float> L2, L3;
+  DevPtr hin, hout, hout2, hr, tmp, tt, t2;
+  int64_t cnt = 0, calls = 0;
+
+  hipStream_t st() const { return ctx->stream; }
+  int32_t *bs() const { return bseg->as<int32_t>(); }
+
+  template <class T> void alloc_level(BLevel<T> &L, int m) {
+    L.m = m;
+    L.V = dev_alloc(sizeof(T) * N * (m + 1));
+    L.Z = dev_alloc(sizeof(T) * N * m);
+    L.H = dev_alloc(sizeof(T) * nseg * L.hs());
+    L.cs = dev_alloc(sizeof(T) * nseg * m);
+    L.sn = dev_alloc(sizeof(T) * nseg * m);
+    L.e = dev_alloc(sizeof(T) * nseg * (m + 1));
+    L.inv = dev_alloc(sizeof(T) * nseg);
+  }
+
+  // one Arnoldi step j of level L (w = V_{j+1} already holds A z_j)
+  template <class T> void arnoldi(BLevel<T> &L, int j, int e0_one, double *stat) {
+    const int ncols = j + 1;
+    T *H = L.H->template as<T>();
+    T *part = partial->as<T>();
+    if (ncols <= 4)
+      hipLaunchKernelGGL((k_b_mdot<T, T, 4>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1),
+                         (const T *)L.Vc(N, 0), N, ncols, part);
+    else if (ncols <= 8)
+      hipLaunchKernelGGL((k_b_mdot<T, T, 8>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1),
+                         (const T *)L.Vc(N, 0), N, ncols, part);
+    else
+      hipLaunchKernelGGL((k_b_mdot<T, T, 16>), dim3(G, (ncols + 15) / 16), dim3(kBlock), 0, st(),
+                         (const T *)L.Vc(N, j + 1), (const T *)L.Vc(N, 0), N, ncols, part);
+    hipLaunchKernelGGL(k_b_dots_finish<T>, dim3(nseg, ncols), dim3(kBlock), 0, st(), (const T *)part, G,
+                       (const int32_t *)sb0->as<int32_t>(), (const int32_t *)snb->as<int32_t>(),
+                       H + (int64_t)j * (L.m + 1), L.hs());
+    hipLaunchKernelGGL((k_b_update<T, T>), dim3(G), dim3(kBlock), 0, st(), L.Vc(N, j + 1), (const T *)L.Vc(N, 0), N,
+                       ncols, (const T *)(H + (int64_t)j * (L.m + 1)), L.hs(), (const int32_t *)bs(), part);
+    hipLaunchKernelGGL(k_b_norm_finish<T>, dim3(nseg), dim3(kBlock), 0, st(), (const T *)part,
+                       (const int32_t *)sb0->as<int32_t>(), (const int32_t *)snb->as<int32_t>(),
+                       H + (int64_t)j * (L.m + 1), L.hs(), L.cs->template as<T>(), L.sn->template as<T>(),
+                       (int64_t)L.m, L.e->template as<T>(), (int64_t)(L.m + 1), j, e0_one, L.inv->template as<T>(),
+                       stat);
+    hipLaunchKernelGGL((k_b_vscale<T, T>), dim3(G), dim3(kBlock), 0, st(), L.Vc(N, j + 1),
+                       (const T *)L.inv->template as<T>(), (const int32_t *)bs());
+  }
+
+  void apply_factor(const Factor16 &F, const half_t *x, half_t *y) {
+    hipLaunchKernelGGL(k_b_fdiag, dim3(G), dim3(kBlock), 0, st(), x, (const half_t *)F.dval->as<half_t>(), y);
+    if (F.nr > 0)
+      hipLaunchKernelGGL(k_b_frows, dim3((F.nr + kBlock - 1) / kBlock), dim3(kBlock), 0, st(), F.nr,
+                         (const int32_t *)F.rows->as<int32_t>(), (const int32_t *)F.rptr->as<int32_t>(),
+                         (const int32_t *)F.col->as<int32_t>(), (const half_t *)F.val->as<half_t>(), x, y);
+  }
+  // M = Z W^T (ainv.hpp:110-114): W^T first, then Z
+  void precond(const half_t *in, half_t *out) {
+    half_t *t = tmp->as<half_t>();
+    apply_factor(W, in, t);
+    apply_factor(Z, (const half_t *)t, out);
+  }
+
+  void probe(const half_t *x, const half_t *r, int k) {
+    half_t *t = tt->as<half_t>();
+    launch_sell<half_t, half_t, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(), x,
+                                       EpiStore<half_t, float>{t, nullptr}, st());
+    float *part = partial->as<float>();
+    hipLaunchKernelGGL(k_b_probe_dots, dim3(G), dim3(kBlock), 0, st(), (const half_t *)t, r, part);
+    hipLaunchKernelGGL(k_b_probe_finish, dim3(nseg), dim3(kBlock), 0, st(), (const float *)part, G,
+                       (const int32_t *)sb0->as<int32_t>(), (const int32_t *)snb->as<int32_t>(),
+                       rich->as<RichSeg>(), k);
+  }
+
+  // adaptive Richardson in fp16 (work/src/f3r-gpu.cpp:229-260), per segment
+  void level4(const float *in, float *out) {
+    ++cnt;
+    ++calls;
+    const bool pr = (cnt % p.c) == 0;
+    half_t *hi = hin->as<half_t>(), *ho = hout->as<half_t>(), *ho2 = hout2->as<half_t>();
+    half_t *hr_ = hr->as<half_t>(), *tw = t2->as<half_t>();
+    RichSeg *rs = rich->as<RichSeg>();
+    hipLaunchKernelGGL((k_b_entry<float, half_t>), dim3(G), dim3(kBlock), 0, st(), in, hi,
+                       (const float *)scale->as<float>(), (const int32_t *)bs());
+    precond(hi, ho);
+    if (pr) probe(ho, hi, 0);
+    hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, pr ? 1 : 0,
+                       (const int32_t *)bs());
+    for (int k = 1; k < p.m4; ++k) {
+      launch_sell<half_t, half_t, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(), (const half_t *)ho,
+                                         EpiResid<half_t, half_t, float>{hr_, hi, nullptr}, st());
+      precond(hr_, tw);
+      if (pr) probe(tw, hr_, k);
+      hipLaunchKernelGGL(k_b_l4_update, dim3(G), dim3(kBlock), 0, st(), (const half_t *)tw, (const half_t *)ho, ho2,
+                         (const RichSeg *)rs, k, pr ? 1 : 0, (const int32_t *)bs());
+      std::swap(ho, ho2);
+    }
+    hipLaunchKernelGGL((k_b_entry<half_t, float>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)ho, out,
+                       (const float *)iscale->as<float>(), (const int32_t *)bs());
+  }
+
+  // FGMRES(m3) on the fp16 values, fp32 basis, normalize = false
+  void level3(const float *in, float *out) {
+    BLevel<float> &L = L3;
+    F3R_HIP(hipMemcpyAsync(L.Vc(N, 0), in, sizeof(float) * N, hipMemcpyDeviceToDevice, st()));
+    for (int j = 0; j < L.m; ++j) {
+      level4(L.Vc(N, j), L.Zc(N, j));
+      launch_sell<half_t, float, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(),
+                                        (const float *)L.Zc(N, j), EpiStore<float, float>{L.Vc(N, j + 1), nullptr},
+                                        st());
+      arnoldi(L, j, 1, nullptr);
+    }
+    hipLaunchKernelGGL((k_b_combine<float, float, float>), dim3(G), dim3(kBlock), 0, st(), (const float *)L.Zc(N, 0), N,
+                       (const float *)L.H->as<float>(), L.hs(), L.m + 1, (const float *)L.e->as<float>(),
+                       (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m, (const int32_t *)bs(), out, 0);
+  }
+
+  // FGMRES(m2) on the fp32 values, fp32 basis, normalize = false
+  void level2(const double *in, double *out) {
+    BLevel<float> &L = L2;
+    hipLaunchKernelGGL((k_b_cast<double, float>), dim3(G), dim3(kBlock), 0, st(), in, L.Vc(N, 0));
+    for (int j = 0; j < L.m; ++j) {
+      level3(L.Vc(N, j), L.Zc(N, j));
+      launch_sell<float, float, float>(sh->K, sell_args(*sh, false), A32->as<float>(), (const float *)L.Zc(N, j),
+                                       EpiStore<float, float>{L.Vc(N, j + 1), nullptr}, st());
+      arnoldi(L, j, 1, nullptr);
+    }
+    hipLaunchKernelGGL((k_b_combine<float, float, double>), dim3(G), dim3(kBlock), 0, st(),
+                       (const float *)L.Zc(N, 0), N, (const float *)L.H->as<float>(), L.hs(), L.m + 1,
+                       (const float *)L.e->as<float>(), (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m,
+                       (const int32_t *)bs(), out, 0);
+  }
+
+  std::vector<double> read_status() {
+    std::vector<double> h(nseg);
+    F3R_HIP(hipMemcpyAsync(h.data(), status->p, sizeof(double) * nseg, hipMemcpyDeviceToHost, st()));
+    F3R_HIP(hipStreamSynchronize(st()));
+    return h;
+  }
+
+  // Restarted<double>(FGMRES<double>(A, L2, {m1, true}), {restart}) per segment
+  // |e| / ||b|| against eps; a zero right-hand side is met only by a zero residual
+  static double rel(double e, double nb) { return nb > 0 ? e / nb : (e == 0 ? 0.0 : HUGE_VAL); }
+
+  void solve(const std::vector<double> &nrm_b, std::vector<f3r_result> &res) {
+    BLevel<double> &L = L1;
+    const int m1 = L.m;
+    double *V = L.V->as<double>(), *Hd = L.H->as<double>();
+    std::vector<char> solved(nseg, 0);
+    std::vector<int32_t> iters(nseg, 0), cycles(nseg, 0);
+    std::vector<double> last(nseg, 0.0);
+    cnt = p.c;
+    calls = 0;
+    hipLaunchKernelGGL(k_b_rich_reset, dim3((nseg + 63) / 64), dim3(64), 0, st(), nseg, rich->as<RichSeg>());
+    for (int cyc = 0; cyc < p.restart; ++cyc) {
+      std::vector<char> active(nseg, 0);
+      for (int s = 0; s < nseg; ++s) active[s] = !solved[s];
+      // r = b - A x, e_0 = ||r||, V_0 = r / ||r||
+      launch_sell<double, double, double>(sh->K, sell_args(*sh, false), A64->as<double>(),
+                                          (const double *)x_int->as<double>(),
+                                          EpiResid<double, double, double>{V, b_int->as<double>(), nullptr}, st());
+      hipLaunchKernelGGL(k_b_norm2<double>, dim3(G), dim3(kBlock), 0, st(), (const double *)V, partial->as<double>());
+      // no rotation (j = -1): e_0 = ||r|| is written where H(j + 1) would go
+      hipLaunchKernelGGL(k_b_norm_finish<double>, dim3(nseg), dim3(kBlock), 0, st(),
+                         (const double *)partial->as<double>(), (const int32_t *)sb0->as<int32_t>(),
+                         (const int32_t *)snb->as<int32_t>(), L.e->as<double>(), (int64_t)(m1 + 1),
+                         (double *)nullptr, (double *)nullptr, (int64_t)0, (double *)nullptr, (int64_t)0, -1, 0,
+                         L.inv->as<double>(), status->as<double>());
+      hipLaunchKernelGGL((k_b_vscale<double, double>), dim3(G), dim3(kBlock), 0, st(), V,
+                         (const double *)L.inv->as<double>(), (const int32_t *)bs());
+      std::vector<double> e = read_status();
+      std::vector<int32_t> jdone(nseg, 0);
+      bool any = false;
+      for (int s = 0; s < nseg; ++s) {
+        if (!active[s]) continue;
+        last[s] = e[s];
+        ++cycles[s];
+        if (rel(e[s], nrm_b[s]) < p.eps) {
+          solved[s] = 1;
+          active[s] = 0;
+        } else {
+          any = true;
+        }
+      }
+      if (!any) break;
+      int j = 0;
+      for (; j < m1; ++j) {
+        level2(V + (int64_t)j * N, L.Zc(N, j));
+        launch_sell<double, double, double>(sh->K, sell_args(*sh, false), A64->as<double>(),
+                                            (const double *)L.Zc(N, j),
+                                            EpiStore<double, double>{V + (int64_t)(j + 1) * N, nullptr}, st());
+        arnoldi(L, j, 0, status->as<double>());
+        std::vector<double> st_h = read_status();
+        bool left = false;
+        for (int s = 0; s < nseg; ++s) {
+          if (!active[s]) continue;
+          ++iters[s];
+          jdone[s] = j + 1;
+          last[s] = st_h[s];
+          if (rel(st_h[s], nrm_b[s]) < p.eps) {
+            solved[s] = 1;
+            active[s] = 0;
+          } else {
+            left = true;
+          }
+        }
+        if (!left) break;
+      }
+      // x += Z y(j_s) for every segment that stepped in this cycle
+      F3R_HIP(hipMemcpyAsync(jseg->p, jdone.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, st()));
+      hipLaunchKernelGGL((k_b_combine<double, double, double>), dim3(G), dim3(kBlock), 0, st(),
+                         (const double *)L.Zc(N, 0), N, (const double *)Hd, L.hs(), m1 + 1,
+                         (const double *)L.e->as<double>(), (int64_t)(m1 + 1), (const int32_t *)jseg->as<int32_t>(),
+                         0, (const int32_t *)bs(), x_int->as<double>(), 1);
+      F3R_HIP(hipStreamSynchronize(st()));  // jdone is host memory of this scope
+    }
+    F3R_HIP(hipGetLastError());
+    res.resize(nseg);
+    for (int s = 0; s < nseg; ++s) {
+      res[s] = f3r_result{};
+      res[s].is_solved = solved[s];
+      res[s].res_iter = iters[s];
+      res[s].res_nrm2 = last[s];
+      res[s].nrm_b = nrm_b[s];
+      res[s].precond_calls = calls;
+      res[s].outer_cycles = cycles[s];
+    }
+  }
+};
+
+BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::shared_ptr<Mat>> &A,
+                          const std::vector<std::shared_ptr<Prec>> &P, const f3r_params &p) {
+  F3R_CHECK(!A.empty() && A.size() == P.size(), F3R_ERR_ARG, "batch: one matrix and one preconditioner per segment");
+  F3R_CHECK(p.m1 >= 1 && p.m1 <= 120 && p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16 && p.m4 >= 1 &&
+                p.m4 <= 16 && p.c >= 1 && p.restart >= 1,
+            F3R_ERR_ARG, "batch: m1 in [1, 120], m2, m3, m4 in [1, 16], c, restart >= 1");
+  auto B = std::make_unique<BatchSolver>();
+  B->ctx = ctx;
+  B->p = p;
+  B->nseg = (int)A.size();
+  const int nseg = B->nseg;
+  B->off.assign(nseg + 1, 0);
+  B->nrow.resize(nseg);
+  std::vector<const HostCSR *> pa(nseg), pw(nseg), pz(nseg);
+  std::vector<float> sc(nseg), isc(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    F3R_CHECK(A[s] && A[s]->host && A[s]->format == 0, F3R_ERR_ARG, "batch: CSR matrices");
+    const HostCSR &h = *A[s]->host;
+    F3R_CHECK(h.n > 0 && h.n == h.ncols, F3R_ERR_ARG, "batch: square non-empty matrices");
+    F3R_CHECK(P[s] && P[s]->kind == 0 && P[s]->host.f[0].n == h.n, F3R_ERR_ARG,
+              "batch: an SDAINV preconditioner of each segment's matrix");
+    B->nrow[s] = h.n;
+    B->off[s + 1] = B->off[s] + (h.n + kTile - 1) / kTile * kTile;
+    pa[s] = &h;
+    pw[s] = &P[s]->host.f[1];
+    pz[s] = &P[s]->host.f[0];
+    const double rs = p.range_scale > 0 ? p.range_scale
+                                        : std::ldexp(1.0, (int)std::floor(std::log2(std::max(1.0, std::sqrt((double)h.n) / 4.0))));
+    sc[s] = (float)rs;
+    isc[s] = (float)(1.0 / rs);
+  }
+  const int64_t N = B->off[nseg];
+  F3R_CHECK(N / kTile < INT32_MAX, F3R_ERR_RANGE, "batch: too many rows");
+  B->N = N;
+  B->G = (int)(N / kTile);
+  Context *c = ctx.get();
+  // block -> segment, segment -> first block and block count
+  std::vector<int32_t> bseg(B->G), sb0(nseg), snb(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    sb0[s] = (int32_t)(B->off[s] / kTile);
+    snb[s] = (int32_t)((B->off[s + 1] - B->off[s]) / kTile);
+    for (int32_t b = 0; b < snb[s]; ++b) bseg[sb0[s] + b] = s;
+  }
+  auto up32 = [&](const std::vector<int32_t> &v) {
+    DevPtr d = dev_alloc(sizeof(int32_t) * std::max<size_t>(1, v.size()));
+    F3R_HIP(hipMemcpyAsync(d->p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice, c->stream));
+    return d;
+  };
+  auto upf = [&](const std::vector<float> &v) {
+    DevPtr d = dev_alloc(sizeof(float) * std::max<size_t>(1, v.size()));
+    F3R_HIP(hipMemcpyAsync(d->p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice, c->stream));
+    return d;
+  };
+  B->bseg = up32(bseg);
+  B->sb0 = up32(sb0);
+  B->snb = up32(snb);
+  B->scale = upf(sc);
+  B->iscale = upf(isc);
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  // the block-diagonal matrices; one row order (sigma = 1) keeps segments contiguous
+  const int C = p.C > 0 ? p.C : 64;
+  const int K = p.K > 0 ? p.K : 4;
+  const bool offsets = p.index != 0;
+  {
+    HostCSR big = concat(pa, B->off, N);
+    // fp16 copy: K = 8 once rows average 12 entries (a 27-point row then
+    // takes 4 slot groups instead of 7; a 7-point row takes 1 either way)
+    const int K16 = p.K > 0 ? p.K : (big.nnz() >= (int64_t)12 * N ? 8 : 4);
+    HostSell hs = build_sell(big, C, 1, K, nullptr, false, offsets);
+    B->sh = upload_sell_shared(c, hs, false);
+    B->A64 = upload_sell_values(c, hs, F3R_F64);
+    B->A32 = upload_sell_values(c, hs, F3R_F32);
+    if (K16 == K) {
+      B->sh16 = B->sh;
+      B->A16 = upload_sell_values(c, hs, F3R_F16);
+    } else {
+      HostSell h16 = build_sell(big, C, 1, K16, nullptr, false, offsets);
+      B->sh16 = upload_sell_shared(c, h16, false);
+      B->A16 = upload_sell_values(c, h16, F3R_F16);
+    }
+  }
+  // the SDAINV factors: diagonal-only rows as a vector, the rest as CSR rows
+  auto factor = [&](const std::vector<const HostCSR *> &parts) {
+    HostCSR M = concat(parts, B->off, N);
+    std::vector<double> d(N, 0.0);
+    std::vector<int32_t> rows, rp(1, 0), cl;
+    std::vector<double> vl;
+    for (int32_t i = 0; i < M.n; ++i) {
+      const int32_t b = M.rptr[i], e = M.rptr[i + 1];
+      if (e - b == 1 && M.idx[b] == i) {
+        d[i] = M.val[b];
+      } else if (e > b) {
+        rows.push_back(i);
+        for (int32_t k = b; k < e; ++k) {
+          cl.push_back(M.idx[k]);
+          vl.push_back(M.val[k]);
+        }
+        rp.push_back((int32_t)cl.size());
+      }
+    }
+    Factor16 F;
+    F.dval = upload_values(c, d, F3R_F16);
+    F.nr = (int32_t)rows.size();
+    F.rows = up32(rows);
+    F.rptr = up32(rp);
+    F.col = up32(cl);
+    F.val = upload_values(c, vl, F3R_F16);
+    F3R_HIP(hipStreamSynchronize(c->stream));  // the host arrays above are released here
+    return F;
+  };
+  B->W = factor(pw);
+  B->Z = factor(pz);
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  // work space
+  B->alloc_level(B->L1, p.m1);
+  B->alloc_level(B->L2, p.m2);
+  B->alloc_level(B->L3, p.m3);
+  auto hv = [&]() { return dev_alloc(sizeof(half_t) * N); };
+  B->hin = hv();
+  B->hout = hv();
+  B->hout2 = hv();
+  B->hr = hv();
+  B->tmp = hv();
+  B->tt = hv();
+  B->t2 = hv();
+  B->b_int = dev_alloc(sizeof(double) * N);
+  B->x_int = dev_alloc(sizeof(double) * N);
+  B->partial = dev_alloc(sizeof(double) * (int64_t)B->G * (p.m1 + 1 + 16));
+  B->status = dev_alloc(sizeof(double) * nseg);
+  B->jseg = dev_alloc(sizeof(int32_t) * nseg);
+  B->rich = dev_alloc(sizeof(RichSeg) * nseg);
+  F3R_HIP(hipStreamSynchronize(c->stream));
+  return B.release();
+}
+
+}  // namespace f3r
+
+// ================================================================ C ABI ===
+using namespace f3r;
+
+struct f3r_batch_s {
+  BatchSolver *b;
+};
+
+extern "C" {
+
+int f3r_batch_create(f3r_ctx ctx, int32_t nseg, const f3r_mat *A, const f3r_prec *P, const f3r_params *params,
+                     f3r_batch *out) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && A && P && params && out && nseg > 0, F3R_ERR_ARG, "batch_create: null argument");
+  *out = nullptr;
+  auto c = ctx_of(ctx);
+  std::vector<std::shared_ptr<Mat>> am(nseg);
+  std::vector<std::shared_ptr<Prec>> pp(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    F3R_CHECK(A[s] && P[s], F3R_ERR_ARG, "batch_create: null matrix or preconditioner");
+    am[s] = mat_of(A[s]);
+    pp[s] = prec_of(P[s]);
+  }
+  BatchSolver *b = batch_create(c, am, pp, *params);
+  *out = new f3r_batch_s{b};
+  F3R_API_END
+}
+
+int f3r_batch_solve(f3r_batch B, const f3r_vec *b, f3r_vec *x, f3r_result *res) {
+  F3R_API_BEGIN
+  F3R_CHECK(B && b && x && res, F3R_ERR_ARG, "batch_solve: null argument");
+  BatchSolver *S = B->b;
+  hipStream_t st = S->ctx->stream;
+  const int nseg = S->nseg;
+  std::vector<double> nrm_b(nseg);
+  F3R_HIP(hipMemsetAsync(S->b_int->p, 0, sizeof(double) * S->N, st));
+  F3R_HIP(hipMemsetAsync(S->x_int->p, 0, sizeof(double) * S->N, st));
+  for (int s = 0; s < nseg; ++s) {
+    Vec *bv = vec_of(b[s]), *xv = vec_of(x[s]);
+    F3R_CHECK(bv && xv && bv->dtype == F3R_F64 && xv->dtype == F3R_F64 && bv->n == S->nrow[s] && xv->n == S->nrow[s],
+              F3R_ERR_ARG, "batch_solve: b and x of each segment: fp64 vectors of its matrix size");
+    F3R_HIP(hipMemcpyAsync(S->b_int->as<double>() + S->off[s], bv->ptr(), sizeof(double) * S->nrow[s],
+                           hipMemcpyDeviceToDevice, st));
+    F3R_HIP(hipMemcpyAsync(S->x_int->as<double>() + S->off[s], xv->ptr(), sizeof(double) * S->nrow[s],
+                           hipMemcpyDeviceToDevice, st));
+    nrm_b[s] = std::sqrt(dot_raw(S->ctx.get(), F3R_F64, bv->ptr(), F3R_F64, bv->ptr(), S->nrow[s]));
+  }
+  F3R_HIP(hipStreamSynchronize(st));
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<f3r_result> r;
+  S->solve(nrm_b, r);
+  F3R_HIP(hipStreamSynchronize(st));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int s = 0; s < nseg; ++s) {
+    Vec *xv = vec_of(x[s]);
+    F3R_HIP(hipMemcpyAsync(xv->ptr(), S->x_int->as<double>() + S->off[s], sizeof(double) * S->nrow[s],
+                           hipMemcpyDeviceToDevice, st));
+    r[s].solve_s = secs;
+    res[s] = r[s];
+  }
+  F3R_HIP(hipStreamSynchronize(st));
+  F3R_API_END
+}
+
+int f3r_batch_info(f3r_batch B, int64_t *rows, int32_t *segments) {
+  F3R_API_BEGIN
+  F3R_CHECK(B && rows && segments, F3R_ERR_ARG, "batch_info: null argument");
+  *rows = B->b->N;
+  *segments = B->b->nseg;
+  F3R_API_END
+}
+
+int f3r_batch_destroy(f3r_batch B) {
+  F3R_API_BEGIN
+  if (B) {
+    (void)hipStreamSynchronize(B->b->ctx->stream);
+    delete B->b;
+    delete B;
+  }
+  F3R_API_END
+}
+
+}  // extern "C"

@@ -270,6 +270,30 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
 int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms,
                            double *bytes);
 int f3r_solver_destroy(f3r_solver S);
+
+/* ----------------------------------------------------- batched solves ---- */
+/* Many independent systems (each right-hand side its own segment) solved in
+ * lockstep: every launch covers all segments.  Segment s is A[s] with the
+ * SDAINV preconditioner P[s] (fp16, any context: only its host factors are
+ * read); a system solved for k right-hand sides appears k times.  The
+ * segments are laid out as one block-diagonal matrix with each segment
+ * padded to whole 2048-row tiles; all reductions, Hessenberg/Givens updates
+ * and Richardson weights are per segment, and the host reads one array of
+ * residual norms per outer step.  The nest and its per-segment semantics
+ * are f3r_solver_create's HALF build (params: m1..m4, c, restart, eps, C, K,
+ * index, range_scale; 0 = the per-segment automatic range factor).  The
+ * reference solves one system per run (work/src/f3r-gpu.cpp:44-288); this
+ * has no counterpart there. */
+typedef struct f3r_batch_s *f3r_batch;
+int f3r_batch_create(f3r_ctx ctx, int32_t nseg, const f3r_mat *A, const f3r_prec *P, const f3r_params *params,
+                     f3r_batch *B);
+/* b[s], x[s]: fp64 vectors of segment s's size (x is the initial guess and
+ * the solution); res[s]: per-segment result, solve_s = the whole batch */
+int f3r_batch_solve(f3r_batch B, const f3r_vec *b, f3r_vec *x, f3r_result *res);
+/* padded rows of all segments, and the segment count */
+int f3r_batch_info(f3r_batch B, int64_t *rows, int32_t *segments);
+int f3r_batch_destroy(f3r_batch B);
+
 /* record copies of the innermost level's input vectors during later solves:
  * every `every`-th call, up to `max_vectors` (0 stops and frees).  Level 2
  * runs without HIP graphs while vectors are being recorded.  These are the

@@ -26,3 +26,91 @@ def test_autotune_picks_the_fastest():
     assert best["ms"] == min(t[0] for t in table) and best["ms"] > 0
     assert len(table) == 3 * 2 * 2   # every (C, sigma, K) with sigma = 1 or sigma >= C
     ctx.close()
+
+
+# ------------------------------------------------ lockstep batch (f3r_batch) --
+def _family(F, count, seed):
+    from f3r_amd import batch
+    fam = batch.family(seed, count)
+    out = []
+    for (i, npts, nx, delta) in fam:
+        nx = min(nx, 20)  # small systems: the test is about the per-segment bookkeeping
+        out.append((i, npts, nx, delta, batch.make_system(npts, nx, delta)))
+    return out
+
+
+def _true_res(A, x, b):
+    import numpy as np
+    import scipy.sparse as sp
+    M = sp.csr_matrix((A.val, A.idx, A.rptr), shape=(A.n, A.n))
+    return np.linalg.norm(b - M @ x) / np.linalg.norm(b)
+
+
+def test_lockstep_batch_matches_single_solves():
+    """every segment: true residual <= 1e-8, outer steps within +-1 of its own
+    single-system solve and +-10% (at least +-1) of the oracle's"""
+    import numpy as np
+    import f3r_amd as F
+    import oracle_lib as O
+    ctx = F.Context(0)
+    fam = _family(F, 6, seed=3)
+    As, Ps, bs = [], [], []
+    for (i, npts, nx, delta, A) in fam:
+        P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+        for r in range(2):
+            As.append(A)
+            Ps.append(P)
+            bs.append(F.uniform(A.n, -1.0, 1.0, seed=1000 * i + r))
+    B = F.Batch(ctx, As, Ps)
+    rows, segs = B.info()
+    assert segs == len(As) and rows % 2048 == 0 and rows >= sum(A.n for A in As)
+    bv = [F.vector.from_numpy(ctx, b) for b in bs]
+    xv = [F.vector(ctx, "f64", A.n).fill(0.0) for A in As]
+    res = B.solve(bv, xv)
+    for s, (A, P, b) in enumerate(zip(As, Ps, bs)):
+        x = xv[s].download()
+        print(f"segment {s}: n={A.n} outer={res[s].res_iter} solved={res[s].is_solved}", flush=True)
+        assert res[s].is_solved and _true_res(A, x, b) <= 1e-8
+        S = F.F3R(ctx, A, P)
+        x1 = F.vector(ctx, "f64", A.n).fill(0.0)
+        r1 = S.solve(bv[s], x1)
+        assert abs(r1.res_iter - res[s].res_iter) <= 1, (s, r1.res_iter, res[s].res_iter)
+        if s % 4 == 0:
+            scale = float(2.0 ** np.floor(np.log2(max(1.0, np.sqrt(A.n) / 4.0))))
+            _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, range_scale=scale)
+            assert abs(ro.res_iter - res[s].res_iter) <= max(1, round(0.1 * ro.res_iter))
+    ctx.close()
+
+
+def test_lockstep_batch_segments_are_independent():
+    """a zero right-hand side and a solved start finish at step 0 without
+    disturbing the others; a segment's result does not depend on its company"""
+    import numpy as np
+    import f3r_amd as F
+    ctx = F.Context(0)
+    A1 = F.CSR.dominant(7, 12, margin=0.7)
+    F.algorithm.scaling(A1)
+    A2 = F.CSR.dominant(27, 9, margin=1.2)
+    F.algorithm.scaling(A2)
+    P1 = F.SDAINV(A1, 0.1, 1.0, "f16", ctx=ctx)
+    P2 = F.SDAINV(A2, 0.1, 1.0, "f16", ctx=ctx)
+    b1 = F.uniform(A1.n, -1.0, 1.0, seed=5)
+    b2 = F.uniform(A2.n, -1.0, 1.0, seed=6)
+    # alone
+    B = F.Batch(ctx, [A1], [P1])
+    x = F.vector(ctx, "f64", A1.n).fill(0.0)
+    alone = B.solve([F.vector.from_numpy(ctx, b1)], [x])[0]
+    x_alone = x.download()
+    # in company: A2 with b2, A1 with 0, A1 with b1
+    B3 = F.Batch(ctx, [A2, A1, A1], [P2, P1, P1])
+    bv = [F.vector.from_numpy(ctx, b2), F.vector(ctx, "f64", A1.n).fill(0.0), F.vector.from_numpy(ctx, b1)]
+    xv = [F.vector(ctx, "f64", A2.n).fill(0.0), F.vector(ctx, "f64", A1.n).fill(0.0),
+          F.vector(ctx, "f64", A1.n).fill(0.0)]
+    res = B3.solve(bv, xv)
+    assert res[1].is_solved and res[1].res_iter == 0 and not np.any(xv[1].download())
+    assert res[2].res_iter == alone.res_iter and np.array_equal(xv[2].download(), x_alone)
+    assert res[0].is_solved and _true_res(A2, xv[0].download(), b2) <= 1e-8
+    # warm start from the solution: solved at step 0
+    res2 = B3.solve(bv, xv)
+    assert all(r.is_solved and r.res_iter == 0 for r in res2)
+    ctx.close()
