@@ -151,6 +151,102 @@ __global__ __launch_bounds__(1024) void k_bj_trsv_u(const int32_t *__restrict__ 
   }
 }
 
+// The same two solves fused in one launch with the block's vector in LDS:
+// xs = in (the block's rows), the L levels in place, then the U levels in
+// place, then out = xs.  Each row's entries are stored level by level as
+// structure-of-arrays records (local column, value, in the order the solves
+// above visit them; the U record also carries 1/u_ii), so the next level's
+// records are loaded before the barrier that ends the current one and only
+// LDS reads remain on the path from level to level.  Same arithmetic as
+// k_trsv_l / k_trsv_u.
+template <class S> struct BjRec {
+  const int32_t *start, *lptr;  // per block: first level pointer; level -> slot range
+  const int32_t *row, *cnt, *col;  // per slot: local row, entries, columns (R x slots)
+  const S *val, *dinv;             // values (R x slots), 1/u_ii (U only)
+  int32_t slots;
+};
+template <class S, int R> struct BjSlot {
+  int32_t row, cnt;
+  int32_t col[R];
+  S val[R];
+  S dinv;
+};
+// Loads the record of this thread's slot in level q (row -1: no slot, or q
+// past the block's last level).  Every call issues the same loads (indices
+// clamped into range; entries past the row's count are zero padding), so the
+// compiler can count them and wait for exactly the oldest ones.
+typedef const __attribute__((address_space(4))) int32_t *bj_const_i32;  // uniform read-only: scalar loads
+
+template <class S, int R>
+__device__ __forceinline__ void bj_load(const BjRec<S> &r, int32_t q, int32_t q1, bool upper, BjSlot<S, R> &sl) {
+  const bool lvl = q + 1 < q1;
+  const int32_t qq = __builtin_amdgcn_readfirstlane(lvl ? q : q1 - 2);
+  // the level pointers by scalar loads: the vector-load counter then holds
+  // only the records, and waiting for the oldest does not wait for the rest
+  const bj_const_i32 lp = (bj_const_i32)r.lptr;
+  const int32_t t0 = lp[qq] + (int32_t)threadIdx.x, te = lp[qq + 1];
+  const bool on = lvl && t0 < te;
+  const int32_t t = on ? t0 : te - 1;
+  sl.row = on ? r.row[t] : -1;
+  sl.cnt = r.cnt[t];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    sl.col[k] = r.col[(int64_t)k * r.slots + t];
+    sl.val[k] = r.val[(int64_t)k * r.slots + t];
+  }
+  sl.dinv = upper ? r.dinv[t] : S(0);
+}
+// Barrier for LDS traffic only: this wave's LDS operations complete, then
+// s_barrier.  __syncthreads() would also wait for every outstanding global
+// load (its release fence), i.e. for the prefetched records below.  The
+// memory clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// The records of the next D levels stay in flight: level q + D is requested
+// as level q finishes, so its latency overlaps D levels of work.  The loop
+// is unrolled by D so the ring is indexed statically (registers).
+template <class S, int R>
+__device__ __forceinline__ void bj_levels(const BjRec<S> &r, bool upper, S *xs) {
+  using A = typename Acc<S>::type;
+  constexpr int D = R <= 4 ? 4 : R <= 8 ? 3 : 2;
+  const bj_const_i32 st = (bj_const_i32)r.start;
+  const int32_t q0 = st[blockIdx.x], q1 = st[blockIdx.x + 1];
+  BjSlot<S, R> ring[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) bj_load<S, R>(r, q0 + d, q1, upper, ring[d]);
+  // straight-line steps; the at most D - 1 steps past the last level have
+  // no slots and only meet the barrier (the same count for every thread)
+  for (int32_t q = q0; q + 1 < q1; q += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      BjSlot<S, R> &cur = ring[d];
+      const int32_t row = cur.row;
+      A s = to_acc(xs[row >= 0 ? row : 0]);
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (k < cur.cnt) s = fma_<A>(-to_acc(cur.val[k]), to_acc(xs[cur.col[k]]), s);
+      const S res = upper ? cvt<S>(s * to_acc(cur.dinv)) : cvt<S>(s);
+      bj_load<S, R>(r, q + d + D, q1, upper, cur);
+      if (row >= 0) xs[row] = res;
+      lds_barrier();
+    }
+  }
+}
+
+template <class S, int R>
+__global__ __launch_bounds__(1024) void k_bj_lds(const int32_t *__restrict__ b0, const int32_t *__restrict__ bn,
+                                                 BjRec<S> L, BjRec<S> U, const S *__restrict__ in,
+                                                 S *__restrict__ out) {
+  extern __shared__ unsigned char bj_smem[];
+  S *xs = reinterpret_cast<S *>(bj_smem);
+  const int32_t r0 = b0[blockIdx.x], nr = bn[blockIdx.x];
+  for (int32_t i = threadIdx.x; i < nr; i += blockDim.x) xs[i] = in[r0 + i];
+  __syncthreads();
+  bj_levels<S, R>(L, false, xs);
+  bj_levels<S, R>(U, true, xs);
+  for (int32_t i = threadIdx.x; i < nr; i += blockDim.x) out[r0 + i] = xs[i];
+}
+
 static int grid_for(int64_t n, int cu) {
   int64_t g = (n + kBlock - 1) / kBlock;
   int64_t cap = (int64_t)cu * 16;
@@ -472,6 +568,38 @@ static void group_by_block(const std::vector<int32_t> &rows, const std::vector<i
   bstart[nb] = (int32_t)blptr.size();
 }
 
+// SoA records of a triangular factor in the grouped order `brows` (see
+// k_bj_lds): entries in the visiting order of the solve, columns local to
+// the row's block.  False when a row has more than `rmax` entries.
+static bool bj_records(const HostCSR &M, bool upper, const std::vector<int32_t> &brows, int32_t bsz, int rmax,
+                       std::vector<int32_t> &row, std::vector<int32_t> &cnt, std::vector<int32_t> &col,
+                       std::vector<double> &val) {
+  const int64_t ns = (int64_t)brows.size();
+  row.assign(ns, 0);
+  cnt.assign(ns, 0);
+  col.assign(ns * rmax, 0);
+  val.assign(ns * rmax, 0.0);
+  for (int64_t t = 0; t < ns; ++t) {
+    const int32_t i = brows[t], base = i / bsz * bsz;
+    row[t] = i - base;
+    int c = 0;
+    auto put = [&](int32_t k) {
+      if (c < rmax) {
+        col[c * ns + t] = M.idx[k] - base;
+        val[c * ns + t] = M.val[k];
+      }
+      ++c;
+    };
+    if (upper)
+      for (int32_t k = M.rptr[i + 1] - 1; k > M.rptr[i]; --k) put(k);
+    else
+      for (int32_t k = M.rptr[i]; k < M.rptr[i + 1] - 1; ++k) put(k);
+    if (c > rmax) return false;
+    cnt[t] = c;
+  }
+  return true;
+}
+
 static std::vector<int32_t> levels_upper(const HostCSR &U, std::vector<int32_t> &lptr) {
   const int32_t n = U.n;
   std::vector<int32_t> lev(n, 0);
@@ -584,6 +712,50 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
       P->ub_ptr = upload_i32(c, bl);
       P->ub_start = upload_i32(c, bs);
       P->bj_groups = (int32_t)bs.size() - 1;
+      // the fused LDS kernel: the block's vector in LDS, at most 1024 rows
+      // per level of a block and 16 entries per row
+      int32_t maxrows = 0, maxw = 0, maxe = 0;
+      for (int32_t b = 0; b < P->bj_groups; ++b) maxrows = std::max(maxrows, std::min(n, (b + 1) * bsz) - b * bsz);
+      std::vector<int32_t> lbr, lbl, lbs, ubr, ubl, ubs;
+      group_by_block(lrows, P->l_lptr, n, bsz, lbr, lbl, lbs);
+      group_by_block(urows, P->u_lptr, n, bsz, ubr, ubl, ubs);
+      for (const std::vector<int32_t> *v : {&lbl, &ubl})
+        for (size_t q = 0; q + 1 < v->size(); ++q) maxw = std::max(maxw, (*v)[q + 1] - (*v)[q]);
+      for (int32_t i = 0; i < n; ++i) {
+        maxe = std::max(maxe, L.rptr[i + 1] - L.rptr[i] - 1);
+        maxe = std::max(maxe, U.rptr[i + 1] - U.rptr[i] - 1);
+      }
+      const size_t lds = (size_t)maxrows * dtype_size(P->dtype);
+      if (maxw <= 1024 && maxe <= 16 && lds <= kBjLdsBytes) {
+        const int R = maxe <= 4 ? 4 : maxe <= 8 ? 8 : 16;
+        std::vector<int32_t> row, cnt, col;
+        std::vector<double> val;
+        bj_records(L, false, lbr, bsz, R, row, cnt, col, val);
+        P->lr_row = upload_i32(c, row);
+        P->lr_cnt = upload_i32(c, cnt);
+        P->lr_col = upload_i32(c, col);
+        P->lr_val = upload_values(c, val, P->dtype);
+        bj_records(U, true, ubr, bsz, R, row, cnt, col, val);
+        P->ur_row = upload_i32(c, row);
+        P->ur_cnt = upload_i32(c, cnt);
+        P->ur_col = upload_i32(c, col);
+        P->ur_val = upload_values(c, val, P->dtype);
+        std::vector<double> dv(n);
+        for (int32_t t = 0; t < n; ++t) dv[t] = 1. / U.val[U.rptr[ubr[t]]];
+        P->ur_dinv = upload_values(c, dv, P->dtype);
+        std::vector<int32_t> b0(P->bj_groups), bn(P->bj_groups);
+        for (int32_t b = 0; b < P->bj_groups; ++b) {
+          b0[b] = b * bsz;
+          bn[b] = std::min(n, (b + 1) * bsz) - b * bsz;
+        }
+        P->bj_b0 = upload_i32(c, b0);
+        P->bj_bn = upload_i32(c, bn);
+        P->bj_R = R;
+        P->bj_lds = lds;
+        // one thread per row of the widest level: idle waves would still
+        // load records and meet every barrier
+        P->bj_threads = std::max(64, (maxw + 63) / 64 * 64);
+      }
     }
     P->l_rptr = upload_i32(c, L.rptr);
     P->l_idx = upload_i32(c, L.idx);
@@ -599,6 +771,21 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
 
 template <class S> static void ilu_apply_t(Prec *P, const S *in, S *out, hipStream_t st) {
   S *tmp = P->tmp->as<S>();
+  if (P->bj_R) {
+    const int32_t ns = P->n;
+    BjRec<S> L{P->lb_start->as<int32_t>(), P->lb_ptr->as<int32_t>(), P->lr_row->as<int32_t>(),
+               P->lr_cnt->as<int32_t>(), P->lr_col->as<int32_t>(), P->lr_val->as<S>(), nullptr, ns};
+    BjRec<S> U{P->ub_start->as<int32_t>(), P->ub_ptr->as<int32_t>(), P->ur_row->as<int32_t>(),
+               P->ur_cnt->as<int32_t>(), P->ur_col->as<int32_t>(), P->ur_val->as<S>(), P->ur_dinv->as<S>(), ns};
+    const int32_t *b0 = P->bj_b0->as<int32_t>(), *bn = P->bj_bn->as<int32_t>();
+    const unsigned lds = (unsigned)P->bj_lds;
+    switch (P->bj_R) {
+      case 4: hipLaunchKernelGGL((k_bj_lds<S, 4>), dim3(P->bj_groups), dim3(P->bj_threads), lds, st, b0, bn, L, U, in, out); break;
+      case 8: hipLaunchKernelGGL((k_bj_lds<S, 8>), dim3(P->bj_groups), dim3(P->bj_threads), lds, st, b0, bn, L, U, in, out); break;
+      default: hipLaunchKernelGGL((k_bj_lds<S, 16>), dim3(P->bj_groups), dim3(P->bj_threads), lds, st, b0, bn, L, U, in, out); break;
+    }
+    return;
+  }
   if (P->bj_blocks) {
     hipLaunchKernelGGL(k_bj_trsv_l<S>, dim3(P->bj_groups), dim3(1024), 0, st, (const int32_t *)P->lb_start->as<int32_t>(),
                        (const int32_t *)P->lb_ptr->as<int32_t>(), (const int32_t *)P->lb_rows->as<int32_t>(),

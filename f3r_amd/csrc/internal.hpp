@@ -29,7 +29,13 @@ struct Prec {
   // block, one workgroup per block (bj_groups of them)
   int32_t bj_blocks = 0, bj_groups = 0;
   DevPtr lb_rows, lb_ptr, lb_start, ub_rows, ub_ptr, ub_start;
+  // fused L+U solve with the block's vector in LDS (bj_R entries per record; 0: not used)
+  int32_t bj_R = 0, bj_threads = 1024;
+  size_t bj_lds = 0;
+  DevPtr lr_row, lr_cnt, lr_col, lr_val, ur_row, ur_cnt, ur_col, ur_val, ur_dinv, bj_b0, bj_bn;
 };
+// LDS the fused block-Jacobi kernel may take for a block's vector
+constexpr size_t kBjLdsBytes = 64 * 1024;
 // fewer diagonal blocks than this: one launch per level instead (a block
 // per workgroup would leave most of the GPU idle)
 constexpr int32_t kBjBlockKernel = 32;
