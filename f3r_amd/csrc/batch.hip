@@ -319,6 +319,68 @@ __global__ __launch_bounds__(kBlock) void k_b_l4_update(const half_t *__restrict
   }
 }
 
+// SpMV of a block-diagonal matrix stored once per system, applied to the NR
+// copies (right-hand sides) of each system's vectors: copy c of the rows of
+// chunk s lies at vector row = matrix row + shift[s] + c * stride[s].  One
+// lane takes a matrix row and keeps NR sums, so each value and column index
+// is read once for all copies; every sum runs in slot order, as in k_sell.
+// Epilogue: y = acc (b null) or y = b - acc, at the vector rows.
+template <class V, class X, class ACC, int K, int NR, class Y>
+__global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restrict__ val, const X *__restrict__ x,
+                                                    const int32_t *__restrict__ cshift,
+                                                    const int32_t *__restrict__ cstride, Y *__restrict__ y,
+                                                    const Y *__restrict__ b) {
+  typedef const __attribute__((address_space(4))) int32_t *const_i32;
+  for (int r = xcd_block() * kBlock + threadIdx.x; r < a.n; r += gridDim.x * kBlock) {
+    const int s = r >> a.log2C;
+    const int j = r - (s << a.log2C);
+    const int C = 1 << a.log2C;
+    const int rem = a.n - (s << a.log2C);
+    const int tc = rem < C ? rem : C;
+    const int su = __builtin_amdgcn_readfirstlane(s);  // C >= 64: one chunk per wavefront
+    const int groups = a.cwidth[su];
+    const int32_t shift = cshift[su], stride = cstride[su];
+    int64_t pos = (int64_t)a.cptr[su] + (int64_t)j * K;
+    const int64_t step = (int64_t)tc * K;
+    ACC acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0;
+    const int64_t vr = (int64_t)r + shift;
+    const int32_t start = a.iptr ? a.iptr[su] : 0;
+    if (a.iptr && start < 0) {
+      const const_i32 off = (const_i32)(a.coff + (-start - 1));
+      for (int g = 0; g < groups; ++g, pos += step) {
+        const Pack<V, K> pv = load_pack<V, K>(val + pos);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int32_t o = off[g * K + k];
+#pragma unroll
+          for (int c = 0; c < NR; ++c)
+            acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[vr + (int64_t)c * stride + o]), acc[c]);
+        }
+      }
+    } else {
+      const int32_t *__restrict__ ip = a.idx + (a.iptr ? (int64_t)start : (int64_t)a.cptr[su]) + (int64_t)j * K;
+      for (int g = 0; g < groups; ++g, pos += step, ip += step) {
+        const Pack<V, K> pv = load_pack<V, K>(val + pos);
+        const Pack<int32_t, K> pi = load_pack<int32_t, K>(ip);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int64_t col = (int64_t)pi.v[k] + shift;
+#pragma unroll
+          for (int c = 0; c < NR; ++c)
+            acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[col + (int64_t)c * stride]), acc[c]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      const int64_t o = vr + (int64_t)c * stride;
+      y[o] = b ? cvt<Y>((ACC)to_acc(b[o]) - acc[c]) : cvt<Y>(acc[c]);
+    }
+  }
+}
+
 // An SDAINV factor split by row kind: a row whose only entry is its diagonal
 // is an elementwise product (dval), any other row is a CSR row summed in
 // column order with fp32 accumulation, as the SELL SpMV sums it.
@@ -392,6 +454,10 @@ struct BatchSolver {
   DevPtr bseg, sb0, snb, scale, iscale;
   std::shared_ptr<SellShared> sh, sh16;
   DevPtr A64, A32, A16;
+  // NR > 1: each system's matrix is stored once for its NR right-hand sides
+  // (chunk -> vector-row shift and copy stride, per layout)
+  int NR = 1;
+  DevPtr csh, cst, csh16, cst16;
   Factor16 W, Z;
   DevPtr partial, status, jseg, rich;
   DevPtr b_int, x_int, r_int;
@@ -403,6 +469,39 @@ struct BatchSolver {
 
   hipStream_t st() const { return ctx->stream; }
   int32_t *bs() const { return bseg->as<int32_t>(); }
+
+  // y = A x (b null) or y = b - A x over all segments, A the fp64/fp32 copy
+  // (layout sh) or the fp16 one (sh16)
+  template <class V, class X, class Y, class ACC>
+  void spmv(const V *val, bool half_layout, const X *x, Y *y, const Y *b) {
+    const SellShared &L = half_layout ? *sh16 : *sh;
+    const SellArgs a = sell_args(L, false);
+    if (NR == 1) {
+      if (b)
+        launch_sell<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
+      else
+        launch_sell<V, X, ACC>(L.K, a, val, x, EpiStore<Y, ACC>{y, nullptr}, st());
+      return;
+    }
+    const int32_t *shf = (half_layout ? csh16 : csh)->as<int32_t>(), *str = (half_layout ? cst16 : cst)->as<int32_t>();
+    const dim3 g((a.n + kBlock - 1) / kBlock);
+    auto go = [&](auto kt, auto nt) {
+      constexpr int K = decltype(kt)::value, R = decltype(nt)::value;
+      hipLaunchKernelGGL((k_sell_mr<V, X, ACC, K, R, Y>), g, dim3(kBlock), 0, st(), a, val, x, shf, str, y, b);
+    };
+    auto byk = [&](auto nt) {
+      switch (L.K) {
+        case 1: go(std::integral_constant<int, 1>(), nt); break;
+        case 2: go(std::integral_constant<int, 2>(), nt); break;
+        case 4: go(std::integral_constant<int, 4>(), nt); break;
+        default: go(std::integral_constant<int, 8>(), nt); break;
+      }
+    };
+    if (NR == 2)
+      byk(std::integral_constant<int, 2>());
+    else
+      byk(std::integral_constant<int, 4>());
+  }
 
   template <class T> void alloc_level(BLevel<T> &L, int m) {
     L.m = m;
@@ -459,8 +558,7 @@ struct BatchSolver {
 
   void probe(const half_t *x, const half_t *r, int k) {
     half_t *t = tt->as<half_t>();
-    launch_sell<half_t, half_t, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(), x,
-                                       EpiStore<half_t, float>{t, nullptr}, st());
+    spmv<half_t, half_t, half_t, float>(A16->as<half_t>(), true, x, t, nullptr);
     float *part = partial->as<float>();
     hipLaunchKernelGGL(k_b_probe_dots, dim3(G), dim3(kBlock), 0, st(), (const half_t *)t, r, part);
     hipLaunchKernelGGL(k_b_probe_finish, dim3(nseg), dim3(kBlock), 0, st(), (const float *)part, G,
@@ -483,8 +581,7 @@ struct BatchSolver {
     hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, pr ? 1 : 0,
                        (const int32_t *)bs());
     for (int k = 1; k < p.m4; ++k) {
-      launch_sell<half_t, half_t, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(), (const half_t *)ho,
-                                         EpiResid<half_t, half_t, float>{hr_, hi, nullptr}, st());
+      spmv<half_t, half_t, half_t, float>(A16->as<half_t>(), true, (const half_t *)ho, hr_, (const half_t *)hi);
       precond(hr_, tw);
       if (pr) probe(tw, hr_, k);
       hipLaunchKernelGGL(k_b_l4_update, dim3(G), dim3(kBlock), 0, st(), (const half_t *)tw, (const half_t *)ho, ho2,
@@ -501,9 +598,7 @@ struct BatchSolver {
     F3R_HIP(hipMemcpyAsync(L.Vc(N, 0), in, sizeof(float) * N, hipMemcpyDeviceToDevice, st()));
     for (int j = 0; j < L.m; ++j) {
       level4(L.Vc(N, j), L.Zc(N, j));
-      launch_sell<half_t, float, float>(sh16->K, sell_args(*sh16, false), A16->as<half_t>(),
-                                        (const float *)L.Zc(N, j), EpiStore<float, float>{L.Vc(N, j + 1), nullptr},
-                                        st());
+      spmv<half_t, float, float, float>(A16->as<half_t>(), true, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
       arnoldi(L, j, 1, nullptr);
     }
     hipLaunchKernelGGL((k_b_combine<float, float, float>), dim3(G), dim3(kBlock), 0, st(), (const float *)L.Zc(N, 0), N,
@@ -517,8 +612,7 @@ struct BatchSolver {
     hipLaunchKernelGGL((k_b_cast<double, float>), dim3(G), dim3(kBlock), 0, st(), in, L.Vc(N, 0));
     for (int j = 0; j < L.m; ++j) {
       level3(L.Vc(N, j), L.Zc(N, j));
-      launch_sell<float, float, float>(sh->K, sell_args(*sh, false), A32->as<float>(), (const float *)L.Zc(N, j),
-                                       EpiStore<float, float>{L.Vc(N, j + 1), nullptr}, st());
+      spmv<float, float, float, float>(A32->as<float>(), false, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
       arnoldi(L, j, 1, nullptr);
     }
     hipLaunchKernelGGL((k_b_combine<float, float, double>), dim3(G), dim3(kBlock), 0, st(),
@@ -552,9 +646,8 @@ struct BatchSolver {
       std::vector<char> active(nseg, 0);
       for (int s = 0; s < nseg; ++s) active[s] = !solved[s];
       // r = b - A x, e_0 = ||r||, V_0 = r / ||r||
-      launch_sell<double, double, double>(sh->K, sell_args(*sh, false), A64->as<double>(),
-                                          (const double *)x_int->as<double>(),
-                                          EpiResid<double, double, double>{V, b_int->as<double>(), nullptr}, st());
+      spmv<double, double, double, double>(A64->as<double>(), false, (const double *)x_int->as<double>(), V,
+                                           (const double *)b_int->as<double>());
       hipLaunchKernelGGL(k_b_norm2<double>, dim3(G), dim3(kBlock), 0, st(), (const double *)V, partial->as<double>());
       // no rotation (j = -1): e_0 = ||r|| is written where H(j + 1) would go
       hipLaunchKernelGGL(k_b_norm_finish<double>, dim3(nseg), dim3(kBlock), 0, st(),
@@ -582,9 +675,8 @@ struct BatchSolver {
       int j = 0;
       for (; j < m1; ++j) {
         level2(V + (int64_t)j * N, L.Zc(N, j));
-        launch_sell<double, double, double>(sh->K, sell_args(*sh, false), A64->as<double>(),
-                                            (const double *)L.Zc(N, j),
-                                            EpiStore<double, double>{V + (int64_t)(j + 1) * N, nullptr}, st());
+        spmv<double, double, double, double>(A64->as<double>(), false, (const double *)L.Zc(N, j),
+                                             V + (int64_t)(j + 1) * N, nullptr);
         arnoldi(L, j, 0, status->as<double>());
         std::vector<double> st_h = read_status();
         bool left = false;
@@ -687,11 +779,55 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
   const int C = p.C > 0 ? p.C : 64;
   const int K = p.K > 0 ? p.K : 4;
   const bool offsets = p.index != 0;
+  // consecutive segments of one matrix (a system's right-hand sides): when
+  // every system has the same count NR in {2, 4}, its matrix is laid out
+  // once and the SpMVs serve the NR copies together
   {
-    HostCSR big = concat(pa, B->off, N);
+    std::vector<int> runs;
+    for (int s = 0; s < nseg;) {
+      int e = s + 1;
+      while (e < nseg && A[e].get() == A[s].get()) ++e;
+      runs.push_back(e - s);
+      s = e;
+    }
+    const int r0 = runs[0];
+    bool same = r0 == 2 || r0 == 4;
+    for (int r : runs) same = same && r == r0;
+    B->NR = same ? r0 : 1;
+  }
+  const int NR = B->NR;
+  std::vector<const HostCSR *> msys;  // one part per stored matrix
+  std::vector<int64_t> moff(1, 0);     // matrix row offsets (whole tiles)
+  for (int s = 0; s < nseg; s += NR) {
+    msys.push_back(pa[s]);
+    moff.push_back(moff.back() + (B->off[s + 1] - B->off[s]));
+  }
+  const int64_t NM = moff.back();
+  // chunk (of C rows) -> shift from matrix row to vector row, copy stride
+  auto chunk_tables = [&](DevPtr &dsh, DevPtr &dst) {
+    const int64_t nch = (NM + C - 1) / C;
+    std::vector<int32_t> sh_(std::max<int64_t>(1, nch)), st_(std::max<int64_t>(1, nch));
+    for (size_t m = 0; m < msys.size(); ++m) {
+      const int64_t seg0 = (int64_t)m * NR;
+      for (int64_t ch = moff[m] / C; ch < moff[m + 1] / C; ++ch) {
+        sh_[ch] = (int32_t)(B->off[seg0] - moff[m]);
+        st_[ch] = (int32_t)(B->off[seg0 + 1] - B->off[seg0]);
+      }
+    }
+    dsh = up32(sh_);
+    dst = up32(st_);
+    F3R_HIP(hipStreamSynchronize(c->stream));  // the host tables are released here
+  };
+  if (NR > 1) {
+    chunk_tables(B->csh, B->cst);
+    B->csh16 = B->csh;  // both layouts have chunks of C rows in one order
+    B->cst16 = B->cst;
+  }
+  {
+    HostCSR big = NR > 1 ? concat(msys, moff, NM) : concat(pa, B->off, N);
     // fp16 copy: K = 8 once rows average 12 entries (a 27-point row then
     // takes 4 slot groups instead of 7; a 7-point row takes 1 either way)
-    const int K16 = p.K > 0 ? p.K : (big.nnz() >= (int64_t)12 * N ? 8 : 4);
+    const int K16 = p.K > 0 ? p.K : (big.nnz() >= (int64_t)12 * big.n ? 8 : 4);
     HostSell hs = build_sell(big, C, 1, K, nullptr, false, offsets);
     B->sh = upload_sell_shared(c, hs, false);
     B->A64 = upload_sell_values(c, hs, F3R_F64);

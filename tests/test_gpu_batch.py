@@ -114,3 +114,35 @@ def test_lockstep_batch_segments_are_independent():
     res2 = B3.solve(bv, xv)
     assert all(r.is_solved and r.res_iter == 0 for r in res2)
     ctx.close()
+
+
+def test_lockstep_batch_shared_matrix_copies_identical():
+    """four right-hand sides per system: one stored matrix serving all four
+    copies gives the same solutions, bit for bit, as one matrix copy per
+    segment (the same segments interleaved, so no system repeats in a row)"""
+    import numpy as np
+    import f3r_amd as F
+    ctx = F.Context(0)
+    A1 = F.CSR.dominant(7, 13, margin=0.6)
+    F.algorithm.scaling(A1)
+    A2 = F.CSR.dominant(27, 10, margin=0.9)
+    F.algorithm.scaling(A2)
+    P1 = F.SDAINV(A1, 0.1, 1.0, "f16", ctx=ctx)
+    P2 = F.SDAINV(A2, 0.1, 1.0, "f16", ctx=ctx)
+    rhs = {(k, r): F.uniform(A.n, -1.0, 1.0, seed=100 * k + r) for k, A in ((1, A1), (2, A2)) for r in range(4)}
+    grouped = [(1, r) for r in range(4)] + [(2, r) for r in range(4)]
+    mixed = [(k, r) for r in range(4) for k in (1, 2)]
+    sols = {}
+    for order in (grouped, mixed):
+        As = [A1 if k == 1 else A2 for (k, r) in order]
+        Ps = [P1 if k == 1 else P2 for (k, r) in order]
+        B = F.Batch(ctx, As, Ps)
+        bv = [F.vector.from_numpy(ctx, rhs[key]) for key in order]
+        xv = [F.vector(ctx, "f64", A.n).fill(0.0) for A in As]
+        res = B.solve(bv, xv)
+        assert all(r.is_solved for r in res)
+        for key, x, r in zip(order, xv, res):
+            sols.setdefault(key, []).append((r.res_iter, x.download()))
+    for key, ((i0, x0), (i1, x1)) in sols.items():
+        assert i0 == i1 and np.array_equal(x0, x1), key
+    ctx.close()
