@@ -480,11 +480,13 @@ class F3R:
     def level_apply(self, level, vin, vout):
         check(lib.f3r_solver_level_apply(self.handle, level, vin.handle, vout.handle), "level_apply")
 
-    def time_kernel(self, which, reps=20):
+    def time_kernel(self, which, reps=20, cold=False):
         """mean ms per launch and algorithmic bytes of kernel `which`
-        (0 fp16 SpMV, 1 fp16-value/fp32-vector SpMV, 2 fp32 SpMV, 3 fp64 SpMV)"""
+        (0 fp16 SpMV, 1 fp16-value/fp32-vector SpMV, 2 fp32 SpMV, 3 fp64 SpMV);
+        cold: each launch alone, after a 512 MiB pass that empties the caches"""
         ms, by = ctypes.c_double(), ctypes.c_double()
-        check(lib.f3r_solver_time_kernel(self.handle, which, reps, ctypes.byref(ms), ctypes.byref(by)),
+        w = which | (0x100 if cold else 0)
+        check(lib.f3r_solver_time_kernel(self.handle, w, reps, ctypes.byref(ms), ctypes.byref(by)),
               "time_kernel")
         return ms.value, by.value
 

@@ -338,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
     const int rem = a.n - (s << a.log2C);
     const int tc = rem < C ? rem : C;
     const int su = __builtin_amdgcn_readfirstlane(s);  // C >= 64: one chunk per wavefront
-    const int groups = a.cwidth[su];
+    const int cw = a.cwidth[su];
+    const int groups = chunk_groups(cw);
     const int32_t shift = cshift[su], stride = cstride[su];
     int64_t pos = (int64_t)a.cptr[su] + (int64_t)j * K;
     const int64_t step = (int64_t)tc * K;
@@ -357,6 +358,19 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
 #pragma unroll
           for (int c = 0; c < NR; ++c)
             acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[vr + (int64_t)c * stride + o]), acc[c]);
+        }
+      }
+      if constexpr (K >= 2) {
+        if (cw & kHalfTail) {  // the half group (K / 2 slots per row, rows consecutive)
+          constexpr int H = K / 2;
+          const Pack<V, H> pv = load_pack<V, H>(val + ((int64_t)a.cptr[su] + (int64_t)groups * step + (int64_t)j * H));
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            const int32_t o = off[groups * K + k];
+#pragma unroll
+            for (int c = 0; c < NR; ++c)
+              acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[vr + (int64_t)c * stride + o]), acc[c]);
+          }
         }
       }
     } else {

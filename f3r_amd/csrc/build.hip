@@ -134,14 +134,32 @@ __global__ __launch_bounds__(256) void k_chunk_scan(int32_t n, int32_t C, const 
   if (threadIdx.x == 0) info[s] = ci;
 }
 
-// offset tables: coff[tpos[s] + t] = union[t] (0 past |union|, up to the K-padded width)
+// same[s] = 1: offset chunks s - 1 and s have the same union (chunk s then
+// shares the table of chunk s - 1, as in the host builder)
+__global__ void k_union_same(int32_t nch, const ChunkInfo *__restrict__ info, const int32_t *__restrict__ ubuf,
+                             int32_t *__restrict__ same) {
+  __shared__ int diff;
+  const int s = blockIdx.x;
+  if (threadIdx.x == 0) diff = 0;
+  __syncthreads();
+  const ChunkInfo a = info[s];
+  const bool cand = s > 0 && a.offset && info[s - 1].offset && info[s - 1].ulen == a.ulen;
+  if (cand)
+    for (int t = threadIdx.x; t < a.ulen; t += blockDim.x)
+      if (ubuf[(int64_t)s * kSortCap + t] != ubuf[(int64_t)(s - 1) * kSortCap + t]) diff = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) same[s] = cand && !diff ? 1 : 0;
+}
+
+// offset tables: coff[tpos[s] + t] = union[t] (0 past |union|, up to the
+// chunk's slot width); tpos[s] < 0: the chunk shares an earlier table
 __global__ void k_coff_fill(const ChunkInfo *__restrict__ info, const int32_t *__restrict__ ubuf,
                             const int32_t *__restrict__ tpos, const int32_t *__restrict__ cw, int K,
                             int32_t *__restrict__ coff) {
   const int s = blockIdx.x;
   const ChunkInfo ci = info[s];
-  if (!ci.offset) return;
-  const int w = cw[s] * K;
+  if (!ci.offset || tpos[s] < 0) return;
+  const int w = chunk_slots(cw[s], K);
   for (int t = threadIdx.x; t < w; t += blockDim.x)
     coff[tpos[s] + t] = t < ci.ulen ? ubuf[(int64_t)s * kSortCap + t] : 0;
 }
@@ -164,7 +182,7 @@ __global__ void k_sell_fill(int32_t n, int32_t C, int K, const int32_t *__restri
     for (int k = rptr[r]; k < rptr[r + 1]; ++k) {
       const int d = col[k] - r;
       while (u[t] < d) ++t;  // columns ascend: the slot only moves forward
-      val[c0 + ((int64_t)(t / K) * tc + j) * K + (t % K)] = a[k];
+      val[chunk_slot_pos(c0, cw[s], tc, j, t, K)] = a[k];
     }
     return;
   }
@@ -235,7 +253,15 @@ bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, cons
                        dinfo->as<ChunkInfo>(), ubuf->as<int32_t>());
   F3R_HIP(hipGetLastError());
   std::vector<ChunkInfo> info(nch);
+  std::vector<int32_t> same(nch, 0);
   if (nch) F3R_HIP(hipMemcpyAsync(info.data(), dinfo->p, sizeof(ChunkInfo) * nch, hipMemcpyDeviceToHost, st));
+  DevPtr dsame = dev_alloc(sizeof(int32_t) * std::max(1, nch));
+  if (offsets && nch) {
+    hipLaunchKernelGGL(k_union_same, dim3(nch), dim3(64), 0, st, nch, (const ChunkInfo *)dinfo->as<ChunkInfo>(),
+                       (const int32_t *)ubuf->as<int32_t>(), dsame->as<int32_t>());
+    F3R_HIP(hipGetLastError());
+    F3R_HIP(hipMemcpyAsync(same.data(), dsame->p, sizeof(int32_t) * nch, hipMemcpyDeviceToHost, st));
+  }
   F3R_HIP(hipStreamSynchronize(st));
   for (const ChunkInfo &ci : info)
     if (ci.toobig) return false;
@@ -245,17 +271,22 @@ bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, cons
   int32_t off_chunks = 0, off_bias = 0;
   for (int32_t s = 0; s < nch; ++s) {
     const int32_t tc = std::min<int32_t>(C, n - s * C);
-    cw[s] = round_up(info[s].width, K) / K;
-    tot += (int64_t)cw[s] * K * tc;
+    cw[s] = offsets && info[s].offset ? offset_chunk_width(info[s].width, K) : round_up(info[s].width, K) / K;
+    tot += (int64_t)chunk_slots(cw[s], K) * tc;
     F3R_CHECK(tot < INT32_MAX, F3R_ERR_RANGE, "SELL: more than 2^31 slots");
     cptr[s + 1] = (int32_t)tot;
     if (offsets) {
       if (info[s].offset) {
-        iptr[s] = -(int32_t)ctot - 1;
-        tpos[s] = (int32_t)ctot;
-        ctot += (int64_t)cw[s] * K;
-        off_slots += (int64_t)cw[s] * K * tc;
-        off_bias = std::max(off_bias, -info[s].umin);
+        if (same[s]) {
+          iptr[s] = iptr[s - 1];
+          tpos[s] = -1;
+        } else {
+          iptr[s] = -(int32_t)ctot - 1;
+          tpos[s] = (int32_t)ctot;
+          ctot += chunk_slots(cw[s], K);
+          off_bias = std::max(off_bias, -info[s].umin);
+        }
+        off_slots += (int64_t)chunk_slots(cw[s], K) * tc;
         ++off_chunks;
       } else {
         iptr[s] = (int32_t)itot;

@@ -99,9 +99,36 @@ struct HostCSR {
 // slot group g of row j of the chunk occupies elements
 //   cptr[s] + (g * tc + j) * K + k,  k < K
 // (K = 1 gives the reference's column-major layout, sell.hpp:272-287).
+// A chunk-offset chunk (below) may end in a half group: with bit kHalfTail
+// of cwidth[s] set, K / 2 more slots follow its G full groups, K / 2 per
+// row: slot G * K + k of row j at cptr[s] + G * K * tc + j * (K / 2) + k.
+// A 27-entry stencil row then takes 28 slots with K = 8, not 32.
 // Row r of the sorted order is row perm[r] of the original matrix; the
 // column indices stay in the original numbering unless `internal` is set, in
 // which case they are renumbered into the sorted order.
+#ifndef F3R_HALF_GROUPS
+#define F3R_HALF_GROUPS 1
+#endif
+constexpr int32_t kHalfTail = 1 << 30;
+__host__ __device__ inline int32_t chunk_groups(int32_t cw) { return cw & (kHalfTail - 1); }
+__host__ __device__ inline int32_t chunk_slots(int32_t cw, int32_t K) {
+  return chunk_groups(cw) * K + ((cw & kHalfTail) ? K / 2 : 0);
+}
+// width entry of an offset chunk with `width` slots in use: full groups, and
+// a half group when the rest fits in K / 2 slots
+__host__ __device__ inline int32_t offset_chunk_width(int32_t width, int32_t K) {
+  const int32_t g = width / K, rest = width - g * K;
+  if (rest == 0) return g;
+  if (F3R_HALF_GROUPS && K >= 2 && rest <= K / 2) return g | kHalfTail;
+  return g + 1;
+}
+// element of slot t of row j in a chunk starting at c0 with tc rows
+__host__ __device__ inline int64_t chunk_slot_pos(int64_t c0, int32_t cw, int32_t tc, int32_t j, int32_t t, int32_t K) {
+  const int32_t g = chunk_groups(cw);
+  if (t < g * K) return c0 + ((int64_t)(t / K) * tc + j) * K + (t % K);
+  return c0 + (int64_t)g * K * tc + (int64_t)j * (K / 2) + (t - g * K);
+}
+
 struct SellShared {  // index data shared by all precisions of one matrix
   int32_t n = 0, C = 64, sigma = 1, K = 1, nchunks = 0;
   int64_t nslots = 0, nnz = 0;

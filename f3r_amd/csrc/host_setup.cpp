@@ -141,7 +141,8 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
         }
         if (ok) {
           uni[s] = u;
-          width = (int32_t)u.size();
+          S.cwidth[s] = offset_chunk_width((int32_t)u.size(), K);
+          continue;
         }
       }
       S.cwidth[s] = round_up(width, K) / K;
@@ -150,7 +151,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
   int64_t tot = 0;
   for (int32_t s = 0; s < S.nchunks; ++s) {
     int32_t tc = (int32_t)std::min<int64_t>(C, n - (int64_t)s * C);
-    tot += (int64_t)S.cwidth[s] * K * tc;
+    tot += (int64_t)chunk_slots(S.cwidth[s], K) * tc;
     F3R_CHECK(tot < INT32_MAX, F3R_ERR_RANGE, "SELL: more than 2^31 slots");
     S.cptr[s + 1] = (int32_t)tot;
   }
@@ -159,11 +160,17 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
   if (offsets) {
     S.iptr.assign(S.nchunks, 0);
     for (int32_t s = 0; s < S.nchunks; ++s) {
-      const int64_t w = (int64_t)S.cwidth[s] * K;
+      const int64_t w = chunk_slots(S.cwidth[s], K);
       if (!uni[s].empty()) {
-        S.iptr[s] = -(int32_t)S.coff.size() - 1;
-        S.off_bias = std::max(S.off_bias, -uni[s].front());
-        for (int64_t t = 0; t < w; ++t) S.coff.push_back(t < (int64_t)uni[s].size() ? uni[s][t] : 0);
+        if (s > 0 && uni[s] == uni[s - 1]) {
+          // the previous chunk's offsets: share its table, so the tables the
+          // wavefronts read stay few and cached (one for a stencil's interior)
+          S.iptr[s] = S.iptr[s - 1];
+        } else {
+          S.iptr[s] = -(int32_t)S.coff.size() - 1;
+          S.off_bias = std::max(S.off_bias, -uni[s].front());
+          for (int64_t t = 0; t < w; ++t) S.coff.push_back(t < (int64_t)uni[s].size() ? uni[s][t] : 0);
+        }
         ++S.offset_chunks;
       } else {
         S.iptr[s] = (int32_t)itot;
@@ -181,7 +188,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
   for (int64_t s = c0; s < c1; ++s) {
     int64_t r0 = s * C;
     int32_t tc = (int32_t)std::min<int64_t>(C, n - r0);
-    int32_t width = S.cwidth[s] * K;
+    int32_t width = chunk_slots(S.cwidth[s], K);
     if (offsets && !uni[s].empty()) {
       // offset chunk: the value of column r + uni[t] goes to slot t, the
       // other slots stay 0 (their columns r + uni[t] are in range)
@@ -193,8 +200,7 @@ HostSell build_sell(const HostCSR &A, int32_t C, int32_t sigma, int32_t K,
         for (int32_t k = A.rptr[o]; k < A.rptr[o + 1]; ++k) {
           const int32_t d = col(k) - r;
           while (us[t] < d) ++t;  // columns ascend, so the slot only moves forward
-          const int64_t pos = S.cptr[s] + ((int64_t)(t / K) * tc + j) * K + (t % K);
-          S.val[pos] = A.val[k];
+          S.val[chunk_slot_pos(S.cptr[s], S.cwidth[s], tc, j, t, K)] = A.val[k];
         }
       }
       continue;

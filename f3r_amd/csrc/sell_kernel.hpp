@@ -8,6 +8,7 @@
 // fused Krylov / Richardson updates of the solver.
 #pragma once
 
+#include "common.hpp"
 #include "device.hpp"
 
 // groups of K slots of an offset chunk issued back to back (memory-level
@@ -31,7 +32,7 @@ struct SellArgs {
   int32_t n;        // rows
   int32_t log2C;    // chunk height C = 1 << log2C
   const int32_t *__restrict__ cptr;    // element offset of each chunk
-  const int32_t *__restrict__ cwidth;  // slot groups of each chunk
+  const int32_t *__restrict__ cwidth;  // slot groups of each chunk (| kHalfTail: and a half group)
   const int32_t *__restrict__ idx;     // column of each slot
   const int32_t *__restrict__ perm;    // sorted row -> original row (null = identity)
   // chunk-offset format (null = every chunk explicit): iptr[s] >= 0 is the
@@ -57,6 +58,44 @@ __device__ __forceinline__ X buffer_gather(__amdgpu_buffer_rsrc_t rs, int voff, 
   }
 }
 
+#ifndef F3R_FIXED_ROWS
+#define F3R_FIXED_ROWS 1
+#endif
+// Row of an offset chunk of G full groups and, if HALF, a half group, with
+// every value load and x gather issued before the first multiply-add (a
+// runtime loop keeps one or two groups in flight).  Same sum, same order.
+template <int G, bool HALF, class V, class X, class ACC, int K>
+__device__ __forceinline__ ACC sell_row_fixed(const V *__restrict__ val, int64_t pos, int64_t step, int64_t hpos,
+                                              __amdgpu_buffer_rsrc_t rs, int vo, int bias,
+                                              const __attribute__((address_space(4))) int32_t *off) {
+  constexpr int H = K / 2;
+  Pack<V, K> pv[G];
+  X xv[G][K];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    pv[g] = load_pack<V, K>(val + pos + g * step);
+#pragma unroll
+    for (int k = 0; k < K; ++k) xv[g][k] = buffer_gather<X>(rs, vo, (off[g * K + k] + bias) * (int)sizeof(X));
+  }
+  Pack<V, (HALF ? H : 1)> ph{};
+  X xh[HALF ? H : 1];
+  if constexpr (HALF) {
+    ph = load_pack<V, H>(val + hpos);
+#pragma unroll
+    for (int k = 0; k < H; ++k) xh[k] = buffer_gather<X>(rs, vo, (off[G * K + k] + bias) * (int)sizeof(X));
+  }
+  ACC acc = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv[g].v[k]), (ACC)to_acc(xv[g][k]), acc);
+  if constexpr (HALF) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) acc = fma_<ACC>((ACC)to_acc(ph.v[k]), (ACC)to_acc(xh[k]), acc);
+  }
+  return acc;
+}
+
 // Sum of row r (in sorted order) of A*x with accumulator ACC.
 template <class V, class X, class ACC, int K>
 __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__ val,
@@ -66,7 +105,8 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
   const int C = 1 << a.log2C;
   const int rem = a.n - (s << a.log2C);
   const int tc = rem < C ? rem : C;
-  const int groups = a.cwidth[s];
+  const int cw = a.cwidth[s];
+  const int groups = chunk_groups(cw);
   int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K;
   const int64_t step = (int64_t)tc * K;
   ACC acc = 0;
@@ -89,6 +129,25 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
           (int)(((int64_t)a.n + a.off_bias) * (int64_t)sizeof(X)), kBufferWord3);
       const int vo = r * (int)sizeof(X);
       const int bias = a.off_bias;
+#if F3R_FIXED_ROWS
+      {
+        // the widths of 27- and 7-point stencil chunks (uniform per wavefront)
+        const int cwu = __builtin_amdgcn_readfirstlane(cw);
+        const int64_t hpos = (int64_t)a.cptr[s] + (int64_t)groups * step + (int64_t)j * (K / 2);
+        if constexpr (K == 8) {
+          if (cwu == (3 | kHalfTail)) return sell_row_fixed<3, true, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+          if (cwu == 4) return sell_row_fixed<4, false, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+          if (cwu == 1) return sell_row_fixed<1, false, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+        } else if constexpr (K == 4) {
+          if (cwu == 7) return sell_row_fixed<7, false, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+          if (cwu == 2) return sell_row_fixed<2, false, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+        } else if constexpr (K == 2) {
+          if (cwu == (13 | kHalfTail)) return sell_row_fixed<13, true, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+          if (cwu == 14) return sell_row_fixed<14, false, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+          if (cwu == (3 | kHalfTail)) return sell_row_fixed<3, true, V, X, ACC, K>(val, pos, step, hpos, rs, vo, bias, off);
+        }
+      }
+#endif
       F3R_PRAGMA(unroll F3R_OFF_UNROLL)
       for (int g = 0; g < groups; ++g, pos += step) {
         const Pack<V, K> pv = load_pack<V, K>(val + pos);
@@ -97,6 +156,17 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
         for (int k = 0; k < K; ++k) xv[k] = buffer_gather<X>(rs, vo, (off[g * K + k] + bias) * (int)sizeof(X));
 #pragma unroll
         for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
+      }
+      if constexpr (K >= 2) {
+        if (cw & kHalfTail) {  // the half group: K / 2 slots per row, rows consecutive
+          constexpr int H = K / 2;
+          const Pack<V, H> pv = load_pack<V, H>(val + ((int64_t)a.cptr[s] + (int64_t)groups * step + (int64_t)j * H));
+          X xv[H];
+#pragma unroll
+          for (int k = 0; k < H; ++k) xv[k] = buffer_gather<X>(rs, vo, (off[groups * K + k] + bias) * (int)sizeof(X));
+#pragma unroll
+          for (int k = 0; k < H; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
+        }
       }
 #else
       const X *__restrict__ xr = x + r;
@@ -108,6 +178,15 @@ __device__ __forceinline__ ACC sell_row(const SellArgs &a, const V *__restrict__
         for (int k = 0; k < K; ++k) xv[k] = xr[off[g * K + k]];
 #pragma unroll
         for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k]), acc);
+      }
+      if constexpr (K >= 2) {
+        if (cw & kHalfTail) {
+          constexpr int H = K / 2;
+          const Pack<V, H> pv = load_pack<V, H>(val + ((int64_t)a.cptr[s] + (int64_t)groups * step + (int64_t)j * H));
+#pragma unroll
+          for (int k = 0; k < H; ++k)
+            acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xr[off[groups * K + k]]), acc);
+        }
       }
 #endif
       return acc;

@@ -141,7 +141,11 @@ int f3r_sell_layout(int32_t n, const int32_t *rptr, const int32_t *col, const do
  * with the array outputs NULL returns the sizes: value slots, int32 column
  * indices (of the explicit chunks) and offset-table entries.  Then cptr[nchunks+1],
  * cwidth[nchunks], iptr[nchunks] (>= 0: start of an explicit chunk in idx;
- * < 0: offset chunk with table coff[-iptr-1 ...]), idx, coff and sval. */
+ * < 0: offset chunk with table coff[-iptr-1 ...]), idx, coff and sval.
+ * cwidth of an offset chunk counts its full groups of K slots; bit 30 set
+ * adds a half group of K / 2 slots per row after them, stored row by row:
+ * slot G*K + k of row j at cptr[s] + G*K*tc + j*(K/2) + k (G full groups,
+ * tc rows in the chunk). */
 int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, const double *val,
                            int32_t C, int32_t K, int64_t *nslots, int64_t *nidx, int64_t *ncoff,
                            int32_t *cptr, int32_t *cwidth, int32_t *iptr, int32_t *idx,
@@ -266,7 +270,10 @@ int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out)
  * 4 the level-3 SpMV + CGS dots and 5 the CGS update of its last step), measured
  * with HIP events on the solver's stream over `reps` launches; `bytes` = the
  * bytes the layout must move per launch (values and int32 indices of the
- * nonzeros of int32-index chunks, every value slot of chunk-offset chunks, x, y) */
+ * nonzeros of int32-index chunks, every value slot of chunk-offset chunks, x, y).
+ * which | F3R_TIME_COLD: caches cold, each launch timed alone after a pass over
+ * 512 MiB that evicts the operands from the L2s and the last-level cache. */
+#define F3R_TIME_COLD 0x100
 int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms,
                            double *bytes);
 int f3r_solver_destroy(f3r_solver S);

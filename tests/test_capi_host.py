@@ -187,12 +187,18 @@ def _rows_from_offset_layout(L, n, C, K):
     for s in range(nch):
         r0 = s * C
         tc = min(C, n - r0)
-        w = L["cwidth"][s] * K
+        g, half = L["cwidth"][s] & ((1 << 30) - 1), bool(L["cwidth"][s] & (1 << 30))
+        assert not half or L["iptr"][s] < 0  # half groups only in offset chunks
+        w = g * K + (K // 2 if half else 0)
+        assert L["cptr"][s + 1] - L["cptr"][s] == w * tc
         for j in range(tc):
             r = r0 + j
             cols, vals = [], []
             for t in range(w):
-                pos = L["cptr"][s] + ((t // K) * tc + j) * K + t % K
+                if t < g * K:
+                    pos = L["cptr"][s] + ((t // K) * tc + j) * K + t % K
+                else:
+                    pos = L["cptr"][s] + g * K * tc + j * (K // 2) + t - g * K
                 if L["iptr"][s] < 0:
                     c = r + L["coff"][-L["iptr"][s] - 1 + t]
                 else:
@@ -203,7 +209,8 @@ def _rows_from_offset_layout(L, n, C, K):
     return rows
 
 
-@pytest.mark.parametrize("kind,C,K", [("27pt", 64, 4), ("27pt", 128, 2), ("7pt", 64, 1), ("ragged", 64, 4)])
+@pytest.mark.parametrize("kind,C,K", [("27pt", 64, 4), ("27pt", 64, 8), ("27pt", 128, 2), ("7pt", 64, 1),
+                                      ("ragged", 64, 4)])
 def test_offset_layout_reproduces_csr_rows(kind, C, K):
     # every nonzero of every row appears once, in CSR (column) order; the other
     # slots hold 0 at an in-range column; stencil chunks use offsets
@@ -233,6 +240,20 @@ def test_offset_layout_reproduces_csr_rows(kind, C, K):
             assert np.all(vals[k:] == 0)
     if kind != "ragged":
         assert n_off >= A.n // 2
+    if kind != "ragged":
+        # consecutive chunks with the same offsets share one table
+        offc = L["iptr"] < 0
+        assert len(np.unique(L["iptr"][offc])) < offc.sum()
+        for s in range(1, len(L["iptr"])):
+            if offc[s] and offc[s - 1] and L["iptr"][s] != L["iptr"][s - 1]:
+                w = lambda c: (L["cwidth"][c] & ((1 << 30) - 1)) * K + (K // 2 if L["cwidth"][c] & (1 << 30) else 0)
+                t0, t1 = -L["iptr"][s - 1] - 1, -L["iptr"][s] - 1
+                assert w(s) != w(s - 1) or not np.array_equal(L["coff"][t0:t0 + w(s)], L["coff"][t1:t1 + w(s)])
+    if kind == "27pt" and K == 8:
+        # 27 offsets: three groups of 8 and a half group of 4 (28 slots, not 32)
+        half = (L["cwidth"] & (1 << 30)) != 0
+        assert half.any() and np.all(L["iptr"][half] < 0)
+        assert np.all(L["cwidth"][half] & ((1 << 30) - 1) == 3)
     # eligibility rule: an offset chunk has at most 5/4 of the int32 layout's
     # slots, so the layout never grows by more than that
     plain = F.sell_layout(A, C, 1, K)[3]

@@ -19,7 +19,7 @@ COMBOS = [  # (value, x, y) precisions; sums in fp64 for fp64 data, else fp32
 ]
 # (C, sigma, K, chunk offsets)
 LAYOUTS = [(32, 1, 1, False), (64, 1, 4, False), (64, 256, 2, False), (128, 64, 4, False), (16, 1, 4, False),
-           (64, 1, 4, True), (128, 1, 1, True), (256, 1, 2, True), (64, 256, 4, True)]
+           (64, 1, 4, True), (128, 1, 1, True), (256, 1, 2, True), (64, 256, 4, True), (64, 1, 8, True)]
 NP = {"f64": np.float64, "f32": np.float32, "f16": np.float16}
 
 
