@@ -103,6 +103,9 @@ template <class T, int NV> struct EpiDots {
   T *Hcol;
   T *partial;
   unsigned *counter;
+  // x = z * s with s a power of two (fp16 storage of z): the sum is scaled
+  // back by xs = 1/s, exactly, so w is bit-identical to the sum over z
+  T xs = 1;
   __device__ Pre pre(int r) const {
     Pre p;
 #pragma unroll
@@ -110,6 +113,7 @@ template <class T, int NV> struct EpiDots {
     return p;
   }
   __device__ void row(int r, T acc, const Pre &p, State &s) const {
+    acc *= xs;
     w[r] = acc;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -129,38 +133,33 @@ template <class T, int NV> struct EpiDots {
   }
 };
 
-// Hcol[k] = sum over the G block partials of dot k, in block order per
-// thread, then a fixed tree over the 1024 threads
-template <class T, int NV>
-__global__ __launch_bounds__(1024) void k_dots_finish(const T *__restrict__ partial, int G, int ncols, T *Hcol) {
-  __shared__ T lds[NV][16];
+// Hcol[k] = sum over the G block partials of dot k (block k of the launch,
+// one per dot), in block order per thread, then a fixed tree over the 1024
+// threads.  One block per dot keeps the dots' sums on different CUs: a single
+// block summing 8 dots of 8192 partials took 17 us.
+template <class T>
+__global__ __launch_bounds__(1024) void k_dots_finish(const T *__restrict__ partial, int G, T *Hcol) {
+  __shared__ T lds[16];
   // every load of a thread is issued before the first add (one memory
   // latency, not G / 1024 of them); G <= kMaxReduceGrid < 16 * 1024
   constexpr int kPer = 16;
-  T v[NV];
+  const T *__restrict__ p = partial + (int64_t)blockIdx.x * G;
+  T t[kPer];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    T t[kPer];
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int b = threadIdx.x + q * 1024;
-      t[q] = (k < ncols && b < G) ? partial[(int64_t)k * G + b] : (T)0;
-    }
-    v[k] = 0;
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) v[k] += t[q];
+  for (int q = 0; q < kPer; ++q) {
+    const int b = threadIdx.x + q * 1024;
+    t[q] = b < G ? p[b] : (T)0;
   }
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  T v = 0;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    v[k] = wave_sum(v[k]);
-    if (lane == 0) lds[k][wid] = v[k];
-  }
+  for (int q = 0; q < kPer; ++q) v += t[q];
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
   __syncthreads();
-  if (threadIdx.x < NV && threadIdx.x < ncols) {
-    T t = lds[threadIdx.x][0];
-    for (int w = 1; w < 16; ++w) t += lds[threadIdx.x][w];
-    Hcol[threadIdx.x] = t;
+  if (threadIdx.x == 0) {
+    T u = lds[0];
+    for (int w = 1; w < 16; ++w) u += lds[w];
+    Hcol[blockIdx.x] = u;
   }
 }
 
@@ -242,9 +241,16 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
     w[i] = wn;
     v[0] = fma_<T>(wn, wn, v[0]);
   }
-  // block sums of ||w||^2 only; k_cgs_finish adds them (see EpiDots)
-  block_sum<T, 1>(v, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+  // the grid's sum of ||w||^2 in the last block (one wave of blocks: the
+  // arrivals cost less than a one-block finishing launch, 0.12 ms per solve
+  // at 128^3)
+  if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
+    T nv = (T)sqrt(v[0]);
+    Hcol[j + 1] = nv;
+    *inv = (T)1 / nv;
+    givens_step<T>(Hcol, cs, sn, e, j, e0_one != 0);
+    if (status) *status = fabs((double)e[j + 1]);
+  }
 }
 
 // ||w|| from the G block sums (fixed order), then H(j, j+1), 1/h, Givens and
@@ -333,10 +339,11 @@ __global__ void k_vscale(int64_t n, T *__restrict__ v, const T *__restrict__ inv
 }
 
 // out = Z y (with_init) of an inner level; every block solves the small
-// triangular system itself (j <= NV)
-template <class T, class TO, int NV, bool VEC = false>
-__global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<T, NV> Z, int j, const T *H, int ldh,
-                                                     const T *e, TO *__restrict__ out) {
+// triangular system itself (j <= NV).  Columns of type ZT hold z * s (fp16
+// storage, s a power of two): z = column * zs is exact.
+template <class T, class TO, int NV, bool VEC = false, class ZT = T>
+__global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<ZT, NV> Z, int j, const T *H, int ldh,
+                                                     const T *e, TO *__restrict__ out, T zs) {
   __shared__ T y[NV];
   if (threadIdx.x == 0) hess_solve<T>(H, ldh, e, y, j);
   __syncthreads();
@@ -345,17 +352,17 @@ __global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<T, NV> Z, i
   if constexpr (VEC) {
     const int64_t n4 = n / 4;
     for (int64_t q = gid; q < n4; q += gstride) {
-      V4<T> zk[NV];
+      V4<ZT> zk[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k)
         if (k < j) zk[k] = ld4(Z.p[k] + 4 * q);
       V4<TO> o;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        T r = zk[0].v[c] * y[0];
+        T r = ((T)to_acc(zk[0].v[c]) * zs) * y[0];
 #pragma unroll
         for (int k = 1; k < NV; ++k)
-          if (k < j) r = fma_<T>(zk[k].v[c], y[k], r);
+          if (k < j) r = fma_<T>((T)to_acc(zk[k].v[c]) * zs, y[k], r);
         o.v[c] = cvt<TO>(r);
       }
       st4(out + 4 * q, o);
@@ -363,10 +370,10 @@ __global__ __launch_bounds__(kBlock) void k_solution(int64_t n, Cols<T, NV> Z, i
     i0 = 4 * n4;
   }
   for (int64_t i = i0 + gid; i < n; i += gstride) {
-    T r = Z.p[0][i] * y[0];
+    T r = ((T)to_acc(Z.p[0][i]) * zs) * y[0];
 #pragma unroll
     for (int k = 1; k < NV; ++k)
-      if (k < j) r = fma_<T>(Z.p[k][i], y[k], r);
+      if (k < j) r = fma_<T>((T)to_acc(Z.p[k][i]) * zs, y[k], r);
     out[i] = cvt<TO>(r);
   }
 }
@@ -688,6 +695,10 @@ template <class T> struct InnerLevel {
   int m = 0;
   DevPtr Vbuf, Zbuf, H, cs, sn, e, inv;
   std::vector<T *> V, Z;  // column pointers; V[0] aliases the level's input
+  // level 3 of the HALF variant: Z_j as the fp16 Richardson output h
+  // (Z_j = h / s exactly, s the power-of-two range scale), half the bytes
+  DevPtr Zhbuf;
+  std::vector<half_t *> Zh;
   int ldh() const { return m + 1; }
   T *Hp() const { return H->as<T>(); }
 };
@@ -771,10 +782,15 @@ struct Solver {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     EpiDots<T, NV> epi{L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), partial<T>(), counter(0)};
-    launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
+    if (!L.Zh.empty()) {
+      epi.xs = (T)(1.0 / range_s);
+      launch_sell<V, half_t, T>(shv<V>().K, argsv<V>(), val, L.Zh[j], epi, st(), spmv_grid);
+    } else {
+      launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
+    }
     const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell used
     const int G = spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
-    hipLaunchKernelGGL((k_dots_finish<T, NV>), dim3(1), dim3(1024), 0, st(), (const T *)partial<T>(), G, j + 1,
+    hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(), G,
                        L.Hp() + (int64_t)j * L.ldh());
   }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
@@ -787,9 +803,6 @@ struct Solver {
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
                        (double *)nullptr, partial<T>(), counter(1));
-    hipLaunchKernelGGL(k_cgs_finish<T>, dim3(1), dim3(1024), 0, st(), (const T *)partial<T>(), dots_grid,
-                       L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(), L.sn->template as<T>(),
-                       L.e->template as<T>(), j, 1, L.inv->template as<T>(), (double *)nullptr);
   }
   template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
@@ -822,13 +835,23 @@ struct Solver {
     const int j = L.m;
     auto go = [&](auto nvtag) {
       constexpr int NV = decltype(nvtag)::value;
+      if (!L.Zh.empty()) {
+        Cols<half_t, NV> c{};
+        for (int k = 0; k < j; ++k) c.p[k] = L.Zh[k];
+        bool vec = aligned4(out);
+        for (int k = 0; k < j; ++k) vec = vec && aligned4(L.Zh[k]);
+        auto kern = vec ? k_solution<T, TO, NV, true, half_t> : k_solution<T, TO, NV, false, half_t>;
+        hipLaunchKernelGGL(kern, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j, (const T *)L.Hp(),
+                           L.ldh(), (const T *)L.e->template as<T>(), out, (T)(1.0 / range_s));
+        return;
+      }
       Cols<T, NV> c{};
       for (int k = 0; k < j; ++k) c.p[k] = L.Z[k];
       bool vec = aligned4(out);
       for (int k = 0; k < j; ++k) vec = vec && aligned4(L.Z[k]);
       auto kern = vec ? k_solution<T, TO, NV, true> : k_solution<T, TO, NV, false>;
       hipLaunchKernelGGL(kern, dim3(grid()), dim3(kBlock), 0, st(), (int64_t)n, c, j,
-                         (const T *)L.Hp(), L.ldh(), (const T *)L.e->template as<T>(), out);
+                         (const T *)L.Hp(), L.ldh(), (const T *)L.e->template as<T>(), out, (T)1);
     };
     if (j <= 4) go(std::integral_constant<int, 4>());
     else if (j <= 8) go(std::integral_constant<int, 8>());
@@ -836,7 +859,8 @@ struct Solver {
   }
 
   // ------------------------------------------------- level 4 (fp16) ---
-  void level4_half(const float *in, float *out) {
+  // out: fp32 result h / s; or, with hz, the fp16 result h itself
+  void level4_half(const float *in, float *out, half_t *hz = nullptr) {
     ++cnt;
     ++calls;
     const bool probe = p.static_weight <= 0 && (cnt % p.c) == 0;
@@ -875,9 +899,11 @@ struct Solver {
     for (int k = 1; k < p.m4; ++k) {
       const bool last = k == p.m4 - 1;
       half_t *hr_ = hr->as<half_t>(), *t2 = tmp2->as<half_t>();
+      half_t *hn = last && hz ? hz : ho2;     // the step's fp16 result
+      float *of = last && !hz ? out : nullptr;  // or, last step, its fp32 h / s
       if (diag) {
         EpiL4Diag epi{hi, ho, dw, dz, &rs->ome[k], probe ? hr_ : nullptr, probe ? t2 : nullptr,
-                      ho2, last ? out : nullptr, inv_s};
+                      hn, of, inv_s};
         launch_sell<half_t, half_t, float>(K16, a, a16, ho, epi, st());
       } else {
         launch_sell<half_t, half_t, float>(K16, a, a16, ho, EpiResid<half_t, half_t, float>{hr_, hi, nullptr},
@@ -890,15 +916,19 @@ struct Solver {
         hipLaunchKernelGGL(k_probe, dim3(dots_grid), b, 0, st(), nn, (const half_t *)t, (const half_t *)hr_, rs,
                            k, partial<float>(), counter(2));
         hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
-                           (const double *)&rs->alpha[k], ho2, last ? out : nullptr, inv_s);
+                           (const double *)&rs->alpha[k], hn, of, inv_s);
       } else if (!diag) {
         hipLaunchKernelGGL(k_l4_update, g, b, 0, st(), nn, (const half_t *)t2, (const half_t *)ho,
-                           (const double *)&rs->ome[k], ho2, last ? out : nullptr, inv_s);
+                           (const double *)&rs->ome[k], hn, of, inv_s);
       }
       if (!last) std::swap(ho, ho2);
     }
-    if (p.m4 == 1)  // no Richardson step: out = fp32(hout) / s
-      hipLaunchKernelGGL((k_cast2<half_t, float>), g, b, 0, st(), nn, (const half_t *)ho, out);
+    if (p.m4 == 1) {  // no Richardson step: out = fp32(hout) / s, or hz = hout
+      if (hz)
+        hipLaunchKernelGGL((k_cast2<half_t, half_t>), g, b, 0, st(), nn, (const half_t *)ho, hz);
+      else
+        hipLaunchKernelGGL((k_g_entry<half_t, float>), g, b, 0, st(), nn, (const half_t *)ho, out, (double)inv_s);
+    }
   }
 
   // ------------------------------------ level 4 of the FLOAT / DOUBLE builds ---
@@ -938,7 +968,8 @@ struct Solver {
     }
   }
 
-  template <class T> void level4(const T *in, T *out) {
+  // hz (HALF variant, level 3): the fp16 result instead of `out`
+  template <class T> void level4(const T *in, T *out, half_t *hz = nullptr) {
     if (capturing() && calls % cap_every == 0) {
       float *dst = cap->as<float>() + (int64_t)cap_n * n;
       if constexpr (std::is_same<T, float>::value)
@@ -949,7 +980,7 @@ struct Solver {
     }
     if constexpr (std::is_same<T, float>::value) {
       if (p.variant == 0) {
-        level4_half(in, out);
+        level4_half(in, out, hz);
         return;
       }
     }
@@ -961,7 +992,7 @@ struct Solver {
     InnerLevel<T> &L = L3<T>();
     L.V[0] = const_cast<T *>(in);
     for (int j = 0; j < L.m; ++j) {
-      level4<T>(L.V[j], L.Z[j]);
+      level4<T>(L.V[j], L.Z[j], L.Zh.empty() ? nullptr : L.Zh[j]);
       if constexpr (std::is_same<T, double>::value) {
         arnoldi<T, double>(L, A64->as<double>(), j);
       } else {
@@ -1338,6 +1369,12 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   } else {
     mk(S->L2f, p.m2);
     mk(S->L3f, p.m3);
+    if (p.variant == 0) {
+      const int64_t ld = (n + 63) / 64 * 64;
+      S->L3f.Zhbuf = dev_alloc(sizeof(half_t) * ld * p.m3);
+      S->L3f.Zh.resize(p.m3);
+      for (int k = 0; k < p.m3; ++k) S->L3f.Zh[k] = S->L3f.Zhbuf->as<half_t>() + (int64_t)k * ld;
+    }
   }
   const size_t wsz = p.variant == 0 ? sizeof(half_t) : p.variant == 1 ? sizeof(float) : sizeof(double);
   auto dh = [&](void) { return dev_alloc(wsz * n); };

@@ -68,6 +68,27 @@ def test_level4_bitexact(ctx, kind, prec):
     assert bad.size == 0
 
 
+@pytest.mark.parametrize("kind,prec,m4", [("27pt", "sdainv", 1), ("27pt", "sdainv", 3), ("convdiff", "ilu0", 1),
+                                          ("convdiff", "ilu0", 3)])
+def test_level4_richardson_steps_bitexact(ctx, kind, prec, m4):
+    # one (no residual step) and three Richardson steps; the result is h / s
+    # in every case
+    import f3r_amd as F
+    A = _problem(kind, 12)
+    P = _prec(F, A, ctx, prec)
+    s = _scale(A.n)
+    S = F.F3R(ctx, A, P, m=(100, 8, 4, m4), range_scale=s, use_graph=False)
+    v = np.random.default_rng(4).uniform(-1, 1, A.n)
+    v /= np.linalg.norm(v)
+    vin = F.vector.from_numpy(ctx, v)
+    vout = F.vector(ctx, "f64", A.n)
+    print(f"level4 {kind} {prec} m4={m4}", flush=True)
+    S.level_apply(4, vin, vout)
+    ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 4, v, variant=0, ref_arith=0, range_scale=s, m4=m4,
+                        precond=1 if prec == "ilu0" else 0, bj_blocks=8)
+    assert np.array_equal(vout.download(), ref)
+
+
 @pytest.mark.parametrize("kind", ["27pt", "lognormal"])
 def test_level3_close(ctx, kind):
     import f3r_amd as F
