@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <numeric>
 
 #include "common.hpp"
@@ -397,29 +398,133 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
 
 // An SDAINV factor split by row kind: a row whose only entry is its diagonal
 // is an elementwise product (dval), any other row is a CSR row summed in
-// column order with fp32 accumulation, as the SELL SpMV sums it.
-__global__ __launch_bounds__(kBlock) void k_b_fdiag(const half_t *__restrict__ x, const half_t *__restrict__ d,
-                                                    half_t *__restrict__ y) {
-  for (int q = 0; q < kRows; ++q) {
-    const int64_t i = tile_row(q);
-    y[i] = cvt<half_t>(to_acc(d[i]) * to_acc(x[i]));
-  }
-}
-__global__ __launch_bounds__(kBlock) void k_b_frows(int32_t nr, const int32_t *__restrict__ rows,
-                                                    const int32_t *__restrict__ rptr, const int32_t *__restrict__ col,
-                                                    const half_t *__restrict__ val, const half_t *__restrict__ x,
-                                                    half_t *__restrict__ y) {
-  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nr) return;
-  float acc = 0.f;
-  for (int32_t k = rptr[t]; k < rptr[t + 1]; ++k) acc = fma_<float>(to_acc(val[k]), to_acc(x[col[k]]), acc);
-  y[rows[t]] = cvt<half_t>(acc);
-}
-
+// column order with fp32 accumulation, as the SELL SpMV sums it; dval holds
+// an fp16 NaN for those rows.  M = Z W^T x runs as two launches:
+//   k_b_mw: t = W^T x for every row (tile blocks: diagonal rows; row blocks:
+//           the CSR rows), and y = Z t for the rows where Z is diagonal;
+//   k_b_mz: y = Z t for the CSR rows of Z.  They read t of other rows: t of
+//           a diagonal row of W is recomputed from x, t of a CSR row of W
+//           is read back (k_b_mw stores t for those rows only).
+// Each row's products are those of applying W^T, then Z, one at a time.
 struct Factor16 {
-  DevPtr dval, rows, rptr, col, val;
+  DevPtr dval, rows, rseg, rptr, col, val;  // rseg: segment of each CSR row
   int32_t nr = 0;
 };
+
+__device__ __forceinline__ bool is_nan16(half_t v) {
+  return (__builtin_bit_cast(uint16_t, v) & 0x7fffu) > 0x7c00u;
+}
+
+// input x_i of M: fp16 x, or the level-4 entry fp16(in * scale[seg]) computed
+// here (and stored into hin by the tile blocks)
+struct MIn {
+  const half_t *x;
+  const float *in;
+  const float *sc;
+  half_t *hin;
+  __device__ __forceinline__ half_t get(int64_t i, int seg) const {
+    return in ? cvt<half_t>(to_acc(in[i]) * sc[seg]) : x[i];
+  }
+};
+
+// what becomes of y_i = (M x)_i: 0 stored; 1 scaled by the segment's ome_0
+// and stored; 2 the Richardson update fp16(fma(ome_k, y_i, ho_i)) stored;
+// 3 that update converted out of the fp16 level, out_i = fp32(.) * iscale
+struct MOut {
+  int mode;
+  half_t *y;
+  const RichSeg *rs;
+  int k;
+  const half_t *ho;
+  float *out;
+  const float *iscale;
+  // the segment's weight (modes 1-3)
+  __device__ __forceinline__ float weight(int seg) const {
+    return mode == 0 ? 0.f : (float)rs[seg].ome[mode == 1 ? 0 : k];
+  }
+  // hov = ho[i] (modes 2, 3), w = weight(seg)
+  __device__ __forceinline__ void put(int64_t i, int seg, half_t v, half_t hov, float w) const {
+    if (mode == 0) {
+      y[i] = v;
+    } else if (mode == 1) {
+      y[i] = cvt<half_t>(to_acc(v) * w);
+    } else {
+      const half_t h = cvt<half_t>(fma_<float>(w, to_acc(v), to_acc(hov)));
+      if (mode == 2)
+        y[i] = h;
+      else
+        out[i] = cvt<float>(to_acc(h) * iscale[seg]);
+    }
+  }
+  __device__ __forceinline__ void put(int64_t i, int seg, half_t v) const {
+    put(i, seg, v, mode >= 2 ? ho[i] : (half_t)0, weight(seg));
+  }
+};
+
+struct FacArgs {
+  const half_t *__restrict__ d;
+  const int32_t *__restrict__ rows;
+  const int32_t *__restrict__ rseg;
+  const int32_t *__restrict__ rptr;
+  const int32_t *__restrict__ col;
+  const half_t *__restrict__ val;
+  int32_t nr;
+};
+
+__global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_t *__restrict__ zd,
+                                                 half_t *__restrict__ t, int G, const int32_t *__restrict__ bseg,
+                                                 MOut o) {
+  if ((int)blockIdx.x < G) {
+    // every load of the thread's rows first, then the stores (a branch
+    // between a row's loads would keep one row's loads in flight at a time)
+    const int seg = bseg[blockIdx.x];
+    const float w = o.weight(seg);
+    half_t xi[kRows], dw[kRows], dz[kRows], hov[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      const int64_t i = tile_row(q);
+      xi[q] = xin.get(i, seg);
+      dw[q] = W.d[i];
+      dz[q] = zd[i];
+      hov[q] = o.mode >= 2 ? o.ho[i] : (half_t)0;
+    }
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      const int64_t i = tile_row(q);
+      if (xin.in) xin.hin[i] = xi[q];
+      if (is_nan16(dw[q])) continue;  // a CSR row of W: the row blocks
+      const half_t ti = cvt<half_t>(to_acc(dw[q]) * to_acc(xi[q]));
+      if (!is_nan16(dz[q])) o.put(i, seg, cvt<half_t>(to_acc(dz[q]) * to_acc(ti)), hov[q], w);
+    }
+    return;
+  }
+  const int32_t r = (int32_t)(blockIdx.x - G) * kBlock + threadIdx.x;
+  if (r >= W.nr) return;
+  const int64_t i = W.rows[r];
+  const int seg = W.rseg[r];
+  float acc = 0.f;
+  for (int32_t k = W.rptr[r]; k < W.rptr[r + 1]; ++k)
+    acc = fma_<float>(to_acc(W.val[k]), to_acc(xin.get(W.col[k], seg)), acc);
+  const half_t ti = cvt<half_t>(acc);
+  t[i] = ti;
+  const half_t dz = zd[i];
+  if (!is_nan16(dz)) o.put(i, seg, cvt<half_t>(to_acc(dz) * to_acc(ti)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_mz(FacArgs Z, MIn xin, const half_t *__restrict__ wd,
+                                                 const half_t *__restrict__ t, MOut o) {
+  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= Z.nr) return;
+  const int seg = Z.rseg[r];
+  float acc = 0.f;
+  for (int32_t k = Z.rptr[r]; k < Z.rptr[r + 1]; ++k) {
+    const int32_t c = Z.col[k];
+    const half_t dw = wd[c];
+    const half_t tc = is_nan16(dw) ? t[c] : cvt<half_t>(to_acc(dw) * to_acc(xin.get(c, seg)));
+    acc = fma_<float>(to_acc(Z.val[k]), to_acc(tc), acc);
+  }
+  o.put(Z.rows[r], seg, cvt<half_t>(acc));
+}
 
 // ----------------------------------------------------------- solver ---
 template <class T> struct BLevel {
@@ -556,18 +661,26 @@ struct BatchSolver {
                        (const T *)L.inv->template as<T>(), (const int32_t *)bs());
   }
 
-  void apply_factor(const Factor16 &F, const half_t *x, half_t *y) {
-    hipLaunchKernelGGL(k_b_fdiag, dim3(G), dim3(kBlock), 0, st(), x, (const half_t *)F.dval->as<half_t>(), y);
-    if (F.nr > 0)
-      hipLaunchKernelGGL(k_b_frows, dim3((F.nr + kBlock - 1) / kBlock), dim3(kBlock), 0, st(), F.nr,
-                         (const int32_t *)F.rows->as<int32_t>(), (const int32_t *)F.rptr->as<int32_t>(),
-                         (const int32_t *)F.col->as<int32_t>(), (const half_t *)F.val->as<half_t>(), x, y);
+  static FacArgs fac(const Factor16 &F) {
+    return {F.dval->as<half_t>(), F.rows->as<int32_t>(), F.rseg->as<int32_t>(), F.rptr->as<int32_t>(),
+            F.col->as<int32_t>(), F.val->as<half_t>(), F.nr};
   }
-  // M = Z W^T (ainv.hpp:110-114): W^T first, then Z
-  void precond(const half_t *in, half_t *out) {
+  // y = M x with M = Z W^T (ainv.hpp:110-114): W^T first, then Z; x given
+  // by `xin`, y consumed by `o`
+  void precond(const MIn &xin, const MOut &o) {
     half_t *t = tmp->as<half_t>();
-    apply_factor(W, in, t);
-    apply_factor(Z, (const half_t *)t, out);
+    const int gw = (W.nr + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_b_mw, dim3(G + gw), dim3(kBlock), 0, st(), xin, fac(W), (const half_t *)Z.dval->as<half_t>(),
+                       t, G, (const int32_t *)bs(), o);
+    if (Z.nr > 0) {
+      // k_b_mz reads x as k_b_mw does (in the entry call: fp16(in * s) from
+      // the fp32 input)
+      hipLaunchKernelGGL(k_b_mz, dim3((Z.nr + kBlock - 1) / kBlock), dim3(kBlock), 0, st(), fac(Z), xin,
+                         (const half_t *)W.dval->as<half_t>(), (const half_t *)t, o);
+    }
+  }
+  void precond(const half_t *in, half_t *out) {
+    precond(MIn{in, nullptr, nullptr, nullptr}, MOut{0, out, nullptr, 0, nullptr, nullptr, nullptr});
   }
 
   void probe(const half_t *x, const half_t *r, int k) {
@@ -588,22 +701,36 @@ struct BatchSolver {
     half_t *hi = hin->as<half_t>(), *ho = hout->as<half_t>(), *ho2 = hout2->as<half_t>();
     half_t *hr_ = hr->as<half_t>(), *tw = t2->as<half_t>();
     RichSeg *rs = rich->as<RichSeg>();
-    hipLaunchKernelGGL((k_b_entry<float, half_t>), dim3(G), dim3(kBlock), 0, st(), in, hi,
-                       (const float *)scale->as<float>(), (const int32_t *)bs());
-    precond(hi, ho);
-    if (pr) probe(ho, hi, 0);
-    hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, pr ? 1 : 0,
-                       (const int32_t *)bs());
+    // entry hin = fp16(in * s) fused into the first application of M; off
+    // probe calls the weighting by ome_0 is fused too
+    const float *isc = iscale->as<float>();
+    precond(MIn{nullptr, in, scale->as<float>(), hi}, MOut{pr ? 0 : 1, ho, rs, 0, nullptr, nullptr, nullptr});
+    if (pr) {
+      probe(ho, hi, 0);
+      hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, 1,
+                         (const int32_t *)bs());
+    }
+    bool done = false;  // the last step already wrote `out`
     for (int k = 1; k < p.m4; ++k) {
+      const bool last = k == p.m4 - 1;
       spmv<half_t, half_t, half_t, float>(A16->as<half_t>(), true, (const half_t *)ho, hr_, (const half_t *)hi);
-      precond(hr_, tw);
-      if (pr) probe(tw, hr_, k);
-      hipLaunchKernelGGL(k_b_l4_update, dim3(G), dim3(kBlock), 0, st(), (const half_t *)tw, (const half_t *)ho, ho2,
-                         (const RichSeg *)rs, k, pr ? 1 : 0, (const int32_t *)bs());
+      if (pr) {
+        precond(hr_, tw);
+        probe(tw, hr_, k);
+        hipLaunchKernelGGL(k_b_l4_update, dim3(G), dim3(kBlock), 0, st(), (const half_t *)tw, (const half_t *)ho,
+                           ho2, (const RichSeg *)rs, k, 1, (const int32_t *)bs());
+      } else {
+        // the weighted update (and on the last step the way out of the
+        // fp16 level) fused into M
+        precond(MIn{hr_, nullptr, nullptr, nullptr},
+                MOut{last ? 3 : 2, ho2, rs, k, (const half_t *)ho, out, isc});
+        done = last;
+      }
       std::swap(ho, ho2);
     }
-    hipLaunchKernelGGL((k_b_entry<half_t, float>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)ho, out,
-                       (const float *)iscale->as<float>(), (const int32_t *)bs());
+    if (!done)
+      hipLaunchKernelGGL((k_b_entry<half_t, float>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)ho, out, isc,
+                         (const int32_t *)bs());
   }
 
   // FGMRES(m3) on the fp16 values, fp32 basis, normalize = false
@@ -866,6 +993,7 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
       if (e - b == 1 && M.idx[b] == i) {
         d[i] = M.val[b];
       } else if (e > b) {
+        d[i] = std::numeric_limits<double>::quiet_NaN();  // marks a CSR row
         rows.push_back(i);
         for (int32_t k = b; k < e; ++k) {
           cl.push_back(M.idx[k]);
@@ -874,10 +1002,14 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
         rp.push_back((int32_t)cl.size());
       }
     }
+    std::vector<int32_t> rseg(rows.size());
+    for (size_t q = 0; q < rows.size(); ++q)
+      rseg[q] = (int32_t)(std::upper_bound(B->off.begin(), B->off.end(), (int64_t)rows[q]) - B->off.begin() - 1);
     Factor16 F;
     F.dval = upload_values(c, d, F3R_F16);
     F.nr = (int32_t)rows.size();
     F.rows = up32(rows);
+    F.rseg = up32(rseg);
     F.rptr = up32(rp);
     F.col = up32(cl);
     F.val = upload_values(c, vl, F3R_F16);
