@@ -91,6 +91,13 @@ constexpr int kRows = 8;
 constexpr int64_t kTile = (int64_t)kBlock * kRows;
 
 __device__ __forceinline__ int64_t tile_row(int q) { return (int64_t)blockIdx.x * kTile + threadIdx.x + (int64_t)q * kBlock; }
+// the same tile as kRows / 4 groups of 4 consecutive rows per thread (one
+// 8- or 16-byte access per vector and group; segments and vectors start on
+// tile boundaries, so every group is aligned)
+constexpr int kQuads = kRows / 4;
+__device__ __forceinline__ int64_t tile_quad(int q) {
+  return (int64_t)blockIdx.x * kTile + ((int64_t)threadIdx.x + (int64_t)q * kBlock) * 4;
+}
 
 // block partial sums of <w, V_k> for the NV columns of blockIdx.y (k < ncols):
 // partial[k * gridDim.x + block]
@@ -102,12 +109,16 @@ __global__ __launch_bounds__(kBlock) void k_b_mdot(const S *__restrict__ w, cons
   A v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = (A)0;
-  for (int q = 0; q < kRows; ++q) {
-    const int64_t i = tile_row(q);
-    const A wi = (A)to_acc(w[i]);
+  for (int q = 0; q < kQuads; ++q) {
+    const int64_t i = tile_quad(q);
+    const V4<S> wq = ld4(w + i);
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      if (k0 + k < ncols) v[k] = fma_<A>(wi, (A)to_acc(V[(int64_t)(k0 + k) * ld + i]), v[k]);
+      if (k0 + k < ncols) {
+        const V4<S> vq = ld4(V + (int64_t)(k0 + k) * ld + i);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[k] = fma_<A>((A)to_acc(wq.v[c]), (A)to_acc(vq.v[c]), v[k]);
+      }
   }
   block_sum<A, NV>(v, lds);
   if (threadIdx.x == 0)
@@ -141,14 +152,25 @@ __global__ __launch_bounds__(kBlock) void k_b_update(S *__restrict__ w, const S 
   for (int k = threadIdx.x; k < ncols; k += kBlock) h[k] = H[(int64_t)seg * hs + k];
   __syncthreads();
   A v[1] = {(A)0};
-  for (int q = 0; q < kRows; ++q) {
-    const int64_t i = tile_row(q);
-    A r = (A)to_acc(V[i]) * h[0];
-    for (int k = 1; k < ncols; ++k) r = fma_<A>((A)to_acc(V[(int64_t)k * ld + i]), h[k], r);
-    const S ws = cvt<S>((A)to_acc(w[i]) - r);
-    w[i] = ws;
-    const A wf = (A)to_acc(ws);
-    v[0] = fma_<A>(wf, wf, v[0]);
+  for (int q = 0; q < kQuads; ++q) {
+    const int64_t i = tile_quad(q);
+    const V4<S> v0 = ld4(V + i);
+    V4<S> wq = ld4(w + i);
+    A r[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r[c] = (A)to_acc(v0.v[c]) * h[0];
+    for (int k = 1; k < ncols; ++k) {
+      const V4<S> vk = ld4(V + (int64_t)k * ld + i);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(vk.v[c]), h[k], r[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      wq.v[c] = cvt<S>((A)to_acc(wq.v[c]) - r[c]);
+      const A wf = (A)to_acc(wq.v[c]);
+      v[0] = fma_<A>(wf, wf, v[0]);
+    }
+    st4(w + i, wq);
   }
   block_sum<A, 1>(v, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
@@ -201,9 +223,12 @@ template <class S, class A>
 __global__ __launch_bounds__(kBlock) void k_b_vscale(S *__restrict__ v, const A *__restrict__ inv,
                                                      const int32_t *__restrict__ bseg) {
   const A a = inv[bseg[blockIdx.x]];
-  for (int q = 0; q < kRows; ++q) {
-    const int64_t i = tile_row(q);
-    v[i] = cvt<S>((A)to_acc(v[i]) * a);
+  for (int q = 0; q < kQuads; ++q) {
+    const int64_t i = tile_quad(q);
+    V4<S> x = ld4(v + i);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x.v[c] = cvt<S>((A)to_acc(x.v[c]) * a);
+    st4(v + i, x);
   }
 }
 
@@ -224,14 +249,27 @@ __global__ __launch_bounds__(kBlock) void k_b_combine(const S *__restrict__ Z, i
   }
   if (threadIdx.x == 0) hess_solve_guarded<A>(H + (int64_t)seg * hs, ldh, e + (int64_t)seg * es, y, j);
   __syncthreads();
-  for (int q = 0; q < kRows; ++q) {
-    const int64_t i = tile_row(q);
-    A r = (A)to_acc(Z[i]) * y[0];
-    for (int k = 1; k < j; ++k) r = fma_<A>((A)to_acc(Z[(int64_t)k * ld + i]), y[k], r);
-    if (accumulate)
-      out[i] = cvt<TO>((A)to_acc(out[i]) + r);
-    else
-      out[i] = cvt<TO>(r);
+  for (int q = 0; q < kQuads; ++q) {
+    const int64_t i = tile_quad(q);
+    const V4<S> z0 = ld4(Z + i);
+    A r[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r[c] = (A)to_acc(z0.v[c]) * y[0];
+    for (int k = 1; k < j; ++k) {
+      const V4<S> zk = ld4(Z + (int64_t)k * ld + i);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(zk.v[c]), y[k], r[c]);
+    }
+    V4<TO> o;
+    if (accumulate) {
+      o = ld4(out + i);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o.v[c] = cvt<TO>((A)to_acc(o.v[c]) + r[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o.v[c] = cvt<TO>(r[c]);
+    }
+    st4(out + i, o);
   }
 }
 
@@ -459,6 +497,26 @@ struct MOut {
   __device__ __forceinline__ void put(int64_t i, int seg, half_t v) const {
     put(i, seg, v, mode >= 2 ? ho[i] : (half_t)0, weight(seg));
   }
+  // four consecutive rows (i aligned to 4): the same values as put() per row
+  __device__ __forceinline__ void put4(int64_t i, int seg, const V4<half_t> &v, const V4<half_t> &hov,
+                                       float w) const {
+    if (mode <= 2) {
+      V4<half_t> r;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        r.v[c] = mode == 0   ? v.v[c]
+                 : mode == 1 ? cvt<half_t>(to_acc(v.v[c]) * w)
+                             : cvt<half_t>(fma_<float>(w, to_acc(v.v[c]), to_acc(hov.v[c])));
+      st4(y + i, r);
+    } else {
+      const float is = iscale[seg];
+      V4<float> r;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        r.v[c] = cvt<float>(to_acc(cvt<half_t>(fma_<float>(w, to_acc(v.v[c]), to_acc(hov.v[c])))) * is);
+      st4(out + i, r);
+    }
+  }
 };
 
 struct FacArgs {
@@ -479,22 +537,42 @@ __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_
     // between a row's loads would keep one row's loads in flight at a time)
     const int seg = bseg[blockIdx.x];
     const float w = o.weight(seg);
-    half_t xi[kRows], dw[kRows], dz[kRows], hov[kRows];
+    const float sc = xin.in ? xin.sc[seg] : 0.f;
+    V4<half_t> xi[kQuads], dw[kQuads], dz[kQuads], hov[kQuads];
 #pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      const int64_t i = tile_row(q);
-      xi[q] = xin.get(i, seg);
-      dw[q] = W.d[i];
-      dz[q] = zd[i];
-      hov[q] = o.mode >= 2 ? o.ho[i] : (half_t)0;
+    for (int q = 0; q < kQuads; ++q) {
+      const int64_t i = tile_quad(q);
+      if (xin.in) {
+        const V4<float> f = ld4(xin.in + i);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xi[q].v[c] = cvt<half_t>(f.v[c] * sc);
+      } else {
+        xi[q] = ld4(xin.x + i);
+      }
+      dw[q] = ld4(W.d + i);
+      dz[q] = ld4(zd + i);
+      if (o.mode >= 2) hov[q] = ld4(o.ho + i);
     }
 #pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      const int64_t i = tile_row(q);
-      if (xin.in) xin.hin[i] = xi[q];
-      if (is_nan16(dw[q])) continue;  // a CSR row of W: the row blocks
-      const half_t ti = cvt<half_t>(to_acc(dw[q]) * to_acc(xi[q]));
-      if (!is_nan16(dz[q])) o.put(i, seg, cvt<half_t>(to_acc(dz[q]) * to_acc(ti)), hov[q], w);
+    for (int q = 0; q < kQuads; ++q) {
+      const int64_t i = tile_quad(q);
+      if (xin.in) st4(xin.hin + i, xi[q]);
+      bool plain = true;  // all four rows diagonal in W and Z
+      V4<half_t> yv;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        plain = plain && !is_nan16(dw[q].v[c]) && !is_nan16(dz[q].v[c]);
+        const half_t ti = cvt<half_t>(to_acc(dw[q].v[c]) * to_acc(xi[q].v[c]));
+        yv.v[c] = cvt<half_t>(to_acc(dz[q].v[c]) * to_acc(ti));
+      }
+      if (plain) {
+        o.put4(i, seg, yv, o.mode >= 2 ? hov[q] : yv, w);
+        continue;
+      }
+      // a CSR row of W is the row blocks' (and a CSR row of Z k_b_mz's)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (!is_nan16(dw[q].v[c]) && !is_nan16(dz[q].v[c])) o.put(i + c, seg, yv.v[c], hov[q].v[c], w);
     }
     return;
   }

@@ -127,6 +127,20 @@ __device__ __forceinline__ Pack<T, K> load_pack(const T *__restrict__ p) {
   return r;
 }
 
+// four consecutive elements, loaded and stored as one 8- or 16-byte access
+// (p aligned to 4 elements)
+template <class T> struct V4 {
+  T v[4];
+};
+template <class T> __device__ __forceinline__ V4<T> ld4(const T *p) {
+  V4<T> r;
+  __builtin_memcpy(&r, __builtin_assume_aligned(p, 4 * sizeof(T)), sizeof(r));
+  return r;
+}
+template <class T> __device__ __forceinline__ void st4(T *p, const V4<T> &r) {
+  __builtin_memcpy(__builtin_assume_aligned(p, 4 * sizeof(T)), &r, sizeof(r));
+}
+
 // --------------------------------------------------------- reductions ----
 template <class T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -28,17 +28,6 @@
 
 namespace f3r {
 
-template <class T> struct V4 {
-  T v[4];
-};
-template <class T> __device__ __forceinline__ V4<T> ld4(const T *p) {
-  V4<T> r;
-  __builtin_memcpy(&r, __builtin_assume_aligned(p, 4 * sizeof(T)), sizeof(r));
-  return r;
-}
-template <class T> __device__ __forceinline__ void st4(T *p, const V4<T> &r) {
-  __builtin_memcpy(__builtin_assume_aligned(p, 4 * sizeof(T)), &r, sizeof(r));
-}
 // true when p is aligned to 4 elements (the vectorised kernels need it)
 template <class T> inline bool aligned4(const T *p) { return ((uintptr_t)p % (4 * sizeof(T))) == 0; }
 
