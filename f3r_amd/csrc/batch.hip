@@ -101,11 +101,27 @@ __device__ __forceinline__ int64_t tile_quad(int q) {
 
 // block partial sums of <w, V_k> for the NV columns of blockIdx.y (k < ncols):
 // partial[k * gridDim.x + block]
+// Basis columns: column 0 at V0, column k >= 1 at V + k * ld, its values
+// scaled by csc[k * nsegs + seg] when csc is given (a column left
+// unnormalised in memory; the fp32 product is what a normalised copy holds)
+template <class S> __device__ __forceinline__ const S *bcol(const S *V0, const S *V, int64_t ld, int k) {
+  return k == 0 ? V0 : V + (int64_t)k * ld;
+}
+template <class A> __device__ __forceinline__ A bscale(const A *csc, int nsegs, int k, int seg) {
+  return csc && k > 0 ? csc[(int64_t)k * nsegs + seg] : (A)1;
+}
+
 template <class S, class A, int NV>
-__global__ __launch_bounds__(kBlock) void k_b_mdot(const S *__restrict__ w, const S *__restrict__ V, int64_t ld,
-                                                   int ncols, A *__restrict__ partial) {
+__global__ __launch_bounds__(kBlock) void k_b_mdot(const S *__restrict__ w, const S *__restrict__ V0,
+                                                   const S *__restrict__ V, int64_t ld, int ncols,
+                                                   const A *__restrict__ csc, int nsegs,
+                                                   const int32_t *__restrict__ bseg, A *__restrict__ partial) {
   __shared__ A lds[NV * kWavesPerBlock];
   const int k0 = blockIdx.y * NV;
+  const int seg = bseg[blockIdx.x];
+  A sk[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sk[k] = k0 + k < ncols ? bscale(csc, nsegs, k0 + k, seg) : (A)1;
   A v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = (A)0;
@@ -115,9 +131,9 @@ __global__ __launch_bounds__(kBlock) void k_b_mdot(const S *__restrict__ w, cons
 #pragma unroll
     for (int k = 0; k < NV; ++k)
       if (k0 + k < ncols) {
-        const V4<S> vq = ld4(V + (int64_t)(k0 + k) * ld + i);
+        const V4<S> vq = ld4(bcol(V0, V, ld, k0 + k) + i);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) v[k] = fma_<A>((A)to_acc(wq.v[c]), (A)to_acc(vq.v[c]), v[k]);
+        for (int c = 0; c < 4; ++c) v[k] = fma_<A>((A)to_acc(wq.v[c]), (A)to_acc(vq.v[c]) * sk[k], v[k]);
       }
   }
   block_sum<A, NV>(v, lds);
@@ -143,26 +159,30 @@ __global__ __launch_bounds__(kBlock) void k_b_dots_finish(const A *__restrict__ 
 // w = S(w - V h) with h of the block's segment (MULMV order); block partial
 // of ||w||^2 of the stored w
 template <class S, class A>
-__global__ __launch_bounds__(kBlock) void k_b_update(S *__restrict__ w, const S *__restrict__ V, int64_t ld, int ncols,
-                                                     const A *H, int64_t hs, const int32_t *__restrict__ bseg,
-                                                     A *__restrict__ partial) {
-  __shared__ A h[128];
+__global__ __launch_bounds__(kBlock) void k_b_update(S *__restrict__ w, const S *__restrict__ V0,
+                                                     const S *__restrict__ V, int64_t ld, int ncols,
+                                                     const A *__restrict__ csc, int nsegs, const A *H, int64_t hs,
+                                                     const int32_t *__restrict__ bseg, A *__restrict__ partial) {
+  __shared__ A h[128], sk[128];
   __shared__ A lds[kWavesPerBlock];
   const int seg = bseg[blockIdx.x];
-  for (int k = threadIdx.x; k < ncols; k += kBlock) h[k] = H[(int64_t)seg * hs + k];
+  for (int k = threadIdx.x; k < ncols; k += kBlock) {
+    h[k] = H[(int64_t)seg * hs + k];
+    sk[k] = bscale(csc, nsegs, k, seg);
+  }
   __syncthreads();
   A v[1] = {(A)0};
   for (int q = 0; q < kQuads; ++q) {
     const int64_t i = tile_quad(q);
-    const V4<S> v0 = ld4(V + i);
+    const V4<S> v0 = ld4(V0 + i);
     V4<S> wq = ld4(w + i);
     A r[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) r[c] = (A)to_acc(v0.v[c]) * h[0];
+    for (int c = 0; c < 4; ++c) r[c] = ((A)to_acc(v0.v[c]) * sk[0]) * h[0];
     for (int k = 1; k < ncols; ++k) {
       const V4<S> vk = ld4(V + (int64_t)k * ld + i);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(vk.v[c]), h[k], r[c]);
+      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(vk.v[c]) * sk[k], h[k], r[c]);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -460,8 +480,10 @@ struct MIn {
   const float *in;
   const float *sc;
   half_t *hin;
+  const float *inv;  // `in` is an unnormalised basis column: in * inv[seg] (null: 1)
+  __device__ __forceinline__ float iv(int seg) const { return inv ? inv[seg] : 1.f; }
   __device__ __forceinline__ half_t get(int64_t i, int seg) const {
-    return in ? cvt<half_t>(to_acc(in[i]) * sc[seg]) : x[i];
+    return in ? cvt<half_t>((to_acc(in[i]) * iv(seg)) * sc[seg]) : x[i];
   }
 };
 
@@ -537,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_
     // between a row's loads would keep one row's loads in flight at a time)
     const int seg = bseg[blockIdx.x];
     const float w = o.weight(seg);
-    const float sc = xin.in ? xin.sc[seg] : 0.f;
+    const float sc = xin.in ? xin.sc[seg] : 0.f, iv = xin.iv(seg);
     V4<half_t> xi[kQuads], dw[kQuads], dz[kQuads], hov[kQuads];
 #pragma unroll
     for (int q = 0; q < kQuads; ++q) {
@@ -545,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_
       if (xin.in) {
         const V4<float> f = ld4(xin.in + i);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) xi[q].v[c] = cvt<half_t>(f.v[c] * sc);
+        for (int c = 0; c < 4; ++c) xi[q].v[c] = cvt<half_t>((f.v[c] * iv) * sc);
       } else {
         xi[q] = ld4(xin.x + i);
       }
@@ -608,6 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_b_mz(FacArgs Z, MIn xin, const half_
 template <class T> struct BLevel {
   int m = 0;
   DevPtr V, Z, H, cs, sn, e, inv;  // V, Z column-major (ld = N); per segment H, cs, sn, e, inv
+  DevPtr csc;  // per column and segment 1/||w|| of columns left unnormalised (level 3)
   T *Vc(int64_t N, int k) const { return V->as<T>() + (int64_t)k * N; }
   T *Zc(int64_t N, int k) const { return Z->as<T>() + (int64_t)k * N; }
   int64_t hs() const { return (int64_t)(m + 1) * m; }
@@ -709,32 +732,39 @@ struct BatchSolver {
     L.sn = dev_alloc(sizeof(T) * nseg * m);
     L.e = dev_alloc(sizeof(T) * nseg * (m + 1));
     L.inv = dev_alloc(sizeof(T) * nseg);
+    L.csc = dev_alloc(sizeof(T) * nseg * (m + 1));
   }
 
-  // one Arnoldi step j of level L (w = V_{j+1} already holds A z_j)
-  template <class T> void arnoldi(BLevel<T> &L, int j, int e0_one, double *stat) {
+  // one Arnoldi step j of level L (w = V_{j+1} already holds A z_j).  V0:
+  // column 0 (the level's input, not copied).  lazy: V_{j+1} stays
+  // unnormalised, its 1/||w|| per segment goes to csc column j + 1 and every
+  // reader applies it (no scaling pass)
+  template <class T> void arnoldi(BLevel<T> &L, int j, int e0_one, double *stat, const T *V0, bool lazy) {
     const int ncols = j + 1;
     T *H = L.H->template as<T>();
     T *part = partial->as<T>();
+    const T *csc = lazy ? L.csc->template as<T>() : nullptr;
+    const T *Vb = L.Vc(N, 0);
     if (ncols <= 4)
-      hipLaunchKernelGGL((k_b_mdot<T, T, 4>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1),
-                         (const T *)L.Vc(N, 0), N, ncols, part);
+      hipLaunchKernelGGL((k_b_mdot<T, T, 4>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1), V0, Vb,
+                         N, ncols, csc, nseg, (const int32_t *)bs(), part);
     else if (ncols <= 8)
-      hipLaunchKernelGGL((k_b_mdot<T, T, 8>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1),
-                         (const T *)L.Vc(N, 0), N, ncols, part);
+      hipLaunchKernelGGL((k_b_mdot<T, T, 8>), dim3(G, 1), dim3(kBlock), 0, st(), (const T *)L.Vc(N, j + 1), V0, Vb,
+                         N, ncols, csc, nseg, (const int32_t *)bs(), part);
     else
       hipLaunchKernelGGL((k_b_mdot<T, T, 16>), dim3(G, (ncols + 15) / 16), dim3(kBlock), 0, st(),
-                         (const T *)L.Vc(N, j + 1), (const T *)L.Vc(N, 0), N, ncols, part);
+                         (const T *)L.Vc(N, j + 1), V0, Vb, N, ncols, csc, nseg, (const int32_t *)bs(), part);
     hipLaunchKernelGGL(k_b_dots_finish<T>, dim3(nseg, ncols), dim3(kBlock), 0, st(), (const T *)part, G,
                        (const int32_t *)sb0->as<int32_t>(), (const int32_t *)snb->as<int32_t>(),
                        H + (int64_t)j * (L.m + 1), L.hs());
-    hipLaunchKernelGGL((k_b_update<T, T>), dim3(G), dim3(kBlock), 0, st(), L.Vc(N, j + 1), (const T *)L.Vc(N, 0), N,
-                       ncols, (const T *)(H + (int64_t)j * (L.m + 1)), L.hs(), (const int32_t *)bs(), part);
+    hipLaunchKernelGGL((k_b_update<T, T>), dim3(G), dim3(kBlock), 0, st(), L.Vc(N, j + 1), V0, Vb, N, ncols, csc,
+                       nseg, (const T *)(H + (int64_t)j * (L.m + 1)), L.hs(), (const int32_t *)bs(), part);
     hipLaunchKernelGGL(k_b_norm_finish<T>, dim3(nseg), dim3(kBlock), 0, st(), (const T *)part,
                        (const int32_t *)sb0->as<int32_t>(), (const int32_t *)snb->as<int32_t>(),
                        H + (int64_t)j * (L.m + 1), L.hs(), L.cs->template as<T>(), L.sn->template as<T>(),
-                       (int64_t)L.m, L.e->template as<T>(), (int64_t)(L.m + 1), j, e0_one, L.inv->template as<T>(),
-                       stat);
+                       (int64_t)L.m, L.e->template as<T>(), (int64_t)(L.m + 1), j, e0_one,
+                       lazy ? L.csc->template as<T>() + (int64_t)(j + 1) * nseg : L.inv->template as<T>(), stat);
+    if (lazy) return;
     hipLaunchKernelGGL((k_b_vscale<T, T>), dim3(G), dim3(kBlock), 0, st(), L.Vc(N, j + 1),
                        (const T *)L.inv->template as<T>(), (const int32_t *)bs());
   }
@@ -758,7 +788,7 @@ struct BatchSolver {
     }
   }
   void precond(const half_t *in, half_t *out) {
-    precond(MIn{in, nullptr, nullptr, nullptr}, MOut{0, out, nullptr, 0, nullptr, nullptr, nullptr});
+    precond(MIn{in, nullptr, nullptr, nullptr, nullptr}, MOut{0, out, nullptr, 0, nullptr, nullptr, nullptr});
   }
 
   void probe(const half_t *x, const half_t *r, int k) {
@@ -772,7 +802,8 @@ struct BatchSolver {
   }
 
   // adaptive Richardson in fp16 (work/src/f3r-gpu.cpp:229-260), per segment
-  void level4(const float *in, float *out) {
+  // in_inv: per-segment 1/||w|| of an unnormalised input column (null: 1)
+  void level4(const float *in, const float *in_inv, float *out) {
     ++cnt;
     ++calls;
     const bool pr = (cnt % p.c) == 0;
@@ -782,7 +813,8 @@ struct BatchSolver {
     // entry hin = fp16(in * s) fused into the first application of M; off
     // probe calls the weighting by ome_0 is fused too
     const float *isc = iscale->as<float>();
-    precond(MIn{nullptr, in, scale->as<float>(), hi}, MOut{pr ? 0 : 1, ho, rs, 0, nullptr, nullptr, nullptr});
+    precond(MIn{nullptr, in, scale->as<float>(), hi, in_inv},
+            MOut{pr ? 0 : 1, ho, rs, 0, nullptr, nullptr, nullptr});
     if (pr) {
       probe(ho, hi, 0);
       hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, 1,
@@ -800,7 +832,7 @@ struct BatchSolver {
       } else {
         // the weighted update (and on the last step the way out of the
         // fp16 level) fused into M
-        precond(MIn{hr_, nullptr, nullptr, nullptr},
+        precond(MIn{hr_, nullptr, nullptr, nullptr, nullptr},
                 MOut{last ? 3 : 2, ho2, rs, k, (const half_t *)ho, out, isc});
         done = last;
       }
@@ -812,13 +844,16 @@ struct BatchSolver {
   }
 
   // FGMRES(m3) on the fp16 values, fp32 basis, normalize = false
+  // (V_0 is the level-2 column itself; the other columns stay unnormalised)
   void level3(const float *in, float *out) {
     BLevel<float> &L = L3;
-    F3R_HIP(hipMemcpyAsync(L.Vc(N, 0), in, sizeof(float) * N, hipMemcpyDeviceToDevice, st()));
     for (int j = 0; j < L.m; ++j) {
-      level4(L.Vc(N, j), L.Zc(N, j));
+      if (j == 0)
+        level4(in, nullptr, L.Zc(N, 0));
+      else
+        level4(L.Vc(N, j), L.csc->as<float>() + (int64_t)j * nseg, L.Zc(N, j));
       spmv<half_t, float, float, float>(A16->as<half_t>(), true, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
-      arnoldi(L, j, 1, nullptr);
+      arnoldi(L, j, 1, nullptr, in, true);
     }
     hipLaunchKernelGGL((k_b_combine<float, float, float>), dim3(G), dim3(kBlock), 0, st(), (const float *)L.Zc(N, 0), N,
                        (const float *)L.H->as<float>(), L.hs(), L.m + 1, (const float *)L.e->as<float>(),
@@ -832,7 +867,7 @@ struct BatchSolver {
     for (int j = 0; j < L.m; ++j) {
       level3(L.Vc(N, j), L.Zc(N, j));
       spmv<float, float, float, float>(A32->as<float>(), false, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
-      arnoldi(L, j, 1, nullptr);
+      arnoldi(L, j, 1, nullptr, (const float *)L.Vc(N, 0), false);
     }
     hipLaunchKernelGGL((k_b_combine<float, float, double>), dim3(G), dim3(kBlock), 0, st(),
                        (const float *)L.Zc(N, 0), N, (const float *)L.H->as<float>(), L.hs(), L.m + 1,
@@ -896,7 +931,7 @@ struct BatchSolver {
         level2(V + (int64_t)j * N, L.Zc(N, j));
         spmv<double, double, double, double>(A64->as<double>(), false, (const double *)L.Zc(N, j),
                                              V + (int64_t)(j + 1) * N, nullptr);
-        arnoldi(L, j, 0, status->as<double>());
+        arnoldi(L, j, 0, status->as<double>(), (const double *)L.Vc(N, 0), false);
         std::vector<double> st_h = read_status();
         bool left = false;
         for (int s = 0; s < nseg; ++s) {
