@@ -384,9 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_b_l4_update(const half_t *__restrict
 // lane takes a matrix row and keeps NR sums, so each value and column index
 // is read once for all copies; every sum runs in slot order, as in k_sell.
 // Epilogue: y = acc (b null) or y = b - acc, at the vector rows.
+// Offset chunks gather x through a buffer resource over x's nx elements: the
+// per-lane byte offset is the row's, the chunk offset and the copy's stride
+// ride in the scalar offset (no per-lane address arithmetic per gather).
 template <class V, class X, class ACC, int K, int NR, class Y>
 __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restrict__ val, const X *__restrict__ x,
-                                                    const int32_t *__restrict__ cshift,
+                                                    int64_t nx, const int32_t *__restrict__ cshift,
                                                     const int32_t *__restrict__ cstride, Y *__restrict__ y,
                                                     const Y *__restrict__ b) {
   typedef const __attribute__((address_space(4))) int32_t *const_i32;
@@ -409,27 +412,41 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
     const int32_t start = a.iptr ? a.iptr[su] : 0;
     if (a.iptr && start < 0) {
       const const_i32 off = (const_i32)(a.coff + (-start - 1));
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)((uintptr_t)x - (uintptr_t)a.off_bias * sizeof(X)), (short)0,
+          (int)((nx + a.off_bias) * (int64_t)sizeof(X)), kBufferWord3);
+      const int vo = (int)(vr * (int64_t)sizeof(X));
+      const int bias = a.off_bias, cst = stride * (int)sizeof(X);
+#pragma unroll 2
       for (int g = 0; g < groups; ++g, pos += step) {
         const Pack<V, K> pv = load_pack<V, K>(val + pos);
+        X xv[K][NR];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const int32_t o = off[g * K + k];
+          const int so = (off[g * K + k] + bias) * (int)sizeof(X);
 #pragma unroll
-          for (int c = 0; c < NR; ++c)
-            acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[vr + (int64_t)c * stride + o]), acc[c]);
+          for (int c = 0; c < NR; ++c) xv[k][c] = buffer_gather<X>(rs, vo, so + c * cst);
         }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k][c]), acc[c]);
       }
       if constexpr (K >= 2) {
         if (cw & kHalfTail) {  // the half group (K / 2 slots per row, rows consecutive)
           constexpr int H = K / 2;
           const Pack<V, H> pv = load_pack<V, H>(val + ((int64_t)a.cptr[su] + (int64_t)groups * step + (int64_t)j * H));
+          X xv[H][NR];
 #pragma unroll
           for (int k = 0; k < H; ++k) {
-            const int32_t o = off[groups * K + k];
+            const int so = (off[groups * K + k] + bias) * (int)sizeof(X);
 #pragma unroll
-            for (int c = 0; c < NR; ++c)
-              acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(x[vr + (int64_t)c * stride + o]), acc[c]);
+            for (int c = 0; c < NR; ++c) xv[k][c] = buffer_gather<X>(rs, vo, so + c * cst);
           }
+#pragma unroll
+          for (int k = 0; k < H; ++k)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xv[k][c]), acc[c]);
         }
       }
     } else {
@@ -707,7 +724,8 @@ struct BatchSolver {
     const dim3 g((a.n + kBlock - 1) / kBlock);
     auto go = [&](auto kt, auto nt) {
       constexpr int K = decltype(kt)::value, R = decltype(nt)::value;
-      hipLaunchKernelGGL((k_sell_mr<V, X, ACC, K, R, Y>), g, dim3(kBlock), 0, st(), a, val, x, shf, str, y, b);
+      hipLaunchKernelGGL((k_sell_mr<V, X, ACC, K, R, Y>), g, dim3(kBlock), 0, st(), a, val, x, (int64_t)N, shf, str, y,
+                         b);
     };
     auto byk = [&](auto nt) {
       switch (L.K) {
