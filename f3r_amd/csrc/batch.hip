@@ -485,6 +485,8 @@ struct Factor16 {
   DevPtr dval, rows, rseg, rptr, col, val;  // rseg: segment of each CSR row
   int32_t nr = 0;
 };
+// dval is indexed by vector row + dsh[segment]: the right-hand sides of one
+// system share one copy of its diagonals when they share its factors
 
 __device__ __forceinline__ bool is_nan16(half_t v) {
   return (__builtin_bit_cast(uint16_t, v) & 0x7fffu) > 0x7c00u;
@@ -569,14 +571,15 @@ struct FacArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_t *__restrict__ zd,
-                                                 half_t *__restrict__ t, int G, const int32_t *__restrict__ bseg,
-                                                 MOut o) {
+                                                 const int64_t *__restrict__ dsh, half_t *__restrict__ t, int G,
+                                                 const int32_t *__restrict__ bseg, MOut o) {
   if ((int)blockIdx.x < G) {
     // every load of the thread's rows first, then the stores (a branch
     // between a row's loads would keep one row's loads in flight at a time)
     const int seg = bseg[blockIdx.x];
     const float w = o.weight(seg);
     const float sc = xin.in ? xin.sc[seg] : 0.f, iv = xin.iv(seg);
+    const int64_t sh = dsh[seg];
     V4<half_t> xi[kQuads], dw[kQuads], dz[kQuads], hov[kQuads];
 #pragma unroll
     for (int q = 0; q < kQuads; ++q) {
@@ -588,8 +591,8 @@ __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_
       } else {
         xi[q] = ld4(xin.x + i);
       }
-      dw[q] = ld4(W.d + i);
-      dz[q] = ld4(zd + i);
+      dw[q] = ld4(W.d + i + sh);
+      dz[q] = ld4(zd + i + sh);
       if (o.mode >= 2) hov[q] = ld4(o.ho + i);
     }
 #pragma unroll
@@ -624,19 +627,21 @@ __global__ __launch_bounds__(kBlock) void k_b_mw(MIn xin, FacArgs W, const half_
     acc = fma_<float>(to_acc(W.val[k]), to_acc(xin.get(W.col[k], seg)), acc);
   const half_t ti = cvt<half_t>(acc);
   t[i] = ti;
-  const half_t dz = zd[i];
+  const half_t dz = zd[i + dsh[seg]];
   if (!is_nan16(dz)) o.put(i, seg, cvt<half_t>(to_acc(dz) * to_acc(ti)));
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_mz(FacArgs Z, MIn xin, const half_t *__restrict__ wd,
-                                                 const half_t *__restrict__ t, MOut o) {
+                                                 const int64_t *__restrict__ dsh, const half_t *__restrict__ t,
+                                                 MOut o) {
   const int32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= Z.nr) return;
   const int seg = Z.rseg[r];
+  const int64_t sh = dsh[seg];
   float acc = 0.f;
   for (int32_t k = Z.rptr[r]; k < Z.rptr[r + 1]; ++k) {
     const int32_t c = Z.col[k];
-    const half_t dw = wd[c];
+    const half_t dw = wd[c + sh];
     const half_t tc = is_nan16(dw) ? t[c] : cvt<half_t>(to_acc(dw) * to_acc(xin.get(c, seg)));
     acc = fma_<float>(to_acc(Z.val[k]), to_acc(tc), acc);
   }
@@ -696,6 +701,7 @@ struct BatchSolver {
   int NR = 1;
   DevPtr csh, cst, csh16, cst16;
   Factor16 W, Z;
+  DevPtr dsh;  // per segment: vector row -> row of the factor diagonals
   DevPtr partial, status, jseg, rich;
   DevPtr b_int, x_int, r_int;
   // outer level (fp64)
@@ -797,12 +803,13 @@ struct BatchSolver {
     half_t *t = tmp->as<half_t>();
     const int gw = (W.nr + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_b_mw, dim3(G + gw), dim3(kBlock), 0, st(), xin, fac(W), (const half_t *)Z.dval->as<half_t>(),
-                       t, G, (const int32_t *)bs(), o);
+                       (const int64_t *)dsh->as<int64_t>(), t, G, (const int32_t *)bs(), o);
     if (Z.nr > 0) {
       // k_b_mz reads x as k_b_mw does (in the entry call: fp16(in * s) from
       // the fp32 input)
       hipLaunchKernelGGL(k_b_mz, dim3((Z.nr + kBlock - 1) / kBlock), dim3(kBlock), 0, st(), fac(Z), xin,
-                         (const half_t *)W.dval->as<half_t>(), (const half_t *)t, o);
+                         (const half_t *)W.dval->as<half_t>(), (const int64_t *)dsh->as<int64_t>(), (const half_t *)t,
+                         o);
     }
   }
   void precond(const half_t *in, half_t *out) {
@@ -1114,6 +1121,13 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
     }
   }
   // the SDAINV factors: diagonal-only rows as a vector, the rest as CSR rows
+  // a system's right-hand sides share its diagonals when they share its
+  // preconditioner
+  bool share_d = NR > 1;
+  for (int s = 0; s < nseg && share_d; ++s) share_d = P[s].get() == P[s - s % NR].get();
+  std::vector<int64_t> dshift(nseg, 0);
+  if (share_d)
+    for (int s = 0; s < nseg; ++s) dshift[s] = moff[s / NR] - B->off[s];
   auto factor = [&](const std::vector<const HostCSR *> &parts) {
     HostCSR M = concat(parts, B->off, N);
     std::vector<double> d(N, 0.0);
@@ -1136,6 +1150,12 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
     std::vector<int32_t> rseg(rows.size());
     for (size_t q = 0; q < rows.size(); ++q)
       rseg[q] = (int32_t)(std::upper_bound(B->off.begin(), B->off.end(), (int64_t)rows[q]) - B->off.begin() - 1);
+    if (share_d) {  // segment s's diagonals at matrix rows: those of its first copy
+      std::vector<double> ds(NM, 0.0);
+      for (int s = 0; s < nseg; s += NR)
+        std::copy(d.begin() + B->off[s], d.begin() + B->off[s + 1], ds.begin() + moff[s / NR]);
+      d.swap(ds);
+    }
     Factor16 F;
     F.dval = upload_values(c, d, F3R_F16);
     F.nr = (int32_t)rows.size();
@@ -1147,6 +1167,12 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
     F3R_HIP(hipStreamSynchronize(c->stream));  // the host arrays above are released here
     return F;
   };
+  {
+    DevPtr d = dev_alloc(sizeof(int64_t) * std::max(1, nseg));
+    F3R_HIP(hipMemcpyAsync(d->p, dshift.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, c->stream));
+    F3R_HIP(hipStreamSynchronize(c->stream));
+    B->dsh = d;
+  }
   B->W = factor(pw);
   B->Z = factor(pz);
   F3R_HIP(hipStreamSynchronize(c->stream));
