@@ -258,7 +258,7 @@ template <class S, class A, class TO>
 __global__ __launch_bounds__(kBlock) void k_b_combine(const S *__restrict__ Z, int64_t ld, const A *H, int64_t hs,
                                                       int ldh, const A *e, int64_t es, const int32_t *jseg, int jall,
                                                       const int32_t *__restrict__ bseg, TO *__restrict__ out,
-                                                      int accumulate) {
+                                                      int accumulate, const float *__restrict__ zsc) {
   __shared__ A y[128];
   const int seg = bseg[blockIdx.x];
   const int j = jseg ? jseg[seg] : jall;
@@ -269,16 +269,18 @@ __global__ __launch_bounds__(kBlock) void k_b_combine(const S *__restrict__ Z, i
   }
   if (threadIdx.x == 0) hess_solve_guarded<A>(H + (int64_t)seg * hs, ldh, e + (int64_t)seg * es, y, j);
   __syncthreads();
+  // zsc: Z holds z * s (fp16 storage, s a power of two): z = Z * zsc exactly
+  const A zs = zsc ? (A)zsc[seg] : (A)1;
   for (int q = 0; q < kQuads; ++q) {
     const int64_t i = tile_quad(q);
     const V4<S> z0 = ld4(Z + i);
     A r[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) r[c] = (A)to_acc(z0.v[c]) * y[0];
+    for (int c = 0; c < 4; ++c) r[c] = ((A)to_acc(z0.v[c]) * zs) * y[0];
     for (int k = 1; k < j; ++k) {
       const V4<S> zk = ld4(Z + (int64_t)k * ld + i);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(zk.v[c]), y[k], r[c]);
+      for (int c = 0; c < 4; ++c) r[c] = fma_<A>((A)to_acc(zk.v[c]) * zs, y[k], r[c]);
     }
     V4<TO> o;
     if (accumulate) {
@@ -391,7 +393,8 @@ template <class V, class X, class ACC, int K, int NR, class Y>
 __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restrict__ val, const X *__restrict__ x,
                                                     int64_t nx, const int32_t *__restrict__ cshift,
                                                     const int32_t *__restrict__ cstride, Y *__restrict__ y,
-                                                    const Y *__restrict__ b) {
+                                                    const Y *__restrict__ b, const int32_t *__restrict__ cseg,
+                                                    const float *__restrict__ ysc) {
   typedef const __attribute__((address_space(4))) int32_t *const_i32;
   for (int r = xcd_block() * kBlock + threadIdx.x; r < a.n; r += gridDim.x * kBlock) {
     const int s = r >> a.log2C;
@@ -463,10 +466,13 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
         }
       }
     }
+    // ysc: x held z * s (fp16 storage, s a power of two per segment), so
+    // y = sum * (1/s) is the sum over z exactly
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       const int64_t o = vr + (int64_t)c * stride;
-      y[o] = b ? cvt<Y>((ACC)to_acc(b[o]) - acc[c]) : cvt<Y>(acc[c]);
+      const ACC sum = ysc ? acc[c] * (ACC)ysc[cseg[su] + c] : acc[c];
+      y[o] = b ? cvt<Y>((ACC)to_acc(b[o]) - sum) : cvt<Y>(sum);
     }
   }
 }
@@ -702,6 +708,12 @@ struct BatchSolver {
   DevPtr csh, cst, csh16, cst16;
   Factor16 W, Z;
   DevPtr dsh;  // per segment: vector row -> row of the factor diagonals
+  DevPtr cseg;  // NR > 1: per chunk, the segment of its first copy
+  // level 3's Z as the fp16 Richardson output h (Z = h / s exactly, s the
+  // segment's power-of-two range scale): NR > 1 and every s a power of two
+  bool zh = false;
+  DevPtr Zh3;
+  half_t *zh3(int k) const { return Zh3->as<half_t>() + (int64_t)k * N; }
   DevPtr partial, status, jseg, rich;
   DevPtr b_int, x_int, r_int;
   // outer level (fp64)
@@ -716,10 +728,12 @@ struct BatchSolver {
   // y = A x (b null) or y = b - A x over all segments, A the fp64/fp32 copy
   // (layout sh) or the fp16 one (sh16)
   template <class V, class X, class Y, class ACC>
-  void spmv(const V *val, bool half_layout, const X *x, Y *y, const Y *b) {
+  // ysc (NR > 1 only): per-segment factor on the sums (x stored scaled)
+  void spmv(const V *val, bool half_layout, const X *x, Y *y, const Y *b, const float *ysc = nullptr) {
     const SellShared &L = half_layout ? *sh16 : *sh;
     const SellArgs a = sell_args(L, false);
     if (NR == 1) {
+      F3R_CHECK(ysc == nullptr, F3R_ERR_STATE, "batch: scaled SpMV needs shared matrices");
       if (b)
         launch_sell<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
       else
@@ -731,7 +745,7 @@ struct BatchSolver {
     auto go = [&](auto kt, auto nt) {
       constexpr int K = decltype(kt)::value, R = decltype(nt)::value;
       hipLaunchKernelGGL((k_sell_mr<V, X, ACC, K, R, Y>), g, dim3(kBlock), 0, st(), a, val, x, (int64_t)N, shf, str, y,
-                         b);
+                         b, (const int32_t *)cseg->as<int32_t>(), ysc);
     };
     auto byk = [&](auto nt) {
       switch (L.K) {
@@ -828,7 +842,8 @@ struct BatchSolver {
 
   // adaptive Richardson in fp16 (work/src/f3r-gpu.cpp:229-260), per segment
   // in_inv: per-segment 1/||w|| of an unnormalised input column (null: 1)
-  void level4(const float *in, const float *in_inv, float *out) {
+  // hz: the fp16 result h itself instead of out = h / s
+  void level4(const float *in, const float *in_inv, float *out, half_t *hz = nullptr) {
     ++cnt;
     ++calls;
     const bool pr = (cnt % p.c) == 0;
@@ -845,25 +860,30 @@ struct BatchSolver {
       hipLaunchKernelGGL(k_b_l4_scale, dim3(G), dim3(kBlock), 0, st(), ho, (const RichSeg *)rs, 1,
                          (const int32_t *)bs());
     }
-    bool done = false;  // the last step already wrote `out`
+    bool done = false;  // the last step already wrote the result
     for (int k = 1; k < p.m4; ++k) {
       const bool last = k == p.m4 - 1;
+      half_t *hn = last && hz ? hz : ho2;  // the step's fp16 result
       spmv<half_t, half_t, half_t, float>(A16->as<half_t>(), true, (const half_t *)ho, hr_, (const half_t *)hi);
       if (pr) {
         precond(hr_, tw);
         probe(tw, hr_, k);
         hipLaunchKernelGGL(k_b_l4_update, dim3(G), dim3(kBlock), 0, st(), (const half_t *)tw, (const half_t *)ho,
-                           ho2, (const RichSeg *)rs, k, 1, (const int32_t *)bs());
+                           hn, (const RichSeg *)rs, k, 1, (const int32_t *)bs());
+        done = last && hz;
       } else {
         // the weighted update (and on the last step the way out of the
-        // fp16 level) fused into M
+        // fp16 level, unless hz takes h itself) fused into M
         precond(MIn{hr_, nullptr, nullptr, nullptr, nullptr},
-                MOut{last ? 3 : 2, ho2, rs, k, (const half_t *)ho, out, isc});
+                MOut{last && !hz ? 3 : 2, hn, rs, k, (const half_t *)ho, out, isc});
         done = last;
       }
       std::swap(ho, ho2);
     }
-    if (!done)
+    if (done) return;
+    if (hz)
+      hipLaunchKernelGGL((k_b_cast<half_t, half_t>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)ho, hz);
+    else
       hipLaunchKernelGGL((k_b_entry<half_t, float>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)ho, out, isc,
                          (const int32_t *)bs());
   }
@@ -872,17 +892,30 @@ struct BatchSolver {
   // (V_0 is the level-2 column itself; the other columns stay unnormalised)
   void level3(const float *in, float *out) {
     BLevel<float> &L = L3;
+    const float *isc = iscale->as<float>();
     for (int j = 0; j < L.m; ++j) {
-      if (j == 0)
-        level4(in, nullptr, L.Zc(N, 0));
-      else
-        level4(L.Vc(N, j), L.csc->as<float>() + (int64_t)j * nseg, L.Zc(N, j));
-      spmv<half_t, float, float, float>(A16->as<half_t>(), true, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
+      const float *vin = j == 0 ? in : L.Vc(N, j);
+      const float *vinv = j == 0 ? nullptr : L.csc->as<float>() + (int64_t)j * nseg;
+      if (zh) {
+        level4(vin, vinv, nullptr, zh3(j));
+        spmv<half_t, half_t, float, float>(A16->as<half_t>(), true, (const half_t *)zh3(j), L.Vc(N, j + 1), nullptr,
+                                           isc);
+      } else {
+        level4(vin, vinv, L.Zc(N, j));
+        spmv<half_t, float, float, float>(A16->as<half_t>(), true, (const float *)L.Zc(N, j), L.Vc(N, j + 1),
+                                          nullptr);
+      }
       arnoldi(L, j, 1, nullptr, in, true);
     }
-    hipLaunchKernelGGL((k_b_combine<float, float, float>), dim3(G), dim3(kBlock), 0, st(), (const float *)L.Zc(N, 0), N,
-                       (const float *)L.H->as<float>(), L.hs(), L.m + 1, (const float *)L.e->as<float>(),
-                       (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m, (const int32_t *)bs(), out, 0);
+    if (zh)
+      hipLaunchKernelGGL((k_b_combine<half_t, float, float>), dim3(G), dim3(kBlock), 0, st(), (const half_t *)zh3(0),
+                         N, (const float *)L.H->as<float>(), L.hs(), L.m + 1, (const float *)L.e->as<float>(),
+                         (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m, (const int32_t *)bs(), out, 0, isc);
+    else
+      hipLaunchKernelGGL((k_b_combine<float, float, float>), dim3(G), dim3(kBlock), 0, st(), (const float *)L.Zc(N, 0),
+                         N, (const float *)L.H->as<float>(), L.hs(), L.m + 1, (const float *)L.e->as<float>(),
+                         (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m, (const int32_t *)bs(), out, 0,
+                         (const float *)nullptr);
   }
 
   // FGMRES(m2) on the fp32 values, fp32 basis, normalize = false
@@ -897,7 +930,7 @@ struct BatchSolver {
     hipLaunchKernelGGL((k_b_combine<float, float, double>), dim3(G), dim3(kBlock), 0, st(),
                        (const float *)L.Zc(N, 0), N, (const float *)L.H->as<float>(), L.hs(), L.m + 1,
                        (const float *)L.e->as<float>(), (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m,
-                       (const int32_t *)bs(), out, 0);
+                       (const int32_t *)bs(), out, 0, (const float *)nullptr);
   }
 
   std::vector<double> read_status() {
@@ -978,7 +1011,7 @@ struct BatchSolver {
       hipLaunchKernelGGL((k_b_combine<double, double, double>), dim3(G), dim3(kBlock), 0, st(),
                          (const double *)L.Zc(N, 0), N, (const double *)Hd, L.hs(), m1 + 1,
                          (const double *)L.e->as<double>(), (int64_t)(m1 + 1), (const int32_t *)jseg->as<int32_t>(),
-                         0, (const int32_t *)bs(), x_int->as<double>(), 1);
+                         0, (const int32_t *)bs(), x_int->as<double>(), 1, (const float *)nullptr);
       F3R_HIP(hipStreamSynchronize(st()));  // jdone is host memory of this scope
     }
     F3R_HIP(hipGetLastError());
@@ -1085,16 +1118,18 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
   // chunk (of C rows) -> shift from matrix row to vector row, copy stride
   auto chunk_tables = [&](DevPtr &dsh, DevPtr &dst) {
     const int64_t nch = (NM + C - 1) / C;
-    std::vector<int32_t> sh_(std::max<int64_t>(1, nch)), st_(std::max<int64_t>(1, nch));
+    std::vector<int32_t> sh_(std::max<int64_t>(1, nch)), st_(std::max<int64_t>(1, nch)), sg_(std::max<int64_t>(1, nch));
     for (size_t m = 0; m < msys.size(); ++m) {
       const int64_t seg0 = (int64_t)m * NR;
       for (int64_t ch = moff[m] / C; ch < moff[m + 1] / C; ++ch) {
         sh_[ch] = (int32_t)(B->off[seg0] - moff[m]);
         st_[ch] = (int32_t)(B->off[seg0 + 1] - B->off[seg0]);
+        sg_[ch] = (int32_t)seg0;
       }
     }
     dsh = up32(sh_);
     dst = up32(st_);
+    B->cseg = up32(sg_);
     F3R_HIP(hipStreamSynchronize(c->stream));  // the host tables are released here
   };
   if (NR > 1) {
@@ -1180,6 +1215,15 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
   B->alloc_level(B->L1, p.m1);
   B->alloc_level(B->L2, p.m2);
   B->alloc_level(B->L3, p.m3);
+  {
+    bool pow2 = true;
+    for (float v : sc) {
+      int e = 0;
+      pow2 = pow2 && std::frexp(v, &e) == 0.5f;
+    }
+    B->zh = NR > 1 && pow2;
+    if (B->zh) B->Zh3 = dev_alloc(sizeof(half_t) * N * p.m3);
+  }
   auto hv = [&]() { return dev_alloc(sizeof(half_t) * N); };
   B->hin = hv();
   B->hout = hv();
