@@ -240,6 +240,15 @@ def test_c2_shape_64cube(ctx):
     assert r.is_solved and _true_res(A, xv.download(), b) <= 1e-8
     _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=_scale(A.n))
     assert abs(r.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter)))
+    # the kernel timings of the bench, back to back and with the caches emptied first
+    for which in range(4):
+        ms, by = S.time_kernel(which, reps=3)
+        msc, byc = S.time_kernel(which, reps=2, cold=True)
+        assert ms > 0 and msc > 0 and by == byc and by > 0, (which, ms, msc, by, byc)
+    # and the solve after them is unchanged
+    xv.fill(0.0)
+    r2 = S.solve(bv, xv)
+    assert r2.res_iter == r.res_iter and r2.res_nrm2 == r.res_nrm2
 
 
 @pytest.mark.parametrize("variant,prec", [("float", "sdainv"), ("double", "sdainv"), ("float", "ilu0"),
