@@ -1358,7 +1358,8 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   } else {
     mk(S->L2f, p.m2);
     mk(S->L3f, p.m3);
-    if (p.variant == 0) {
+    int e2 = 0;
+    if (p.variant == 0 && std::frexp(S->range_s, &e2) == 0.5f) {  // exact only for a power-of-two scale
       const int64_t ld = (n + 63) / 64 * 64;
       S->L3f.Zhbuf = dev_alloc(sizeof(half_t) * ld * p.m3);
       S->L3f.Zh.resize(p.m3);

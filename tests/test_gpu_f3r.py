@@ -89,6 +89,28 @@ def test_level4_richardson_steps_bitexact(ctx, kind, prec, m4):
     assert np.array_equal(vout.download(), ref)
 
 
+def test_non_power_of_two_range_scale(ctx):
+    # level 3 keeps an fp32 Z when Z = h / s would not be exact in fp16
+    # storage; level 4 stays bit-exact and the solve matches the oracle's
+    import f3r_amd as F
+    A = _problem("27pt", 14)
+    P = _prec(F, A, ctx, "sdainv")
+    S = F.F3R(ctx, A, P, range_scale=3.0, use_graph=False)
+    v = np.random.default_rng(6).uniform(-1, 1, A.n)
+    v /= np.linalg.norm(v)
+    vin, vout = F.vector.from_numpy(ctx, v), F.vector(ctx, "f64", A.n)
+    S.level_apply(4, vin, vout)
+    ref = O.level_apply(A.n, A.rptr, A.idx, A.val, 4, v, variant=0, ref_arith=0, range_scale=3.0)
+    assert np.array_equal(vout.download(), ref)
+    b = F.uniform(A.n, -1.0, 1.0, seed=8)
+    S2 = F.F3R(ctx, A, P, range_scale=3.0)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    res = S2.solve(F.vector.from_numpy(ctx, b), xv)
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=3.0)
+    assert res.is_solved and _true_res(A, xv.download(), b) <= 1e-8
+    assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter)))
+
+
 @pytest.mark.parametrize("kind", ["27pt", "lognormal"])
 def test_level3_close(ctx, kind):
     import f3r_amd as F
