@@ -223,7 +223,8 @@ def main():
     # every byte then comes from HBM
     ms16c, _ = S.time_kernel(0, reps=10, cold=True)
     kern = {}
-    for w, name in ((1, "spmv_f16val_f32vec"), (2, "spmv_f32"), (3, "spmv_f64")):
+    for w, name in ((1, "spmv_f16val_f32vec"), (2, "spmv_f32"), (3, "spmv_f64"),
+                    (4, "l3_spmv_cgs_dots"), (5, "l3_cgs_update_norm")):
         ms, by = S.time_kernel(w, reps=20)
         kern[name] = {"us": round(ms * 1e3, 2), "GB/s": round(by / ms / 1e6, 1)}
     # the same fp16 SpMV in the int32-index layout (a column index per slot),

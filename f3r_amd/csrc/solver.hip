@@ -1609,9 +1609,9 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
                                             EpiStore<double, double>{y64, nullptr}, s->st());
         by = s->sh->format_bytes(8, 8, 8);
         break;
-      case 4:  // level-3 Arnoldi step j = m3 - 1: fp16 SpMV of Z_j + CGS dots with V_0..V_j
+      case 4:  // level-3 Arnoldi step j = m3 - 1: fp16 SpMV of Z_j + CGS dots with V_0..V_j, and their finish
         s->template spmv_dots<float, half_t, 4>(s->L3f, s->A16->as<half_t>(), s->L3f.m - 1);
-        by = s->sh16->format_bytes(2, 4, 4) + 4.0 * s->n * s->L3f.m;
+        by = s->sh16->format_bytes(2, s->L3f.Zh.empty() ? 4 : 2, 4) + 4.0 * s->n * s->L3f.m;
         break;
       case 5:  // level-3 CGS update + norm + Givens of that step
         s->template cgs_update<float, 4>(s->L3f, s->L3f.m - 1);
