@@ -677,6 +677,16 @@ void prec_build_device(Prec *P, const std::vector<int32_t> *perm, int32_t C, int
       }
       P->dw = upload_values(c, wd, P->dtype);
       P->dz = upload_values(c, zd, P->dtype);
+      if (P->dtype == F3R_F16) {
+        std::vector<uint16_t> wz(2 * (size_t)n);
+        for (int32_t r = 0; r < n; ++r) {
+          wz[2 * (size_t)r] = d2h_bits(wd[r]);
+          wz[2 * (size_t)r + 1] = d2h_bits(zd[r]);
+        }
+        P->dwz = dev_alloc(sizeof(uint16_t) * std::max<size_t>(2, wz.size()));
+        if (n) F3R_HIP(hipMemcpyAsync(P->dwz->p, wz.data(), sizeof(uint16_t) * wz.size(), hipMemcpyHostToDevice, c->stream));
+        F3R_HIP(hipStreamSynchronize(c->stream));  // wz is released at scope end
+      }
     }
     for (int w = 0; w < 2; ++w) {
       const HostCSR &M = w == 0 ? W : Z;

@@ -22,6 +22,7 @@ struct Prec {
   std::shared_ptr<Mat> W, Z;  // W holds W^T (rows w_i), Z holds Z D^-1
   bool diag = false;
   DevPtr dw, dz;              // diagonals when `diag`
+  DevPtr dwz;                 // fp16 only: (dw[i], dz[i]) interleaved, one 4-byte load per row
   // ILU(0)
   std::vector<int32_t> l_lptr, u_lptr;  // level pointers into *_rows
   DevPtr l_rows, u_rows, l_rptr, l_idx, l_val, u_rptr, u_idx, u_val, u_dinv;

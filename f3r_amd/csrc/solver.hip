@@ -404,6 +404,9 @@ __global__ void k_scatter(int64_t n, const double *__restrict__ s, const int32_t
 }
 
 // ---------------------------------------------- Richardson (fp16 level) ---
+// the interleaved diagonal factors of a row: (w, z) in one 32-bit word
+__device__ __forceinline__ half_t dwz_w(uint32_t v) { return __builtin_bit_cast(half_t, (uint16_t)(v & 0xffffu)); }
+__device__ __forceinline__ half_t dwz_z(uint32_t v) { return __builtin_bit_cast(half_t, (uint16_t)(v >> 16)); }
 // Entry of one call (work/src/f3r-gpu.cpp:231-243 in this build's
 // arithmetic): hin = fp16(in * s); with diagonal SDAINV factors also
 // hout = fp16(z * fp16(w * hin)) and, outside probe calls, hout *= ome_0.
@@ -422,25 +425,24 @@ __device__ __forceinline__ void l4_entry_row(float &v, const float iv, bool has_
 }
 template <bool VEC>
 __global__ void k_l4_entry(int64_t n, float *__restrict__ in, const float *__restrict__ inv, float s,
-                           half_t *__restrict__ hin, half_t *__restrict__ hout, const half_t *__restrict__ dw,
-                           const half_t *__restrict__ dz, const RichDev *st, int scale_ome) {
+                           half_t *__restrict__ hin, half_t *__restrict__ hout, const uint32_t *__restrict__ dwz,
+                           const RichDev *st, int scale_ome) {
   const float om = scale_ome ? (float)st->ome[0] : 1.f;
   const float iv = inv ? *inv : 1.f;
-  const bool diag = dw != nullptr;
+  const bool diag = dwz != nullptr;
   const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
   int64_t i0 = 0;
   if constexpr (VEC) {
     const int64_t n4 = n / 4;
     for (int64_t q = gid; q < n4; q += gstride) {
       V4<float> v = ld4(in + 4 * q);
-      V4<half_t> dwv{}, dzv{}, h, o{};
-      if (diag) {
-        dwv = ld4(dw + 4 * q);
-        dzv = ld4(dz + 4 * q);
-      }
+      V4<uint32_t> wz{};
+      V4<half_t> h, o{};
+      if (diag) wz = ld4(dwz + 4 * q);
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        l4_entry_row(v.v[c], iv, inv != nullptr, s, h.v[c], o.v[c], dwv.v[c], dzv.v[c], diag, scale_ome != 0, om);
+        l4_entry_row(v.v[c], iv, inv != nullptr, s, h.v[c], o.v[c], dwz_w(wz.v[c]), dwz_z(wz.v[c]), diag,
+                     scale_ome != 0, om);
       if (inv) st4(in + 4 * q, v);
       st4(hin + 4 * q, h);
       if (diag) st4(hout + 4 * q, o);
@@ -450,8 +452,8 @@ __global__ void k_l4_entry(int64_t n, float *__restrict__ in, const float *__res
   for (int64_t i = i0 + gid; i < n; i += gstride) {
     float v = in[i];
     half_t h, o = (half_t)0;
-    l4_entry_row(v, iv, inv != nullptr, s, h, o, diag ? dw[i] : (half_t)0, diag ? dz[i] : (half_t)0, diag,
-                 scale_ome != 0, om);
+    const uint32_t wz = diag ? dwz[i] : 0u;
+    l4_entry_row(v, iv, inv != nullptr, s, h, o, dwz_w(wz), dwz_z(wz), diag, scale_ome != 0, om);
     if (inv) in[i] = v;
     hin[i] = h;
     if (diag) hout[i] = o;
@@ -485,23 +487,23 @@ __global__ void k_l4_update(int64_t n, const half_t *__restrict__ tmp2, const ha
 struct EpiL4Diag {
   using State = NoState;
   struct Pre {
-    half_t hin, dw, dz, hout;
+    half_t hin, hout;
+    uint32_t wz;
   };
   const half_t *__restrict__ hin;
   const half_t *__restrict__ hout;
-  const half_t *__restrict__ dw;
-  const half_t *__restrict__ dz;
+  const uint32_t *__restrict__ dwz;  // (w, z) diagonals interleaved
   const double *w;  // ome_k
   half_t *__restrict__ hr_out;    // probe calls: keep hr and tmp2
   half_t *__restrict__ tmp2_out;
   half_t *__restrict__ hnext;
   float *__restrict__ out;
   float inv_s;
-  __device__ Pre pre(int r) const { return {hin[r], dw[r], dz[r], hr_out ? (half_t)0 : hout[r]}; }
+  __device__ Pre pre(int r) const { return {hin[r], hr_out ? (half_t)0 : hout[r], dwz[r]}; }
   __device__ void row(int r, float acc, const Pre &p, State &) const {
     const half_t hr = cvt<half_t>((float)p.hin - acc);
-    const half_t t = cvt<half_t>(__builtin_fmaf((float)p.dw, (float)hr, 0.f));
-    const half_t t2 = cvt<half_t>(__builtin_fmaf((float)p.dz, (float)t, 0.f));
+    const half_t t = cvt<half_t>(__builtin_fmaf((float)dwz_w(p.wz), (float)hr, 0.f));
+    const half_t t2 = cvt<half_t>(__builtin_fmaf((float)dwz_z(p.wz), (float)t, 0.f));
     if (hr_out) {
       hr_out[r] = hr;
       tmp2_out[r] = t2;
@@ -859,21 +861,20 @@ struct Solver {
     half_t *hi = hin->as<half_t>(), *ho = hout->as<half_t>(), *ho2 = hout2->as<half_t>();
     const float inv_s = 1.f / range_s;
     const bool diag = P->kind == 0 && P->diag;
-    const half_t *dw = diag ? P->dw->as<half_t>() : nullptr;
-    const half_t *dz = diag ? P->dz->as<half_t>() : nullptr;
+    const uint32_t *dwz = diag ? P->dwz->as<uint32_t>() : nullptr;
     const half_t *a16 = A16->as<half_t>();
     SellArgs a = argsv<half_t>();
     const int K16 = sh16->K;
     // entry (normalising `in` first when the Arnoldi step that made it did not)
     const float *pinv = pending_inv;
     pending_inv = nullptr;
-    const bool vec = aligned4(in) && aligned4(hi) && aligned4(ho) && (!diag || (aligned4(dw) && aligned4(dz)));
+    const bool vec = aligned4(in) && aligned4(hi) && aligned4(ho) && (!diag || aligned4(dwz));
     if (vec)
-      hipLaunchKernelGGL(k_l4_entry<true>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw,
-                         dz, (const RichDev *)rs, (diag && !probe) ? 1 : 0);
+      hipLaunchKernelGGL(k_l4_entry<true>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dwz,
+                         (const RichDev *)rs, (diag && !probe) ? 1 : 0);
     else
-      hipLaunchKernelGGL(k_l4_entry<false>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dw,
-                         dz, (const RichDev *)rs, (diag && !probe) ? 1 : 0);
+      hipLaunchKernelGGL(k_l4_entry<false>, g, b, 0, st(), nn, const_cast<float *>(in), pinv, range_s, hi, ho, dwz,
+                         (const RichDev *)rs, (diag && !probe) ? 1 : 0);
     if (!diag) {
       prec_apply_raw(P.get(), hi, ho, st());
       if (!probe) hipLaunchKernelGGL(k_l4_scale, g, b, 0, st(), nn, ho, (const double *)&rs->ome[0]);
@@ -891,7 +892,7 @@ struct Solver {
       half_t *hn = last && hz ? hz : ho2;     // the step's fp16 result
       float *of = last && !hz ? out : nullptr;  // or, last step, its fp32 h / s
       if (diag) {
-        EpiL4Diag epi{hi, ho, dw, dz, &rs->ome[k], probe ? hr_ : nullptr, probe ? t2 : nullptr,
+        EpiL4Diag epi{hi, ho, dwz, &rs->ome[k], probe ? hr_ : nullptr, probe ? t2 : nullptr,
                       hn, of, inv_s};
         launch_sell<half_t, half_t, float>(K16, a, a16, ho, epi, st());
       } else {
