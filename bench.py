@@ -13,7 +13,7 @@ of SDAINV, trained on the 7-point lognormal-diffusion family at 96^3
 (f3r_amd/learn.py; parameters in f3r_amd/data/), against plain SDAINV F3R on
 held-out systems of that family, reported under "learned".
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--no-cpu] [--learned N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--no-cpu] [--learned N] [--batch 0|1]
 
 Prints one JSON line.
 """
@@ -28,40 +28,83 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solve time to 1e-8 fp64 rel. residual, learned vs plain F3R; fp16 SpMV HBM GB/s"
 PEAK_HBM_GBS = 8000.0
+# the solve's hot kernels (f3r_solver_time_kernel modes): the metric's fp16
+# SpMV, then the four kernels of every level-3 Arnoldi step
+KERNELS = (("spmv_f16", 0), ("l3_spmv_cgs_dots", 6), ("l4_richardson_step", 8), ("l3_cgs_update_norm", 7),
+           ("l4_entry", 9))
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(A, b, outer_gpu, scale):
-    """The oracle's F3R (the CPU restatement of the reference, fp16 emulated
-    through F16C conversions) on the same matrix, timed on the host's cores
-    over a bounded sample: one outer step, scaled to the GPU's outer count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(F, A, b, reps=3):
+    """The oracle's F3R (the C++/OpenMP restatement of the reference's nest,
+    fp16 emulated through F16C conversions) timed on the host's cores: full
+    solves, solve only (set-up excluded, as work/src/f3r.cpp:72-74), mean of
+    `reps` repetitions (work/src/f3r.cpp:261-272), on
+      C2: the bench's own 128^3 nonsymmetric system, this build's arithmetic
+          (fp32 sums of fp16 data; the reference's fp16 arithmetic emulated
+          on the host is slower still, measured on C1);
+      C1: the symmetric 27-point 64^3 system with b = A*1 and restart 3 (the
+          CPU-path configuration), in both arithmetics."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
     import oracle_lib as O
-    threads = O.lib().orc_num_threads()
-    _, r, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=scale,
-                          max_outer=1)
-    per_outer = r.solve_s
-    return {"value": per_outer * outer_gpu, "unit": "s", "cores": int(r.nthreads),
-            "kind": "port",
-            "sample": f"1 outer FGMRES step of the same 128^3 solve ({per_outer:.2f} s, "
-                      f"{r.precond_calls} fp16 Richardson calls) scaled by the GPU's {outer_gpu} outer steps; "
-                      f"{threads} OpenMP threads; setup excluded"}
+
+    def timed(Ab, bb, **kw):
+        ts, r = [], None
+        for _ in range(reps):
+            _, r, _ = O.f3r_solve(Ab.n, Ab.rptr, Ab.idx, Ab.val, bb, **kw)
+            ts.append(r.solve_s)
+        return {"mean_s": round(float(np.mean(ts)), 4), "reps_s": [round(t, 4) for t in ts],
+                "outer_steps": int(r.res_iter), "true_rel_res": float(r.expl_res), "solved": bool(r.is_solved)}, r
+
+    scale2 = float(2.0 ** np.floor(np.log2(np.sqrt(A.n) / 4.0)))
+    c2, r2 = timed(A, b, variant=0, ref_arith=0, range_scale=scale2)
+    log(f"[bench] cpu C2: {c2}")
+    A1 = F.CSR.stencil27(64)
+    b1 = np.add.reduceat(A1.val, A1.rptr[:-1])  # b = A * 1 of the unscaled matrix
+    F.algorithm.scaling(A1)
+    scale1 = float(2.0 ** np.floor(np.log2(np.sqrt(A1.n) / 4.0)))
+    c1, _ = timed(A1, b1, variant=0, ref_arith=0, range_scale=scale1, restart=3)
+    c1r, _ = timed(A1, b1, variant=0, ref_arith=1, restart=3)
+    log(f"[bench] cpu C1: {c1}, reference arithmetic {c1r}")
+    return {"value": c2["mean_s"], "unit": "s", "cores": int(r2.nthreads), "kind": "port",
+            "cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
+            "sample": f"full F3R solves of the 128^3 bench system (C2), mean of {reps}, solve only; "
+                      f"{int(r2.nthreads)} OpenMP threads on {cpu_model()}",
+            "c2_128": c2,
+            "c1_64_symmetric_rowsum_rhs": {"build_arithmetic": c1, "reference_arithmetic": c1r}}
 
 
-def measured_traffic(us_per_launch):
-    """HBM bytes per launch of the fp16 SpMV from the committed hardware-counter
-    summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a
-    copy kernel of known size); None when absent"""
-    path = os.path.join(ROOT, "profiles", "spmv_f16_c2_128_traffic.json")
+def measured_traffic(kernel):
+    """bytes per launch of one of the solve's kernels from the committed
+    hardware-counter summary (two rocprofv3 --pmc runs of `bench.py
+    --kernel-only`, FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+    counter reads for a copy kernel of known size); None when absent"""
+    path = os.path.join(ROOT, "profiles", "kernels_c2_128_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return {"bytes": t["hbm_bytes_per_launch"], "fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
-            "copy_scale": t["copy_scale"], "source": "profiles/spmv_f16_c2_128_traffic.md"}
+    if kernel not in t.get("kernels", {}):
+        return None
+    k = t["kernels"][kernel]
+    return {"bytes": k["bytes_per_launch"], "fetch_bytes": k["fetch_bytes"], "write_bytes": k["write_bytes"],
+            "fetch_scale": t["fetch_scale"], "write_scale": t["write_scale"],
+            "source": "profiles/kernels_c2_128_traffic.md"}
 
 
 def learned_vs_plain(F, ctx, count, nx=96, reps=3):
@@ -146,14 +189,15 @@ def main():
                     help="factorise SDAINV and build the matrix layouts on the host (default: device)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--batch", type=int, default=1, help="1: also time the 64 x 4 lockstep batch (0: skip)")
     ap.add_argument("--learned", type=int, default=16, metavar="N",
                     help="held-out 96^3 lognormal systems for learned vs plain F3R (0: skip)")
     ap.add_argument("--setup-only", action="store_true",
                     help="no solves: time the set-up steps (SDAINV and ILU(0) factors, solver layouts) on the "
                          "device and on the host, check the factors agree, print one JSON line and exit")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="REPS",
-                    help="no solves: launch the dominant fp16 SpMV REPS times and a plain copy of a known "
-                         "size REPS times (for hardware-counter runs), print their timings and exit")
+                    help="no solves: launch each hot kernel of the solve (KERNELS) REPS rounds and a plain copy "
+                         "of a known size REPS times (for hardware-counter runs), print their timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the solve is not sharded across GPUs "
@@ -181,7 +225,13 @@ def main():
     setup_s = time.time() - t0
     log(f"[bench] setup {setup_s:.2f} s (SDAINV factors on the {P.where}: {factor_s:.2f} s)")
     if args.kernel_only:
-        ms16, by16 = S.time_kernel(0, reps=args.kernel_only)
+        # for hardware-counter runs: each kernel back to back `REPS` rounds,
+        # in this order, then a copy of known size for calibration
+        out = {"reps": args.kernel_only, "kernels": {}}
+        for name, w in KERNELS:
+            ms, by = S.time_kernel(w, reps=args.kernel_only)
+            out["kernels"][name] = {"which": w, "us": round(ms * 1e3, 2), "algorithmic_bytes": by,
+                                    "launches": (args.kernel_only + 1) * (S.params.m3 if w in (6, 7) else 1)}
         # calibration copy: 2^27 fp32 elements (512 MiB read, 512 MiB written),
         # far larger than the L2s and the last-level cache
         nc = 1 << 27
@@ -192,10 +242,10 @@ def main():
         for _ in range(args.kernel_only):
             yc.copy(xc)
         ctx.sync()
-        tcopy = (time.time() - tc) / args.kernel_only
-        print(json.dumps({"spmv_f16_us": round(ms16 * 1e3, 2), "spmv_f16_algorithmic_bytes": by16,
-                          "copy_elements": nc, "copy_bytes_read": 4 * nc, "copy_bytes_written": 4 * nc,
-                          "copy_us_wall": round(tcopy * 1e6, 2), "reps": args.kernel_only}), flush=True)
+        out.update({"copy_elements": nc, "copy_bytes_read": 4 * nc, "copy_bytes_written": 4 * nc,
+                    "copy_launches": args.kernel_only,
+                    "copy_us_wall": round((time.time() - tc) / args.kernel_only * 1e6, 2)})
+        print(json.dumps(out), flush=True)
         return
     bv = F.vector.from_numpy(ctx, b)
     xv = F.vector(ctx, "f64", A.n)
@@ -216,17 +266,19 @@ def main():
     rv = F.vector(ctx, "f64", A.n)
     A64.compute_residual(bv, xv, rv)
     true_rel = F.nrm(rv) / F.nrm(bv)
-    # dominant kernel: the fp16 SpMV of the solver's layout, timed with HIP
-    # events on the solver's stream
-    ms16, by16 = S.time_kernel(0, reps=50)
-    # the same kernel with cold caches (each launch after a 512 MiB pass):
-    # every byte then comes from HBM
-    ms16c, _ = S.time_kernel(0, reps=10, cold=True)
+    # every hot kernel of the solve, timed with HIP events on the solver's
+    # stream (back-to-back launches in the solve's own mix of Arnoldi steps),
+    # with its algorithmic bytes per launch
     kern = {}
-    for w, name in ((1, "spmv_f16val_f32vec"), (2, "spmv_f32"), (3, "spmv_f64"),
-                    (4, "l3_spmv_cgs_dots"), (5, "l3_cgs_update_norm")):
+    for name, w in KERNELS + (("spmv_f16val_f32vec", 1), ("spmv_f32", 2), ("spmv_f64", 3)):
         ms, by = S.time_kernel(w, reps=20)
-        kern[name] = {"us": round(ms * 1e3, 2), "GB/s": round(by / ms / 1e6, 1)}
+        kern[name] = {"us": round(ms * 1e3, 2), "algorithmic_bytes": by, "GB/s": round(by / ms / 1e6, 1),
+                      "frac": round(by / ms / 1e6 / PEAK_HBM_GBS, 4)}
+        log(f"[bench] {name}: {kern[name]}")
+    # the metric's fp16 SpMV with cold caches (each launch after a 512 MiB pass)
+    ms16c, by16 = S.time_kernel(0, reps=10, cold=True)
+    kern["spmv_f16"]["us_cold"] = round(ms16c * 1e3, 2)
+    kern["spmv_f16"]["frac_cold"] = round(by16 / ms16c / 1e6 / PEAK_HBM_GBS, 4)
     # the same fp16 SpMV in the int32-index layout (a column index per slot),
     # the layout the 70%-of-peak target is stated for
     xi = F.vector.from_numpy(ctx, np.ones(A.n), "f16")
@@ -234,10 +286,14 @@ def main():
     S32 = F.SELL(ctx, A, "f16", C=64, sigma=1, K=4, offsets=False)
     ms_i = S32.time_spmv(xi, yi, reps=50)
     by_i = 6.0 * A.nnz + 4.0 * A.n
-    kern["spmv_f16_int32_index"] = {"us": round(ms_i * 1e3, 2), "GB/s": round(by_i / ms_i / 1e6, 1),
+    kern["spmv_f16_int32_index"] = {"us": round(ms_i * 1e3, 2), "algorithmic_bytes": by_i,
+                                    "GB/s": round(by_i / ms_i / 1e6, 1),
                                     "frac": round(by_i / ms_i / 1e6 / PEAK_HBM_GBS, 4)}
     del S32
-    ach = by16 / (ms16 * 1e-3) / 1e9
+    # roofline: the kernel that takes the most time in the solve, the level-3
+    # fp16 SpMV with the fused CGS dots
+    head = kern["l3_spmv_cgs_dots"]
+    ach = head["algorithmic_bytes"] / (head["us"] * 1e-6) / 1e9
     mean_s = float(np.mean(times))
     out = {
         "metric": METRIC,
@@ -260,29 +316,34 @@ def main():
                    "sdainv_factors": P.where, "sdainv_s": round(factor_s, 3),
                    "graph": not args.no_graph},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
-                     "traffic_detail": None,
-                     "kernel": "fp16 SELL SpMV y = A16 x (fp16 x, y; fp32 sums), chunk-offset layout "
-                               "(C=64, K=8 and a half group: 28 slots per 27-point row): bytes are those of the layout "
-                               "(values; no column indices); achieved: back-to-back launches (the 256 MB "
-                               "last-level cache keeps part of the 117 MB of values), achieved_cold: each "
-                               "launch after a 512 MiB cache-emptying pass",
-                     "us_per_launch": round(ms16 * 1e3, 2), "algorithmic_bytes": by16,
-                     "us_per_launch_cold": round(ms16c * 1e3, 2),
-                     "achieved_cold": round(by16 / (ms16c * 1e-3) / 1e9, 1),
-                     "int32_index_equivalent_GBs": round((6.0 * A.nnz + 4.0 * A.n) / (ms16 * 1e-3) / 1e9, 1)},
+                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "traffic_detail": None,
+                     "kernel": "k_sell<half,half,float,8,EpiDots<float,4>>: level-3 fp16 SpMV w = A16 z_j (fp16 "
+                               "values, fp16 z, fp32 sums) with the CGS dots <w, v_k>, k <= j, fused; the solve's "
+                               "mix j = 0..3, back to back; algorithmic bytes per launch = 2 nnz (values, unpadded; "
+                               "no column indices in the chunk-offset layout) + 4 B per nonzero of the int32-index "
+                               "chunks + n (2 z + 4 w + 4 (j+1) v), averaged over j",
+                     "us_per_launch": head["us"], "algorithmic_bytes": head["algorithmic_bytes"],
+                     "metric_spmv_f16": kern["spmv_f16"]},
         "kernels": kern,
     }
-    tr = measured_traffic(ms16 * 1e3)
+    tr = measured_traffic("l3_spmv_cgs_dots")
     if tr:
         out["roofline"]["traffic"] = tr["bytes"]
         out["roofline"]["traffic_detail"] = tr
     if args.learned > 0:
         out["learned"] = learned_vs_plain(F, ctx, args.learned)
+    if args.batch:
+        from f3r_amd import batch
+        del S
+        bo = batch.run_lockstep(64, 4, seed=0, log=log)
+        out["batch"] = {"workload": "64 systems (16^3..64^3, 7- and 27-point, diagonal margin U[0.5,2]) x 4 "
+                                    "right-hand sides, lockstep F3R (one set of launches for all 256 solves)",
+                        **{k: bo[k] for k in ("solves", "padded_rows", "rows_total", "wall_s", "walls_s",
+                                              "all_solved", "outer_steps_max", "outer_steps_mean",
+                                              "worst_true_rel_res", "setup_s")}}
     if not args.no_cpu:
-        scale = float(2.0 ** np.floor(np.log2(np.sqrt(A.n) / 4.0)))
-        log("[bench] cpu baseline (oracle, one outer step) ...")
-        out["cpu_baseline"] = cpu_baseline(A, b, res.res_iter, scale)
+        log("[bench] cpu baseline (oracle, full solves) ...")
+        out["cpu_baseline"] = cpu_baseline(F, A, b)
     print(json.dumps(out), flush=True)
 
 

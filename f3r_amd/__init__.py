@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi
 from ._capi import F16, F32, F64, I32, F3RError, check, lib
 
-__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "solve_res_t",
+__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
            "F3RError", "F64", "F32", "F16", "device_count"]
 
 _NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32}
@@ -257,6 +257,60 @@ class algorithm:
         return A
 
 
+class io:
+    """sparse/matrix/io.hpp"""
+
+    class mm:
+        """MatrixMarket files (io::mm, io.hpp:104-217)"""
+
+        @staticmethod
+        def read_matrix(path):
+            """a coordinate file as a host CSR (io::mm::read_matrix + io::CSR)"""
+            h = ctypes.c_void_p()
+            check(lib.f3r_mm_read(None, str(path).encode(), ctypes.byref(h)), "mm_read")
+            try:
+                nr, nc = ctypes.c_int32(), ctypes.c_int32()
+                check(lib.f3r_mat_shape(h, ctypes.byref(nr), ctypes.byref(nc)), "mat_shape")
+                nnz = ctypes.c_int64()
+                check(lib.f3r_mat_info(h, None, ctypes.byref(nnz), None, None, None), "mat_info")
+                rptr = np.empty(nr.value + 1, np.int32)
+                idx = np.empty(nnz.value, np.int32)
+                val = np.empty(nnz.value, np.float64)
+                check(lib.f3r_mat_csr_get(h, _p(rptr), _p(idx), _p(val)), "mat_csr_get")
+            finally:
+                lib.f3r_mat_destroy(h)
+            return CSR(nr.value, rptr, idx, val, ncols=nc.value)
+
+        @staticmethod
+        def read_vector(path):
+            """an "array real general" file of one column"""
+            n = ctypes.c_int64()
+            check(lib.f3r_mm_read_vector(str(path).encode(), 0, None, ctypes.byref(n)), "mm_read_vector")
+            out = np.empty(n.value, np.float64)
+            check(lib.f3r_mm_read_vector(str(path).encode(), out.size, _p(out), ctypes.byref(n)), "mm_read_vector")
+            return out
+
+        @staticmethod
+        def read_system(path, seed=0):
+            """(A, b) of io::mm::read_system: b from <name>_b.mtx when that file
+            exists, else U[0, 1) from std::mt19937_64(seed), the reference
+            drivers' fallback (work/src/f3r.cpp:39)"""
+            import os
+            A = io.mm.read_matrix(path)
+            buf = ctypes.create_string_buffer(len(str(path)) + 8)
+            check(lib.f3r_mm_rhs_path(str(path).encode(), buf, len(buf)), "mm_rhs_path")
+            bpath = buf.value.decode()
+            if os.path.exists(bpath):
+                b = io.mm.read_vector(bpath)
+                if b.size != A.n:
+                    raise F3RError(_capi.ERR_ARG, f"mm.read_system: {bpath} has {b.size} entries, "
+                                                  f"the matrix {A.n} rows")
+            else:
+                b = np.empty(A.n, np.float64)
+                check(lib.f3r_gen_uniform_mt(A.n, 0.0, 1.0, seed, _p(b)), "gen_uniform_mt")
+            return A, b
+
+
 def sell_offset_layout(A, C=64, K=4):
     """host chunk-offset layout of CSR A (sigma = 1): dict of cptr, cwidth,
     iptr, idx, coff, val"""
@@ -482,8 +536,11 @@ class F3R:
 
     def time_kernel(self, which, reps=20, cold=False):
         """mean ms per launch and algorithmic bytes of kernel `which`
-        (0 fp16 SpMV, 1 fp16-value/fp32-vector SpMV, 2 fp32 SpMV, 3 fp64 SpMV);
-        cold: each launch alone, after a 512 MiB pass that empties the caches"""
+        (0 fp16 SpMV, 1 fp16-value/fp32-vector SpMV, 2 fp32 SpMV, 3 fp64 SpMV,
+        4 / 5 the level-3 SpMV + CGS dots / CGS update of the last step,
+        6 / 7 the same over the solve's mix of steps, 8 the fused Richardson
+        step, 9 the innermost entry); cold: each launch alone, after a 512 MiB
+        pass that empties the caches"""
         ms, by = ctypes.c_double(), ctypes.c_double()
         w = which | (0x100 if cold else 0)
         check(lib.f3r_solver_time_kernel(self.handle, w, reps, ctypes.byref(ms), ctypes.byref(by)),

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libf3r_amd.so")
 
 F64, F32, F16, I32 = 0, 1, 2, 3
 OK = 0
+ERR_ARG = -1
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built: run `make -C f3r_amd` or __graft_entry__.build()")
@@ -65,6 +66,12 @@ SIGNATURES = [
     ("f3r_gen_dominant", ctypes.c_int, [c_i32, c_i32, c_i32, c_i32, c_dbl, c_vp, c_vp, c_vp]),
     ("f3r_gen_uniform", ctypes.c_int, [c_i64, c_dbl, c_dbl, ctypes.c_uint64, c_vp]),
     ("f3r_mat_create_csr", ctypes.c_int, [c_vp, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    ("f3r_mat_csr_get", ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    ("f3r_mat_shape", ctypes.c_int, [c_vp, P_i32, P_i32]),
+    ("f3r_mm_read", ctypes.c_int, [c_vp, ctypes.c_char_p, ctypes.POINTER(c_vp)]),
+    ("f3r_mm_read_vector", ctypes.c_int, [ctypes.c_char_p, c_i64, c_vp, P_i64]),
+    ("f3r_mm_rhs_path", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, c_i64]),
+    ("f3r_gen_uniform_mt", ctypes.c_int, [c_i64, c_dbl, c_dbl, ctypes.c_uint64, c_vp]),
     ("f3r_mat_to_sell", ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i32, c_i32, c_i32, ctypes.POINTER(c_vp)]),
     ("f3r_sell_layout", ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                         c_vp]),

@@ -6,10 +6,10 @@
 // Same arguments and the same CSV result line as work/src/f3r-gpu.cpp:45-54,
 // 268-272, 97-102:
 //   Problem,Method,Prec,M2,M3,M4,W,Precond,ACC,Time,Iter,ImplRes,ExplRes
-// The matrix is read as MatrixMarket coordinate (symmetric entries expanded,
-// explicit off-diagonal zeros dropped, columns sorted per row;
-// sparse/matrix/io.hpp:42-64,129-192), the right-hand side from
-// <name>_b.mtx when present, else U[0,1) from std::mt19937_64(0)
+// The matrix is read by the library's MatrixMarket reader (f3r_mm_read:
+// symmetric entries expanded, explicit off-diagonal zeros dropped, columns
+// sorted per row; sparse/matrix/io.hpp:42-64,129-192), the right-hand side
+// from <name>_b.mtx when present, else U[0,1) from std::mt19937_64(0)
 // (work/src/f3r-gpu.cpp:60).  The matrix is scaled (algorithm::scaling) and
 // the solve is timed alone, averaged over <reps> repetitions.
 #include <algorithm>
@@ -17,9 +17,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <random>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -43,76 +40,27 @@ struct Csr {
   std::vector<double> val;
 };
 
-bool read_mm(const std::string &path, Csr &A) {
-  std::ifstream f(path);
-  if (!f) return false;
-  std::string line;
-  std::getline(f, line);
-  bool sym = line.find("symmetric") != std::string::npos;
-  while (std::getline(f, line))
-    if (!line.empty() && line[0] != '%') break;
-  long nr, nc, nl;
-  std::istringstream hdr(line);
-  hdr >> nr >> nc >> nl;
-  std::vector<int32_t> r, c;
-  std::vector<double> v;
-  r.reserve(sym ? 2 * nl : nl);
-  c.reserve(sym ? 2 * nl : nl);
-  v.reserve(sym ? 2 * nl : nl);
-  for (long k = 0; k < nl; ++k) {
-    long i, j;
-    double x;
-    f >> i >> j >> x;
-    --i;
-    --j;
-    if (x == 0.0 && i != j) continue;
-    r.push_back((int32_t)i);
-    c.push_back((int32_t)j);
-    v.push_back(x);
-    if (sym && i != j) {
-      r.push_back((int32_t)j);
-      c.push_back((int32_t)i);
-      v.push_back(x);
+// the library's MatrixMarket reader (f3r_mm_read: io::mm::read_matrix +
+// io::CSR of sparse/matrix/io.hpp) into host arrays
+int read_mm(const std::string &path, Csr &A) {
+  f3r_mat M;
+  int st = f3r_mm_read(nullptr, path.c_str(), &M);
+  if (st != F3R_OK) return st;
+  int32_t nr, nc;
+  int64_t nnz;
+  if ((st = f3r_mat_shape(M, &nr, &nc)) == F3R_OK && (st = f3r_mat_info(M, nullptr, &nnz, nullptr, nullptr, nullptr)) == F3R_OK) {
+    if (nr != nc) {
+      st = F3R_ERR_ARG;
+    } else {
+      A.n = nr;
+      A.rptr.resize(nr + 1);
+      A.idx.resize(nnz);
+      A.val.resize(nnz);
+      st = f3r_mat_csr_get(M, A.rptr.data(), A.idx.data(), A.val.data());
     }
   }
-  A.n = (int32_t)nr;
-  A.rptr.assign(A.n + 1, 0);
-  for (int32_t i : r) A.rptr[i + 1]++;
-  for (int32_t i = 0; i < A.n; ++i) A.rptr[i + 1] += A.rptr[i];
-  A.idx.resize(r.size());
-  A.val.resize(r.size());
-  std::vector<int32_t> fill(A.rptr.begin(), A.rptr.end() - 1);
-  for (size_t k = 0; k < r.size(); ++k) {
-    int32_t p = fill[r[k]]++;
-    A.idx[p] = c[k];
-    A.val[p] = v[k];
-  }
-  std::vector<std::pair<int32_t, double>> row;
-  for (int32_t i = 0; i < A.n; ++i) {
-    row.clear();
-    for (int32_t k = A.rptr[i]; k < A.rptr[i + 1]; ++k) row.push_back({A.idx[k], A.val[k]});
-    std::sort(row.begin(), row.end(), [](auto &a, auto &b) { return a.first < b.first; });
-    for (size_t q = 0; q < row.size(); ++q) {
-      A.idx[A.rptr[i] + q] = row[q].first;
-      A.val[A.rptr[i] + q] = row[q].second;
-    }
-  }
-  return true;
-}
-
-bool read_mm_vector(const std::string &path, std::vector<double> &b) {
-  std::ifstream f(path);
-  if (!f) return false;
-  std::string line;
-  std::getline(f, line);
-  while (std::getline(f, line))
-    if (!line.empty() && line[0] != '%') break;
-  long nr, nc;
-  std::istringstream hdr(line);
-  hdr >> nr >> nc;
-  b.resize(nr);
-  for (long i = 0; i < nr; ++i) f >> b[i];
-  return true;
+  f3r_mat_destroy(M);
+  return st;
 }
 
 }  // namespace
@@ -146,22 +94,25 @@ int main(int argc, char **argv) {
     CK(f3r_gen_stencil27(nx, nx, nx, beta, A.rptr.data(), A.idx.data(), A.val.data()));
     name = "stencil27_" + std::to_string(nx);
   } else {
-    if (!read_mm(src, A)) {
-      std::fprintf(stderr, "cannot read %s\n", src.c_str());
-      return 1;
-    }
+    CK(read_mm(src, A));
     std::string base = src.substr(src.find_last_of('/') + 1);
     name = base.substr(0, base.size() - 4);
-    std::string bpath = src.substr(0, src.size() - 4) + "_b.mtx";
-    read_mm_vector(bpath, b);
+    std::vector<char> bpath(src.size() + 8);
+    CK(f3r_mm_rhs_path(src.c_str(), bpath.data(), (int64_t)bpath.size()));
+    int64_t nb = 0;
+    if (f3r_mm_read_vector(bpath.data(), 0, nullptr, &nb) == F3R_OK) {
+      b.resize(nb);
+      CK(f3r_mm_read_vector(bpath.data(), nb, b.data(), &nb));
+      if ((int32_t)b.size() != A.n) {
+        std::fprintf(stderr, "%s: %lld entries for %d rows\n", bpath.data(), (long long)nb, A.n);
+        return 1;
+      }
+    }
   }
-  if (b.empty()) {  // U[0,1) from mt19937_64(0), as the reference's fallback
+  if (b.empty()) {  // U[0,1) from mt19937_64(0), the reference drivers' fallback
     b.resize(A.n);
-    std::mt19937_64 eng(0);
-    std::uniform_real_distribution<double> dist(0, 1);
-    for (auto &x : b) x = dist(eng);
+    CK(f3r_gen_uniform_mt(A.n, 0.0, 1.0, 0, b.data()));
   }
-  if ((int32_t)b.size() != A.n) return 1;
 
   CK(f3r_csr_scaling(A.n, A.rptr.data(), A.idx.data(), A.val.data()));
   f3r_ctx ctx;

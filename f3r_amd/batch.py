@@ -184,6 +184,7 @@ def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None):
            "rows_total": int(sum(A.n for A in As)), "wall_s": float(min(walls)), "walls_s": walls,
            "setup_s": setup_s, "all_solved": all(x.is_solved for x in res),
            "outer_steps_max": int(max(x.res_iter for x in res)), "outer_steps_mean": float(np.mean([x.res_iter for x in res])),
+           "outer_steps_per_segment": [int(x.res_iter) for x in res],
            "worst_true_rel_res": worst}
     if log:
         log(f"lockstep: {segs} solves in {out['wall_s'] * 1e3:.1f} ms wall (best of 3), all solved "

@@ -1034,6 +1034,15 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
   F3R_CHECK(p.m1 >= 1 && p.m1 <= 120 && p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16 && p.m4 >= 1 &&
                 p.m4 <= 16 && p.c >= 1 && p.restart >= 1,
             F3R_ERR_ARG, "batch: m1 in [1, 120], m2, m3, m4 in [1, 16], c, restart >= 1");
+  // the lockstep nest is the adaptive-weight HALF F3R nest only
+  F3R_CHECK(p.variant == 0 && p.static_weight == 0.0 && p.depth == 0, F3R_ERR_ARG,
+            "batch: only the HALF F3R nest with adaptive weights (variant 0, static_weight 0, depth 0)");
+  // k_sell_mr reads a chunk's header once per wavefront (C >= 64) and a tile
+  // holds whole chunks (C <= kTile); K selects a built kernel
+  F3R_CHECK(p.C == 0 || (p.C >= 64 && p.C <= kTile && (p.C & (p.C - 1)) == 0), F3R_ERR_ARG,
+            "batch: C must be a power of two in [64, 2048] (0 = 64)");
+  F3R_CHECK(p.K == 0 || p.K == 1 || p.K == 2 || p.K == 4 || p.K == 8, F3R_ERR_ARG,
+            "batch: K must be 1, 2, 4 or 8 (0 = 4)");
   auto B = std::make_unique<BatchSolver>();
   B->ctx = ctx;
   B->p = p;
@@ -1154,6 +1163,11 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
       B->sh16 = upload_sell_shared(c, h16, false);
       B->A16 = upload_sell_values(c, h16, F3R_F16);
     }
+    // k_sell_mr gathers x (fp64 at most) of offset chunks through a buffer
+    // resource whose size and byte offsets are 32-bit
+    const int64_t bias = std::max(B->sh->off_bias, B->sh16->off_bias);
+    F3R_CHECK(NR == 1 || (N + bias) * (int64_t)sizeof(double) < ((int64_t)1 << 31), F3R_ERR_RANGE,
+              "batch: the padded rows of all segments exceed the 2^28 a multi-copy SpMV can address");
   }
   // the SDAINV factors: diagonal-only rows as a vector, the rest as CSR rows
   // a system's right-hand sides share its diagonals when they share its

@@ -138,10 +138,11 @@ struct SellShared {  // index data shared by all precisions of one matrix
   int32_t offset_chunks = 0;
   int64_t offset_slots = 0, explicit_nnz = 0;
   int32_t off_bias = 0;
-  // bytes one SpMV must move: values of offset chunks, values + int32 index
-  // of the other nonzeros, x and y
-  double format_bytes(int sv, int sx, int sy) const {
-    return (double)offset_slots * sv + (double)explicit_nnz * (sv + 4) + (double)n * (sx + sy);
+  // algorithmic bytes of one SpMV: every nonzero's value once (no padding),
+  // an int32 index only for the nonzeros of int32-index chunks (offset
+  // chunks read none), x and y once
+  double algo_bytes(int sv, int sx, int sy) const {
+    return (double)(nnz - explicit_nnz) * sv + (double)explicit_nnz * (sv + 4) + (double)n * (sx + sy);
   }
   std::vector<int32_t> h_perm;     // host copy of the permutation
 };

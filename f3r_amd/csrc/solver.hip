@@ -1578,46 +1578,109 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
   F3R_API_BEGIN
   const bool cold = (which & F3R_TIME_COLD) != 0;
   which &= ~F3R_TIME_COLD;
-  F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 5, F3R_ERR_ARG, "time_kernel: bad argument");
+  F3R_CHECK(S && ms && reps > 0 && which >= 0 && which <= 9, F3R_ERR_ARG, "time_kernel: bad argument");
   F3R_CHECK(which < 4 || (S->s->p.variant == 0 && S->s->L3f.m <= 4), F3R_ERR_UNSUPPORTED,
             "time_kernel: the Arnoldi-step kernels need the HALF variant with m3 <= 4");
   Solver *s = S->s;
+  F3R_CHECK(which < 4 || s->p.depth == 0, F3R_ERR_UNSUPPORTED, "time_kernel: the Arnoldi-step kernels of the F3R nest");
   SellArgs a = s->args(), a16 = s->argsv<half_t>();
   F3R_CHECK(s->p.variant == 0 || which == 3, F3R_ERR_UNSUPPORTED, "time_kernel: fp16/fp32 kernels need the HALF variant");
   half_t *x16 = s->hout->as<half_t>(), *y16 = s->tt->as<half_t>();
   float *x32 = s->L2f.V.empty() ? nullptr : s->L2f.V[0], *y32 = s->L2f.V.empty() ? nullptr : s->L2f.V[1];
   double *x64 = s->x_int->as<double>(), *y64 = s->V1->as<double>();
   double by = 0;
+  // launches per launch() call: the Arnoldi-step mixes run steps 0 .. m3 - 1
+  const int per = (which == 6 || which == 7) ? s->L3f.m : 1;
+  const bool diag = s->P->kind == 0 && s->P->diag;
+  F3R_CHECK(which < 8 || diag, F3R_ERR_UNSUPPORTED, "time_kernel: the fused Richardson kernels need diagonal SDAINV");
+  DevPtr one;  // 1/||w|| = 1 for the entry kernel's in-place normalisation
+  if (which == 9) {
+    one = dev_alloc(sizeof(float));
+    const float f1 = 1.f;
+    F3R_HIP(hipMemcpyAsync(one->p, &f1, sizeof(float), hipMemcpyHostToDevice, s->st()));
+  }
   auto launch = [&]() {
     switch (which) {
       case 0:  // fp16 values, fp16 x and y (the metric's fp16 SpMV)
         launch_sell<half_t, half_t, float>(s->sh16->K, a16, s->A16->as<half_t>(), x16,
                                            EpiStore<half_t, float>{y16, nullptr}, s->st());
-        by = s->sh16->format_bytes(2, 2, 2);
+        by = s->sh16->algo_bytes(2, 2, 2);
         break;
       case 1:  // fp16 values, fp32 x and y (level 3)
         launch_sell<half_t, float, float>(s->sh16->K, a16, s->A16->as<half_t>(), x32,
                                           EpiStore<float, float>{y32, nullptr}, s->st());
-        by = s->sh16->format_bytes(2, 4, 4);
+        by = s->sh16->algo_bytes(2, 4, 4);
         break;
       case 2:  // fp32 (level 2)
         launch_sell<float, float, float>(s->sh->K, a, s->A32->as<float>(), x32,
                                          EpiStore<float, float>{y32, nullptr}, s->st());
-        by = s->sh->format_bytes(4, 4, 4);
+        by = s->sh->algo_bytes(4, 4, 4);
         break;
       case 3:  // fp64 (outer level)
         launch_sell<double, double, double>(s->sh->K, a, s->A64->as<double>(), x64,
                                             EpiStore<double, double>{y64, nullptr}, s->st());
-        by = s->sh->format_bytes(8, 8, 8);
+        by = s->sh->algo_bytes(8, 8, 8);
         break;
       case 4:  // level-3 Arnoldi step j = m3 - 1: fp16 SpMV of Z_j + CGS dots with V_0..V_j, and their finish
         s->template spmv_dots<float, half_t, 4>(s->L3f, s->A16->as<half_t>(), s->L3f.m - 1);
-        by = s->sh16->format_bytes(2, s->L3f.Zh.empty() ? 4 : 2, 4) + 4.0 * s->n * s->L3f.m;
+        by = s->sh16->algo_bytes(2, s->L3f.Zh.empty() ? 4 : 2, 4) + 4.0 * s->n * s->L3f.m;
         break;
       case 5:  // level-3 CGS update + norm + Givens of that step
         s->template cgs_update<float, 4>(s->L3f, s->L3f.m - 1);
         by = 4.0 * s->n * (s->L3f.m + 2);
         break;
+      case 6: {  // the solve's mix of level-3 SpMV + CGS dots launches, steps j = 0 .. m3 - 1 (no finish)
+        const int m = s->L3f.m;
+        by = 0;
+        for (int j = 0; j < m; ++j) {
+          Cols<float, 4> c{};
+          for (int k = 0; k <= j; ++k) c.p[k] = s->L3f.V[k];
+          EpiDots<float, 4> epi{s->L3f.V[j + 1], c, j + 1, s->L3f.Hp() + (int64_t)j * s->L3f.ldh(),
+                                s->template partial<float>(), s->counter(0)};
+          if (!s->L3f.Zh.empty()) {
+            epi.xs = 1.f / s->range_s;
+            launch_sell<half_t, half_t, float>(s->sh16->K, a16, s->A16->as<half_t>(), s->L3f.Zh[j], epi, s->st(),
+                                               s->spmv_grid);
+          } else {
+            launch_sell<half_t, float, float>(s->sh16->K, a16, s->A16->as<half_t>(), s->L3f.Z[j], epi, s->st(),
+                                              s->spmv_grid);
+          }
+          // values, Z_j gathered, w written, V_0 .. V_j read
+          by += s->sh16->algo_bytes(2, s->L3f.Zh.empty() ? 4 : 2, 4) + 4.0 * s->n * (j + 1);
+        }
+        by /= m;
+        break;
+      }
+      case 7: {  // the solve's mix of CGS update + norm + Givens launches, steps j = 0 .. m3 - 1
+        const int m = s->L3f.m;
+        by = 0;
+        for (int j = 0; j < m; ++j) {
+          s->template cgs_update<float, 4>(s->L3f, j);
+          by += 4.0 * s->n * (j + 3);  // V_0 .. V_j read, w read and written
+        }
+        by /= m;
+        break;
+      }
+      case 8: {  // the fused Richardson step (residual SpMV + diagonal SDAINV + weighted update), fp16 out
+        RichDev *rs = s->rich->as<RichDev>();
+        EpiL4Diag epi{s->hin->as<half_t>(), s->hout->as<half_t>(), s->P->dwz->as<uint32_t>(), &rs->ome[1],
+                      nullptr, nullptr, s->hout2->as<half_t>(), nullptr, 1.f / s->range_s};
+        launch_sell<half_t, half_t, float>(s->sh16->K, a16, s->A16->as<half_t>(), s->hout->as<half_t>(), epi,
+                                           s->st());
+        // values, h gathered, hin / h / (w, z) read per row, h' written
+        by = s->sh16->algo_bytes(2, 2, 2) + (double)s->n * (2 + 2 + 4);
+        break;
+      }
+      case 9: {  // the innermost entry: normalise the basis vector, fp16 input, first weighted M application
+        const int64_t nn = s->n;
+        float *in = s->L3f.V[s->L3f.m];
+        hipLaunchKernelGGL(k_l4_entry<true>, dim3(s->grid()), dim3(kBlock), 0, s->st(), nn, in,
+                           (const float *)one->as<float>(), s->range_s, s->hin->as<half_t>(),
+                           s->hout2->as<half_t>(), (const uint32_t *)s->P->dwz->as<uint32_t>(),
+                           (const RichDev *)s->rich->as<RichDev>(), 1);
+        by = (double)s->n * (4 + 4 + 4 + 2 + 2);  // in read and written, (w, z) read, hin and hout written
+        break;
+      }
     }
   };
   hipEvent_t e0, e1;
@@ -1640,7 +1703,7 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
       F3R_HIP(hipEventElapsedTime(&t, e0, e1));
       tot += t;
     }
-    *ms = tot / reps;
+    *ms = tot / ((double)reps * per);
   } else {
     launch();
     F3R_HIP(hipEventRecord(e0, s->st()));
@@ -1649,7 +1712,7 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
     F3R_HIP(hipEventSynchronize(e1));
     float t = 0;
     F3R_HIP(hipEventElapsedTime(&t, e0, e1));
-    *ms = (double)t / reps;
+    *ms = (double)t / ((double)reps * per);
   }
   if (bytes) *bytes = by;
   (void)hipEventDestroy(e0);

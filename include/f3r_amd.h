@@ -113,6 +113,37 @@ int f3r_gen_uniform(int64_t n, double lo, double hi, uint64_t seed, double *out)
 int f3r_mat_create_csr(f3r_ctx ctx, int32_t nrows, int32_t ncols, int64_t nnz,
                        const int32_t *rptr, const int32_t *col, const double *val,
                        f3r_mat *A);
+/* host copy of a matrix's CSR (rptr[nrows+1], col[nnz], val[nnz]; sizes from
+ * f3r_mat_info); any output may be NULL.  replaces: direct access to
+ * CSR<double, host>::{rptr, idx, val} (csr.hpp:12-30) */
+int f3r_mat_csr_get(f3r_mat A, int32_t *rptr, int32_t *col, double *val);
+/* rows and columns of a matrix carrying its CSR (spmat::get_shape,
+ * sparse/matrix/base.hpp) */
+int f3r_mat_shape(f3r_mat A, int32_t *nrows, int32_t *ncols);
+
+/* ------------------------------------------------------ MatrixMarket ---- */
+/* io::mm::read_matrix + io::CSR(coo) (sparse/matrix/io.hpp:42-64,129-192):
+ * a coordinate file into a CSR matrix (ctx may be NULL: host-only).  The
+ * matrix is symmetric only when the header's fifth token is exactly
+ * "symmetric" (io.hpp:133-137; skew-symmetric files are read as general);
+ * off-diagonal explicit zeros are dropped; symmetric entries are mirrored;
+ * rows sorted by column.  Errors instead of exit(1): F3R_ERR_ARG for an
+ * unreadable file, header or entry outside the declared shape;
+ * F3R_ERR_UNSUPPORTED for pattern and complex files. */
+int f3r_mm_read(f3r_ctx ctx, const char *path, f3r_mat *A);
+/* io::mm::_read_matrix_market_b (io.hpp:193-217): an "array ... general"
+ * file of one column.  *n = its length; values copied when out != NULL
+ * (cap >= *n).  F3R_ERR_ARG when the file is missing or not of that form. */
+int f3r_mm_read_vector(const char *path, int64_t cap, double *out, int64_t *n);
+/* the right-hand side's path: "_b" inserted before the last 4 characters
+ * (read_system, io.hpp:108-109) */
+int f3r_mm_rhs_path(const char *path, char *out, int64_t cap);
+/* the reference drivers' right-hand side when <name>_b.mtx is absent:
+ * std::uniform_real_distribution(lo, hi) over std::mt19937_64(seed)
+ * (vector::random<0,1>(std::mt19937_64(0)), work/src/f3r-gpu.cpp:60,
+ * core/tensor.hpp:253-262) */
+int f3r_gen_uniform_mt(int64_t n, double lo, double hi, uint64_t seed, double *out);
+
 /* SELL<w, T, device>(CSR) (sell.hpp:35-97) and SELLsig (sell.hpp:99-166):
  * C rows per chunk (multiple of 16), sigma = sort window (1 = unsorted),
  * K = slots interleaved per row (1, 2, 4 or 8; K = 1 is the reference layout).
@@ -267,10 +298,14 @@ int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len);
 int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out);
 /* mean device time per launch of one of the solver's kernels (which: SpMV
  * with 0 fp16 values and vectors, 1 fp16 values / fp32 vectors, 2 fp32, 3 fp64;
- * 4 the level-3 SpMV + CGS dots and 5 the CGS update of its last step), measured
- * with HIP events on the solver's stream over `reps` launches; `bytes` = the
- * bytes the layout must move per launch (values and int32 indices of the
- * nonzeros of int32-index chunks, every value slot of chunk-offset chunks, x, y).
+ * 4 the level-3 SpMV + CGS dots and 5 the CGS update of its last step; the
+ * solve's own mixes of level-3 Arnoldi steps j = 0 .. m3 - 1: 6 the SpMV + CGS
+ * dots, 7 the CGS update + norm + Givens; with diagonal SDAINV factors, 8 the
+ * fused Richardson step and 9 the innermost entry), measured with HIP events
+ * on the solver's stream over `reps` rounds of back-to-back launches; `bytes` =
+ * the algorithmic bytes of one launch (each nonzero's value once, padding
+ * excluded; an int32 index only for the nonzeros of int32-index chunks, since
+ * chunk-offset chunks read none; each vector once), averaged over the mix.
  * which | F3R_TIME_COLD: caches cold, each launch timed alone after a pass over
  * 512 MiB that evicts the operands from the L2s and the last-level cache. */
 #define F3R_TIME_COLD 0x100

@@ -16,12 +16,16 @@
 //   spmv    A.bin x.bin mode y.bin             SELL<32,T,host>::operate, mode:
 //            64 (fp64), 32 (fp32), 1616 (fp16 values, fp16 x, fp16 sums),
 //            1632 (fp16 values, fp32 x, fp32 sums)
+//   mm      A.mtx out.bin [b.bin]              io::mm::read_system + io::CSR(coo);
+//            b.bin gets the right-hand side (<name>_b.mtx, else U[0,1) of
+//            std::mt19937_64(0), as work/src/f3r.cpp:39 asks for it)
 //   f3r     A.bin b.bin prec m2 m3 m4 c restart acc x.bin hist.bin
 //            nested solve composed from the reference's FGMRES / Restarted /
 //            SELL<32,T,host> / sdainv, with the adaptive Richardson level of
 //            work/src/f3r-gpu.cpp:229-260 on the host tag; prec = 64|32|16.
 #include "umineko-sparse/matrix/algorithm.hpp"
 #include "umineko-sparse/matrix/csr.hpp"
+#include "umineko-sparse/matrix/io.hpp"
 #include "umineko-sparse/matrix/sell.hpp"
 #include "umineko-sparse/preconditioner/ainv.hpp"
 #include "umineko-sparse/preconditioner/ilu.hpp"
@@ -144,6 +148,22 @@ int main(int argc, char **argv) {
       S16.operate(x, y);
       write_raw(argv[5], y.raw(), n);
     }
+  } else if (cmd == "mm") {
+    auto [coo, rhs] = io::mm::read_system(argv[2], [](vector<double, host> v) {
+      v.random<0, 1>(std::mt19937_64(0));
+    });
+    auto C = io::CSR<double>(coo);
+    FILE *f = std::fopen(argv[3], "wb");
+    int32_t n = C.nrows(), nc = C.ncols();
+    int64_t nnz = C.nnz();
+    std::fwrite(&n, 4, 1, f);
+    std::fwrite(&nc, 4, 1, f);
+    std::fwrite(&nnz, 8, 1, f);
+    std::fwrite(C.rptr.raw(), 4, n + 1, f);
+    std::fwrite(C.idx.raw(), 4, nnz, f);
+    std::fwrite(C.val.raw(), 8, nnz, f);
+    std::fclose(f);
+    if (argc > 4) write_raw(argv[4], rhs.raw(), (int64_t)rhs.size(0));
   } else if (cmd == "f3r") {
     int prec = std::atoi(argv[4]);
     if (prec == 64) return run_f3r<double>(argc, argv);

@@ -7,7 +7,7 @@ generator matrix/hpcg.cpp) on small inputs, single-threaded so that every
 reduction runs in sequential order, and stores inputs and outputs as .npz
 data.  Run in the container that has the reference checkout:
 
-    make -C oracle/ref_harness && python tests/golden/make_golden.py
+    make -C oracle/ref_harness && python tests/golden/make_golden.py [units f3r steps mm cpu]
 """
 import os
 import subprocess
@@ -21,6 +21,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 import problems as PB  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
 REF = os.path.join(ROOT, "oracle", "_ref")
 ENV = dict(os.environ, OMP_NUM_THREADS="1")
 
@@ -96,6 +97,78 @@ def f3r_fixtures(T):
         print("wrote f3r", name)
 
 
+def step_cases():
+    """Problems on which the reference's nest needs many outer steps, so that
+    a +-10% window on its outer-step count is at least one step: the inner
+    levels are shortened (m2, m3, m4 are arguments of the reference's drivers,
+    work/suite-gpu.py:49), and the 64^3 case is the CPU-path configuration
+    (symmetric 27-point, b = A*1, restart 3).  Each entry: name, problem
+    description (regenerated by the tests), (m2, m3, m4), restart, precisions."""
+    yield ("s27b24", {"kind": "stencil27", "nx": 24, "beta": 0.5, "b": "uniform", "seed": 5}, (2, 2, 1), 20,
+           (16, 32, 64))
+    yield ("s27b24", {"kind": "stencil27", "nx": 24, "beta": 0.5, "b": "uniform", "seed": 5}, (2, 2, 2), 20, (16,))
+    yield ("ln24", {"kind": "convdiff7", "nx": 24, "sigma": 1.5, "corr": 8.0, "peclet": 0.0, "gseed": 3,
+                    "b": "uniform", "seed": 5}, (2, 2, 1), 20, (16,))
+    yield ("ln24", {"kind": "convdiff7", "nx": 24, "sigma": 1.5, "corr": 8.0, "peclet": 0.0, "gseed": 3,
+                    "b": "uniform", "seed": 5}, (3, 2, 1), 20, (16,))
+    yield ("cd24", {"kind": "convdiff7", "nx": 24, "sigma": 2.0, "corr": 8.0, "peclet": 10.0, "gseed": 3,
+                    "b": "uniform", "seed": 5}, (2, 2, 1), 20, (16,))
+    yield ("c1s27_64", {"kind": "stencil27", "nx": 64, "beta": 0.0, "b": "rowsum"}, (8, 4, 2), 3, (16, 64))
+
+
+def f3r_step_fixtures(T):
+    """Outer-step counts and residual histories of the reference's nest on
+    problems that need 3 to 14 outer steps (one .npz, keys per case)."""
+    import json
+    out = {}
+    names = []
+    for name, d, ms, restart, precs in step_cases():
+        n, rp, ix, v, b = PB.from_desc(d)
+        PB.write_csr_bin(f"{T}/A.bin", n, rp, ix, v)
+        run("ref_driver", "scale", f"{T}/A.bin", f"{T}/As.bin")
+        _, _, _, vs = PB.read_csr_bin(f"{T}/As.bin")
+        PB.write_vec_bin(f"{T}/b.bin", b)
+        key = f"{name}_m{ms[0]}{ms[1]}{ms[2]}"
+        names.append(key)
+        out[f"{key}_desc"] = np.array(json.dumps(d))
+        out[f"{key}_m"] = np.array(ms + (64, restart), np.int32)
+        out[f"{key}_sha_unscaled"] = PB.csr_sha256(rp, ix, v)
+        out[f"{key}_sha_scaled"] = PB.csr_sha256(rp, ix, vs)
+        out[f"{key}_sha_b"] = np.frombuffer(__import__("hashlib").sha256(b.tobytes()).digest(), np.uint8)
+        for prec in precs:
+            line = run("ref_driver", "f3r", f"{T}/As.bin", f"{T}/b.bin", prec, *ms, 64, restart, 1.0,
+                       f"{T}/x.bin", f"{T}/h.bin").strip()
+            s, it, impl, expl = line.split(",")
+            out[f"{key}_p{prec}_res"] = np.array([float(s), float(it), float(impl), float(expl)])
+            out[f"{key}_p{prec}_hist"] = PB.read_vec_bin(f"{T}/h.bin", np.float64)
+            print("step case", key, prec, line, flush=True)
+    out["cases"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, "f3r_ref_steps.npz"), **out)
+    print("wrote f3r_ref_steps")
+
+
+MM_FILES = ("general", "symmetric", "zeros", "skew", "integer", "rect")
+
+
+def mm_fixtures(T):
+    """The reference's MatrixMarket reader (io::mm::read_system + io::CSR) on
+    the small files of tests/golden/mm/: CSR arrays and right-hand sides."""
+    out = {}
+    for name in MM_FILES:
+        path = os.path.join(HERE, "mm", name + ".mtx")
+        run("ref_driver", "mm", path, f"{T}/A.bin", f"{T}/b.bin")
+        with open(f"{T}/A.bin", "rb") as f:
+            n, nc = np.frombuffer(f.read(8), np.int32)
+            nnz = int(np.frombuffer(f.read(8), np.int64)[0])
+            out[f"{name}_shape"] = np.array([n, nc], np.int32)
+            out[f"{name}_rptr"] = np.frombuffer(f.read(4 * (n + 1)), np.int32).copy()
+            out[f"{name}_idx"] = np.frombuffer(f.read(4 * nnz), np.int32).copy()
+            out[f"{name}_val"] = np.frombuffer(f.read(8 * nnz), np.float64).copy()
+        out[f"{name}_b"] = PB.read_vec_bin(f"{T}/b.bin", np.float64)
+    np.savez_compressed(os.path.join(HERE, "mm_ref.npz"), **out)
+    print("wrote mm_ref")
+
+
 def cpu_driver_fixture(T):
     """The reference's CPU driver work/src/f3r.cpp (BJILU0, 112 blocks) on the
     reference generator's 32^3 HPCG matrix with a seeded right-hand side."""
@@ -128,7 +201,15 @@ def cpu_driver_fixture(T):
 
 
 if __name__ == "__main__":
+    which = sys.argv[1:] or ["units", "f3r", "steps", "mm", "cpu"]
     with tempfile.TemporaryDirectory() as T:
-        unit_fixtures(T)
-        f3r_fixtures(T)
-        cpu_driver_fixture(T)
+        if "units" in which:
+            unit_fixtures(T)
+        if "f3r" in which:
+            f3r_fixtures(T)
+        if "steps" in which:
+            f3r_step_fixtures(T)
+        if "mm" in which:
+            mm_fixtures(T)
+        if "cpu" in which:
+            cpu_driver_fixture(T)

@@ -72,6 +72,32 @@ def random_sparse(n, per_row, seed, diag_dom=1.5, symmetric_pattern=False):
     return _assemble(n, np.array(R), np.array(C), np.array(V))
 
 
+def from_desc(d):
+    """(n, rptr, idx, val, b) of a problem described by a dict (the step-count
+    fixtures of tests/golden/f3r_ref_steps.npz): unscaled matrix, and b as the
+    reference driver receives it (it scales the matrix, not b).  The 7-point
+    lognormal matrices come from the library's generator."""
+    if d["kind"] == "stencil27":
+        n, rp, ix, v = stencil27(d["nx"], beta=d["beta"])
+    else:
+        import f3r_amd as F
+        A = F.CSR.convdiff7(d["nx"], sigma=d["sigma"], corr=d["corr"], peclet=d["peclet"], seed=d["gseed"])
+        n, rp, ix, v = A.n, A.rptr, A.idx, A.val
+    if d["b"] == "rowsum":
+        b = np.add.reduceat(v, rp[:-1])  # b = A * 1
+    else:
+        b = np.random.default_rng(d["seed"]).uniform(-1, 1, n)
+    return n, rp, ix, v, b
+
+
+def csr_sha256(rp, ix, v):
+    import hashlib
+    h = hashlib.sha256()
+    for a in (np.asarray(rp, np.int32), np.asarray(ix, np.int32), np.asarray(v, np.float64)):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return np.frombuffer(h.digest(), np.uint8)
+
+
 def to_dense(n, rptr, idx, val):
     D = np.zeros((n, n))
     for r in range(n):

@@ -233,6 +233,41 @@ def test_f3r_bitexact_against_reference_composition(golden, case, prec, variant)
     assert np.array_equal(hist[:len(h)], h)
 
 
+def _step_cases():
+    import json
+    g = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "f3r_ref_steps.npz"),
+                     allow_pickle=False))
+    out = []
+    for key in g["cases"]:
+        for prec, variant in ((16, 0), (32, 1), (64, 2)):
+            if f"{key}_p{prec}_res" in g:
+                out.append((str(key), prec, variant))
+    return out
+
+
+@pytest.mark.parametrize("key,prec,variant", _step_cases())
+def test_f3r_step_counts_bitexact_against_reference(golden, key, prec, variant):
+    # problems on which the reference's nest needs 3 to 14 outer steps
+    # (shortened inner levels, and the 64^3 CPU-path configuration with
+    # b = A*1 and restart 3): outer count, implicit residual and the whole
+    # residual history equal to the reference's own run
+    import json
+    g = golden("f3r_ref_steps")
+    d = json.loads(str(g[f"{key}_desc"]))
+    m2, m3, m4, c, restart = (int(t) for t in g[f"{key}_m"])
+    n, rp, ix, v, b = PB.from_desc(d)
+    assert PB.csr_sha256(rp, ix, v).tolist() == g[f"{key}_sha_unscaled"].tolist()
+    vs = O.scaling(n, rp, ix, v)
+    assert PB.csr_sha256(rp, ix, vs).tolist() == g[f"{key}_sha_scaled"].tolist()
+    _, r, hist = O.f3r_solve(n, rp, ix, vs, b, variant=variant, ref_arith=1, nthreads=1, m2=m2, m3=m3, m4=m4,
+                             c=c, restart=restart)
+    s, it, impl, _ = g[f"{key}_p{prec}_res"]
+    assert r.is_solved == int(s) and r.res_iter == int(it)
+    assert r.res_nrm2 / r.nrm_b == impl
+    h = g[f"{key}_p{prec}_hist"]
+    assert np.array_equal(hist[:len(h)], h)
+
+
 @pytest.mark.parametrize("prec,variant", [(64, 2), (32, 1), (16, 0)])
 def test_f3r_bjilu0_matches_reference_cpu_driver(golden, prec, variant):
     g = golden("cpu_driver_hpcg32")
