@@ -173,3 +173,32 @@ def test_lockstep_batch_shared_matrix_copies_identical():
     for key, ((i0, x0), (i1, x1)) in sols.items():
         assert i0 == i1 and np.array_equal(x0, x1), key
     ctx.close()
+
+
+def test_lockstep_rejects_bad_parameters():
+    # every invalid combination is refused before any launch
+    import ctypes
+
+    import f3r_amd as F
+    from f3r_amd import _capi
+    ctx = F.Context(0)
+    A = F.CSR.stencil27(8)
+    F.algorithm.scaling(A)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    for kw in ({"C": 16}, {"C": 32}, {"C": 96}, {"C": 4096}, {"K": 3}, {"K": 16}):
+        with pytest.raises(F.F3RError) as e:
+            F.Batch(ctx, [A, A], [P, P], **kw)
+        assert e.value.status == -1, kw
+    for field, value in (("variant", 1), ("static_weight", 1.0), ("depth", 3)):
+        p = _capi.Params()
+        _capi.lib.f3r_params_default(ctypes.byref(p))
+        setattr(p, field, value)
+        ah = (ctypes.c_void_p * 1)(A.handle(ctx))
+        ph = (ctypes.c_void_p * 1)(P.handle)
+        h = ctypes.c_void_p()
+        assert _capi.lib.f3r_batch_create(ctx.handle, 1, ah, ph, ctypes.byref(p), ctypes.byref(h)) == -1, field
+    # a valid one still builds
+    B = F.Batch(ctx, [A, A], [P, P], C=128, K=8)
+    assert B.info()[1] == 2
+    del B
+    ctx.close()
