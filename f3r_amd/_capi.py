@@ -55,6 +55,7 @@ SIGNATURES = [
     ("f3r_vec_scale", ctypes.c_int, [c_vp, c_dbl]),
     ("f3r_vec_dot", ctypes.c_int, [c_vp, c_vp, P_dbl]),
     ("f3r_vec_nrm2", ctypes.c_int, [c_vp, P_dbl]),
+    ("f3r_vec_dot_noalloc", ctypes.c_int, [c_vp, c_vp, c_vp]),
     ("f3r_csr_scaling", ctypes.c_int, [c_i32, c_vp, c_vp, c_vp]),
     ("f3r_sell_nslots", ctypes.c_int, [c_i32, c_vp, c_i32, c_i32, c_i32, P_i64]),
     ("f3r_gen_stencil27_nnz", ctypes.c_int, [c_i32, c_i32, c_i32, P_i64]),

@@ -75,6 +75,7 @@ struct Context {
   int num_cu = 256;
   Reducer red;
   double *pinned = nullptr;  // host-pinned landing buffer for small reads
+  void *na_scratch = nullptr;  // device address of f3r_vec_dot_noalloc's scratch on this device
 };
 
 struct Vec {

@@ -254,6 +254,42 @@ static int grid_for(int64_t n, int cu) {
   return g < 1 ? 1 : (int)g;
 }
 
+// ------------------------------------------- allocation-free dot scratch ---
+// f3r_vec_dot_noalloc keeps the reduction's arrival counters and per-block
+// partial sums in this file-scope array instead of the context's buffers:
+// [counters of one reduction slot | partials of up to kNaMaxBlocks blocks and
+// their groups].  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes `scratch` itself (a host-side
+// stand-in); it uses the device address from hipGetSymbolAddress, resolved
+// once per device when a context is created there.
+constexpr int kNaMaxBlocks = 1024;
+constexpr size_t kNaCounterBytes = sizeof(unsigned) * kCounterSlot;
+constexpr size_t kNaPartialBytes = sizeof(double) * (kNaMaxBlocks + kNaMaxBlocks / kGroup + 1);
+constexpr size_t kNoAllocScratchBytes = kNaCounterBytes + (kNaPartialBytes + 255) / 256 * 256;
+static_assert(kNaMaxBlocks <= kMaxReduceGrid, "the scratch's counters cover every group");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// the device address of `scratch` on `device` (current), looked up once
+static void *noalloc_scratch(int device) {
+  F3R_CHECK(device >= 0 && device < kMaxDevices, F3R_ERR_RANGE, "more devices than the scratch table holds");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    F3R_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    F3R_CHECK(ptr != nullptr, F3R_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0u;
+}
+
 // ------------------------------------------------------------- context ---
 std::shared_ptr<Context> ctx_create(int device) {
   int count = 0;
@@ -272,6 +308,7 @@ std::shared_ptr<Context> ctx_create(int device) {
   c->red.scalars = dev_alloc(sizeof(double) * 64);
   F3R_HIP(hipMemsetAsync(c->red.counters->p, 0, sizeof(unsigned) * Reducer::kSlots * kCounterSlot, c->stream));
   F3R_HIP(hipHostMalloc((void **)&c->pinned, 4096, hipHostMallocDefault));
+  c->na_scratch = noalloc_scratch(device);
   F3R_HIP(hipStreamSynchronize(c->stream));
   return c;
 }
@@ -384,6 +421,46 @@ double dot_raw(Context *c, int da, const void *pa, int db, const void *pb, int64
 #undef F3R_DOT
   F3R_HIP(hipGetLastError());
   return read_scalar(c, slot);
+}
+
+// dot of two device arrays into *out (device), with the scratch array as
+// the reduction's counters and partials: the same kernel, grid and summation
+// order as dot_raw, so the same bits.  Launches two kernels and allocates
+// nothing.
+static void dot_noalloc(Context *c, int da, const void *pa, int db, const void *pb, int64_t n, double *out) {
+  F3R_CHECK(da != F3R_I32 && db != F3R_I32, F3R_ERR_UNSUPPORTED, "dot_noalloc: int32");
+  F3R_CHECK(c->na_scratch, F3R_ERR_STATE, "dot_noalloc: the context has no scratch address");
+  const int g = reduce_grid(n, c->num_cu);
+  F3R_CHECK(g <= kNaMaxBlocks, F3R_ERR_RANGE, "dot_noalloc: the reduction needs more blocks than the scratch holds");
+  unsigned char *base = static_cast<unsigned char *>(c->na_scratch);
+  unsigned *cnt = reinterpret_cast<unsigned *>(base);
+  void *part = base + kNaCounterBytes;
+  // the counters this reduction uses start at zero (a fill kernel, not a
+  // memset node, so the call can be captured in a graph)
+  const int ng = (g + kGroup - 1) / kGroup;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kBlock), 0, c->stream, cnt, (1 + ng) * kCounterStride);
+#define F3R_DOTNA(TA, TB)                                                                               \
+  if (std::is_same<TA, double>::value || std::is_same<TB, double>::value)                             \
+    hipLaunchKernelGGL((k_dot<double, TA, TB>), dim3(g), dim3(kBlock), 0, c->stream, n, (const TA *)pa, \
+                       (const TB *)pb, (double *)part, cnt, out);                                     \
+  else                                                                                                 \
+    hipLaunchKernelGGL((k_dot<float, TA, TB>), dim3(g), dim3(kBlock), 0, c->stream, n, (const TA *)pa,  \
+                       (const TB *)pb, (float *)part, cnt, out);
+  auto pick_b = [&](auto tagA) {
+    using TA = decltype(tagA);
+    switch (db) {
+      case F3R_F64: { F3R_DOTNA(TA, double) break; }
+      case F3R_F32: { F3R_DOTNA(TA, float) break; }
+      case F3R_F16: { F3R_DOTNA(TA, half_t) break; }
+    }
+  };
+  switch (da) {
+    case F3R_F64: pick_b(double()); break;
+    case F3R_F32: pick_b(float()); break;
+    case F3R_F16: pick_b(half_t()); break;
+  }
+#undef F3R_DOTNA
+  F3R_HIP(hipGetLastError());
 }
 
 double vec_dot(Vec *a, Vec *b) {
@@ -1023,6 +1100,17 @@ int f3r_vec_dot(f3r_vec a, f3r_vec b, double *out) {
   F3R_API_BEGIN
   F3R_CHECK(a && b && out, F3R_ERR_ARG, "null argument");
   *out = vec_dot(&a->v, &b->v);
+  F3R_API_END
+}
+int f3r_vec_dot_noalloc(f3r_vec a, f3r_vec b, f3r_vec out) {
+  F3R_API_BEGIN
+  F3R_CHECK(a && b && out, F3R_ERR_ARG, "dot_noalloc: null argument");
+  Vec *va = &a->v, *vb = &b->v, *vo = &out->v;
+  F3R_CHECK(va->n == vb->n, F3R_ERR_ARG, "dot_noalloc: length mismatch");
+  F3R_CHECK(vo->dtype == F3R_F64 && vo->n >= 1, F3R_ERR_ARG, "dot_noalloc: out must be an fp64 vector of at least one element");
+  F3R_CHECK(va->ctx.get() == vb->ctx.get() && va->ctx.get() == vo->ctx.get(), F3R_ERR_ARG,
+            "dot_noalloc: the vectors of one context");
+  dot_noalloc(va->ctx.get(), va->dtype, va->ptr(), vb->dtype, vb->ptr(), va->n, static_cast<double *>(vo->ptr()));
   F3R_API_END
 }
 int f3r_vec_nrm2(f3r_vec a, double *out) {

@@ -73,6 +73,17 @@ int f3r_vec_scale(f3r_vec x, double alpha);
  * the two element types, fp16 data in fp32; a single-pass grid reduction. */
 int f3r_vec_dot(f3r_vec a, f3r_vec b, double *out);
 int f3r_vec_nrm2(f3r_vec a, double *out);
+/* f3r_vec_dot that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  out[0] (an fp64 vector of >= 1 element on the same
+ * context) receives <a, b> on the device, stream-ordered, without a host
+ * synchronisation; the bits equal f3r_vec_dot's.  The reduction's counters and
+ * partial sums live in a static device array of 73,984 bytes per GPU (the
+ * counters of one reduction and the partials of up to 1024 blocks; any n fits,
+ * the grid is capped), held until the process ends.  Calls that overlap on
+ * one GPU (two contexts or threads at once) race on that array; calls on one
+ * context are ordered by its stream. */
+int f3r_vec_dot_noalloc(f3r_vec a, f3r_vec b, f3r_vec out);
 
 /* ------------------------------------------------------- host set-up ---- */
 /* CPU routines; no GPU is needed.  CSR arrays are int32 row pointers and
