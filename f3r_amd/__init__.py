@@ -15,6 +15,7 @@ from . import _capi
 from ._capi import F16, F32, F64, I32, F3RError, check, lib
 
 __all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
+           "Operator", "AutoRichardson", "FGMRES", "Restarted",
            "F3RError", "F64", "F32", "F16", "device_count"]
 
 _NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32}
@@ -67,8 +68,9 @@ class Context:
 class vector:
     """kmm::vector<T, device> (core/tensor.hpp:204-316)."""
 
-    def __init__(self, ctx, dtype, n, _handle=None, _base=None):
+    def __init__(self, ctx, dtype, n, _handle=None, _base=None, _borrowed=False):
         self.ctx = ctx
+        self._borrowed = _borrowed  # a handle the library owns (an Operator's operands)
         self.dtype = _dtype(dtype)
         self.n = int(n)
         self._base = _base
@@ -124,9 +126,9 @@ class vector:
 
     def __del__(self):
         try:
-            if self.handle:
+            if self.handle and not self._borrowed:
                 lib.f3r_vec_destroy(self.handle)
-                self.handle = None
+            self.handle = None
         except Exception:
             pass
 
@@ -546,6 +548,130 @@ class F3R:
         check(lib.f3r_solver_time_kernel(self.handle, w, reps, ctypes.byref(ms), ctypes.byref(by)),
               "time_kernel")
         return ms.value, by.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_solver_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------- composed nests ---
+# The reference composes its solvers from objects (sparse/models.hpp:46-267,
+# solver/gmres.hpp:56-135, solver/restart.hpp:8-37):
+#   Restarted(A, FGMRES(A, FGMRES(A32, FGMRES(A16, AutoRichardson(A16, M, m4, c), m3), m2), 100, True), 20)
+# These descriptors build that chain on the device (f3r_nest_create).
+
+class Operator:
+    """Operator<L>(shape, lambda) (sparse/models.hpp:46-126): fn(in, out) on
+    device vector views of `dtype`; fp16 views carry the solver's power-of-two
+    range factor (the operator must then be linear)."""
+
+    def __init__(self, ctx, n, fn, dtype="f32"):
+        self.ctx, self.n, self.dtype, self.fn = ctx, int(n), _dtype(dtype), fn
+        self.error = None
+
+        def tramp(user, vin, vout):
+            try:
+                fn(vector(ctx, self.dtype, self.n, _handle=ctypes.c_void_p(vin), _borrowed=True),
+                   vector(ctx, self.dtype, self.n, _handle=ctypes.c_void_p(vout), _borrowed=True))
+                return 0
+            except Exception as e:  # re-raised by Restarted.solve
+                self.error = e
+                return -1
+        self._cb = _capi.APPLY_FN(tramp)
+        h = ctypes.c_void_p()
+        check(lib.f3r_op_create(ctx.handle, self.n, self.dtype, self._cb, None, ctypes.byref(h)), "op_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_op_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class AutoRichardson:
+    """AutoRichardson<T, L>(A, M, {max_iter, period}) (sparse/solver/richardson.hpp:96-156):
+    the adaptive fp16 Richardson of work/src/f3r-gpu.cpp:229-260 on the fp16
+    matrix with the fp16 preconditioner M; static_weight > 0 fixes the weights
+    (work/src/static.cpp:99-119)"""
+
+    def __init__(self, A16, M, max_iter, period, static_weight=0.0):
+        self.A16, self.M, self.m4, self.c, self.w = A16, M, int(max_iter), int(period), float(static_weight)
+
+
+def _mat_dtype(A):
+    return A.dtype if isinstance(A, SELL) else _dtype(A)
+
+
+class FGMRES:
+    """FGMRES<T, L>(A, M, {max_iter, normalize}) (sparse/solver/gmres.hpp:56-135).
+    T = "f32" (fp32 basis) or "f16" (fp16 storage, fp32 arithmetic) for an
+    inner level, "f64" for the outer level; A is a SELL (its precision is the
+    level's matrix precision) or a dtype; M another FGMRES, an AutoRichardson,
+    a preconditioner (SDAINV, ILU0) or an Operator."""
+
+    def __init__(self, A, M, max_iter, normalize=False, T="f32"):
+        self.A, self.M, self.m, self.normalize, self.T = A, M, int(max_iter), bool(normalize), _dtype(T)
+
+    def chain(self):
+        """(levels below this one's basis, innermost) with this level first"""
+        levels = [] if self.T == F64 else [(self.m, self.T, _mat_dtype(self.A))]
+        node = self.M
+        while isinstance(node, FGMRES):
+            levels.append((node.m, node.T, _mat_dtype(node.A)))
+            node = node.M
+        return levels, node
+
+
+class Restarted:
+    """Restarted<double, L>(A, S, {max_iter}) (sparse/solver/restart.hpp:8-37) over
+    the fp64 outer FGMRES(A, ..., m1, normalize=True); A is the CSR matrix as
+    solved.  Layout keywords (C, sigma, K, index, range_scale) as for F3R."""
+
+    def __init__(self, ctx, A, S, max_iter=20, eps=1e-8, C=0, sigma=0, K=0, index=1, range_scale=0.0):
+        if not (isinstance(S, FGMRES) and S.T == F64 and S.normalize):
+            raise ValueError("Restarted: the outer level is FGMRES(A, ..., T='f64', normalize=True)")
+        levels, inner = S.chain()
+        self.ctx, self.A, self.keep = ctx, A, (S, inner)
+        p = _capi.Params()
+        check(lib.f3r_params_default(ctypes.byref(p)), "params_default")
+        p.m1, p.restart, p.eps = S.m, int(max_iter), eps
+        p.C, p.sigma, p.K, p.index, p.range_scale = C, sigma, K, index, range_scale
+        lv = (_capi.Level * len(levels))(*[_capi.Level(m, b, mt) for (m, b, mt) in levels])
+        inn = _capi.Inner()
+        if isinstance(inner, AutoRichardson):
+            inn.kind, inn.P, inn.m4, inn.c, inn.static_weight = 1, inner.M.handle, inner.m4, inner.c, inner.w
+        elif isinstance(inner, Operator):
+            inn.kind, inn.op = 2, inner.handle
+        else:
+            inn.kind, inn.P = 0, inner.handle
+        self._op = inner if isinstance(inner, Operator) else None
+        h = ctypes.c_void_p()
+        check(lib.f3r_nest_create(ctx.handle, A.handle(ctx), lv, len(levels), ctypes.byref(inn), ctypes.byref(p),
+                                  ctypes.byref(h)), "nest_create")
+        self.handle = h
+
+    def solve(self, b, x, cond=None):
+        """Solver::solve(b, x, cond) (sparse/models.hpp:191-267); cond(i, |e|) -> bool,
+        None: |e| / ||b|| < eps"""
+        r = _capi.Result()
+        if cond is None:
+            st = lib.f3r_solve(self.handle, b.handle, x.handle, ctypes.byref(r))
+        else:
+            cb = _capi.COND_FN(lambda user, i, e: 1 if cond(int(i), float(e)) else 0)
+            st = lib.f3r_solve_cond(self.handle, b.handle, x.handle, cb, None, ctypes.byref(r))
+        if st != 0 and self._op is not None and self._op.error is not None:
+            err, self._op.error = self._op.error, None
+            raise err
+        check(st, "solve")
+        return solve_res_t(bool(r.is_solved), r.res_iter, r.res_nrm2, r.nrm_b, r.solve_s, r.precond_calls,
+                           r.outer_cycles)
 
     def __del__(self):
         try:

@@ -30,6 +30,20 @@ class Params(ctypes.Structure):
                 ("grid", c_i32), ("index", c_i32), ("static_weight", c_dbl), ("depth", c_i32)]
 
 
+class Level(ctypes.Structure):
+    _fields_ = [("m", c_i32), ("basis", c_i32), ("matrix", c_i32)]
+
+
+class Inner(ctypes.Structure):
+    _fields_ = [("kind", c_i32), ("P", c_vp), ("op", c_vp), ("m4", c_i32), ("c", c_i32),
+                ("static_weight", c_dbl)]
+
+
+# int (*)(void *user, f3r_vec in, f3r_vec out) and int (*)(void *user, int32 i, double e)
+APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, c_vp, c_vp, c_vp)
+COND_FN = ctypes.CFUNCTYPE(ctypes.c_int, c_vp, c_i32, c_dbl)
+
+
 class Result(ctypes.Structure):
     _fields_ = [("is_solved", c_i32), ("res_iter", c_i32), ("res_nrm2", c_dbl), ("nrm_b", c_dbl),
                 ("solve_s", c_dbl), ("precond_calls", c_i64), ("outer_cycles", c_i32)]
@@ -91,6 +105,11 @@ SIGNATURES = [
                                           ctypes.POINTER(c_vp)]),
     ("f3r_prec_setup_where", ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int)]),
     ("f3r_factor_levels", ctypes.c_int, [c_vp, ctypes.c_int, c_vp, P_i32]),
+    ("f3r_op_create", ctypes.c_int, [c_vp, c_i32, ctypes.c_int, APPLY_FN, c_vp, ctypes.POINTER(c_vp)]),
+    ("f3r_op_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_nest_create", ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(Level), c_i32, ctypes.POINTER(Inner),
+                                       ctypes.POINTER(Params), ctypes.POINTER(c_vp)]),
+    ("f3r_solve_cond", ctypes.c_int, [c_vp, c_vp, c_vp, COND_FN, c_vp, ctypes.POINTER(Result)]),
     ("f3r_batch_create", ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, ctypes.POINTER(Params), ctypes.POINTER(c_vp)]),
     ("f3r_batch_solve", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(Result)]),
     ("f3r_batch_info", ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), P_i32]),

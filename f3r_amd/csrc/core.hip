@@ -965,6 +965,8 @@ Vec *vec_of(f3r_vec v) { return v ? &v->v : nullptr; }
 std::shared_ptr<Mat> mat_of(f3r_mat m) { return m ? m->m : nullptr; }
 std::shared_ptr<Prec> prec_of(f3r_prec p) { return p ? p->p : nullptr; }
 f3r_prec wrap_prec(std::shared_ptr<Prec> p) { return new f3r_prec_s{std::move(p)}; }
+f3r_vec wrap_vec(const Vec &v) { return new f3r_vec_s{v}; }
+void release_vec(f3r_vec v) { delete v; }
 }  // namespace f3r
 
 extern "C" {

@@ -81,6 +81,8 @@ Vec *vec_of(f3r_vec v);
 std::shared_ptr<Mat> mat_of(f3r_mat m);
 std::shared_ptr<Prec> prec_of(f3r_prec p);
 f3r_prec wrap_prec(std::shared_ptr<Prec> p);  // new handle owning a reference
+f3r_vec wrap_vec(const Vec &v);                // new handle of a vector (view)
+void release_vec(f3r_vec v);
 // device build of a sigma = 1 SELL layout from a device CSR (build.hip);
 // false (nothing built) when a chunk is too long for the on-chip sort
 bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col,

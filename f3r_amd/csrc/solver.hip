@@ -183,19 +183,100 @@ template <class T> struct EpiResidNorm {
   }
 };
 
+// The finish of the SpMV + dots kernel's block partials inside the CGS
+// update: block k < ncols adds dot k's G partials (dpart[k * G + b]) in the
+// order of k_dots_finish with 1024 threads (thread t: partials t, t + 1024,
+// ... in turn; then the 16 waves' sums in wave order), so the bits are the
+// same, and publishes it; every block waits for the ncols sums before its
+// update.  The producers are the launch's lowest-numbered blocks, which are
+// dispatched first and wait on nothing, so every wait ends; it is bounded
+// all the same (kLookbackSpins polls, then the status becomes NaN).
+// Publication as in grid_sum: an agent-scope atomic store of the value,
+// completed before the flag's atomic store; the waiting block reads both with
+// agent-scope atomic loads.  pub[k * kCounterStride] is dot k's flag, the
+// value sits 8 bytes after it; the last block clears the flags.
+constexpr int kLookbackSpins = 1 << 22;
+template <class T>
+__device__ __forceinline__ T *pub_value(unsigned *pub, int k) {
+  return reinterpret_cast<T *>(pub + k * kCounterStride + 2);
+}
+template <class T, int NV>
+__device__ void dots_lookback(const T *__restrict__ dpart, int G, int ncols, unsigned *pub, T (&h)[NV],
+                              bool &late) {
+  __shared__ T fin[16];
+  __shared__ T hsh[NV];
+  __shared__ int bad;
+  const int lane = threadIdx.x & (kWave - 1), rw = threadIdx.x / kWave;
+  if ((int)blockIdx.x < ncols) {
+    const T *__restrict__ p = dpart + (int64_t)blockIdx.x * G;
+    T vm[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int t = threadIdx.x + kBlock * m;
+      T t16[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) t16[q] = t + q * 1024 < G ? p[t + q * 1024] : (T)0;
+      T a = 0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) a += t16[q];
+      vm[m] = a;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const T sm = wave_sum(vm[m]);
+      if (lane == 0) fin[rw + 4 * m] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T u = fin[0];
+      for (int q = 1; q < 16; ++q) u += fin[q];
+      publish(pub_value<T>(pub, blockIdx.x), u);
+      stores_complete();
+      __hip_atomic_store(&pub[blockIdx.x * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int k = 0; k < ncols; ++k) {
+      int it = 0;
+      while (collect(&pub[k * kCounterStride]) == 0u) {
+        if (++it > kLookbackSpins) {
+          b = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      hsh[k] = collect(pub_value<T>(pub, k));
+    }
+    bad = b;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) h[k] = k < ncols ? hsh[k] : (T)0;
+  late = bad != 0;
+}
+
 // V_{j+1} -= V H(j)[0..j] (MULMV order: V_0 h_0, then fused adds), ||V_{j+1}||,
 // then the last block: H(j, j+1), 1/h, Givens, |e_{j+1}| for the host.
+// dpart != null: H(j)[0..j] are first summed from the SpMV + dots kernel's
+// block partials (dots_lookback), else read from Hcol.
 // VEC: every pointer is 4-element aligned; each lane then takes 4
 // consecutive rows per step with 16-byte (fp32) loads and stores
 template <class T, int NV, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict__ w, Cols<T, NV> cols,
                                                        int ncols, T *Hcol, T *cs, T *sn, T *e, int j,
                                                        int e0_one, T *inv, double *status, T *partial,
-                                                       unsigned *counter) {
+                                                       unsigned *counter, const T *__restrict__ dpart, int G,
+                                                       unsigned *pub) {
   __shared__ T lds[kWavesPerBlock];
   T h[NV];
+  bool late = false;
+  if (dpart) {
+    dots_lookback<T, NV>(dpart, G, ncols, pub, h, late);
+  } else {
 #pragma unroll
-  for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
+    for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
+  }
   T v[1] = {0};
   const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
   int64_t i0 = 0;
@@ -230,10 +311,18 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
     w[i] = wn;
     v[0] = fma_<T>(wn, wn, v[0]);
   }
+  // a late look-back (never expected) poisons the step
+  if (late && threadIdx.x == 0) v[0] = (T)NAN;
   // the grid's sum of ||w||^2 in the last block (one wave of blocks: the
   // arrivals cost less than a one-block finishing launch, 0.12 ms per solve
   // at 128^3)
   if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
+    if (dpart) {  // every block is past its wait: the dots are final, the flags reusable
+      for (int k = 0; k < ncols; ++k) {
+        Hcol[k] = h[k];
+        __hip_atomic_store(&pub[k * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     T nv = (T)sqrt(v[0]);
     Hcol[j + 1] = nv;
     *inv = (T)1 / nv;
@@ -722,6 +811,17 @@ struct Solver {
     template <class S> S *Zc(int k) const { return Z->as<S>() + (int64_t)k * ld; }
   };
   std::vector<GLevel> glev;
+  // innermost operator under the generic levels: 0 M applied directly (the
+  // depth ablations), 1 the Richardson of the F3R nest (m4, c, static_weight
+  // of p), 2 the caller's operator (f3r_op)
+  int inner = 0;
+  f3r_apply_fn op_fn = nullptr;
+  void *op_user = nullptr;
+  int op_dtype = F3R_F32;
+  DevPtr op_in, op_out;
+  // stopping rule of the outer level: null = |e| / ||b|| < eps
+  f3r_cond_fn cond_fn = nullptr;
+  void *cond_user = nullptr;
   // Richardson work vectors (fp16 in HALF, else the basis type)
   DevPtr hin, hout, hout2, hr, tmp2, tt, rich;
   int64_t cnt = 0, calls = 0;
@@ -757,6 +857,10 @@ struct Solver {
   }
   template <class V> SellArgs argsv() const { return sell_args(shv<V>(), false); }
   unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + (int64_t)i * kCounterSlot; }
+  // the look-back flags of the fused dots finish, and the partial sums the
+  // SpMV + dots kernel may hold (16 dots of at most kMaxReduceGrid blocks)
+  static constexpr int kLookbackSlot = 16;  // 8..14: the outer dots (k_mdot_strided), 20..: learning
+  static constexpr int64_t kDotsPartials = (int64_t)16 * kMaxReduceGrid;
   template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
   template <class T> InnerLevel<T> &L2() {
     if constexpr (std::is_same<T, float>::value) return L2f; else return L2d;
@@ -779,21 +883,32 @@ struct Solver {
     } else {
       launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
     }
-    const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell used
-    const int G = spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
-    hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(), G,
-                       L.Hp() + (int64_t)j * L.ldh());
+    if (!fuse_finish())
+      hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(),
+                         dots_partials(), L.Hp() + (int64_t)j * L.ldh());
   }
+  // block partials the SpMV + dots kernel wrote per dot (its grid)
+  int dots_partials() const {
+    const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell uses
+    return spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
+  }
+  // flags bit 3 keeps the dots' finish in a launch of its own
+  bool fuse_finish() const { return (p.flags & 8) == 0; }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     bool vec = aligned4(L.V[j + 1]);
     for (int k = 0; k <= j; ++k) vec = vec && aligned4(L.V[k]);
     auto kern = vec ? k_cgs_update<T, NV, true> : k_cgs_update<T, NV, false>;
-    hipLaunchKernelGGL(kern, dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
+    // with the fused finish, blocks 0..j sum the dots: at least j + 1 blocks;
+    // the update's own reduction uses the partials after the dots'
+    const bool fuse = fuse_finish();
+    const int grid = fuse ? std::max(dots_grid, j + 1) : dots_grid;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st(), (int64_t)n,
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
-                       (double *)nullptr, partial<T>(), counter(1));
+                       (double *)nullptr, fuse ? partial<T>() + kDotsPartials : partial<T>(), counter(1),
+                       fuse ? (const T *)partial<T>() : (const T *)nullptr, dots_partials(), counter(kLookbackSlot));
   }
   template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
@@ -1066,10 +1181,47 @@ struct Solver {
                          (const half_t *)ho, out, (double)s_in / range_s);
     }
   }
+  // the caller's operator on copies of the level's vectors in the operator's
+  // dtype: fp16 operands carry the range factor range_s (as M's operands do
+  // in mdirect), fp32 / fp64 operands none
+  template <class SI> void op_apply(const SI *in, SI *out, float s_in) {
+    ++cnt;
+    ++calls;
+    const int64_t nn = n;
+    const dim3 g(grid()), b(kBlock);
+    const double s_op = op_dtype == F3R_F16 ? (double)range_s : 1.0;
+    auto to_op = [&](auto tag) {
+      using T = decltype(tag);
+      hipLaunchKernelGGL((k_g_entry<SI, T>), g, b, 0, st(), nn, in, op_in->as<T>(), s_op / s_in);
+    };
+    auto from_op = [&](auto tag) {
+      using T = decltype(tag);
+      hipLaunchKernelGGL((k_g_entry<T, SI>), g, b, 0, st(), nn, (const T *)op_out->as<T>(), out, s_in / s_op);
+    };
+    if (op_dtype == F3R_F16) to_op(half_t()); else if (op_dtype == F3R_F32) to_op(float()); else to_op(double());
+    Vec vi{ctx, op_dtype, n, op_in, 0}, vo{ctx, op_dtype, n, op_out, 0};
+    f3r_vec hi = wrap_vec(vi), ho = wrap_vec(vo);
+    const int rc = op_fn(op_user, hi, ho);
+    release_vec(hi);
+    release_vec(ho);
+    F3R_CHECK(rc == F3R_OK, rc, "the caller's operator returned an error");
+    if (op_dtype == F3R_F16) from_op(half_t()); else if (op_dtype == F3R_F32) from_op(float()); else from_op(double());
+  }
+  template <class SI> void innermost(const SI *in, SI *out, float s_in) {
+    if (inner == 2) {
+      op_apply<SI>(in, out, s_in);
+    } else if (inner == 1) {
+      // the fp16 Richardson of the F3R nest on an fp32 level's vectors
+      if constexpr (std::is_same<SI, float>::value) level4<float>(in, out, nullptr);
+      else throw Error{F3R_ERR_UNSUPPORTED, "the Richardson level needs an fp32-basis level above it"};
+    } else {
+      mdirect<SI>(in, out, s_in);
+    }
+  }
   // generic level l applied to `in` (storage SI, scaled by s_in), out likewise
   template <class SI> void glevel(int l, const SI *in, SI *out, float s_in) {
     if (l == (int)glev.size()) {
-      mdirect<SI>(in, out, s_in);
+      innermost<SI>(in, out, s_in);
       return;
     }
     if (glev[l].store == F3R_F16)
@@ -1142,7 +1294,10 @@ struct Solver {
     hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V, (const double *)inv1->as<double>());
     double e = read_scalar(ctx.get(), stat);
     if (record) hist.push_back(e / nrm_b);
-    if (e / nrm_b < p.eps) return {true, 0, e};
+    // the caller's cond(i, |e|) of Solver::solve (sparse/models.hpp:191-267),
+    // called as FGMRES::kernel calls it (i = 0 first, then the step count)
+    auto stop = [&](int i) { return cond_fn ? cond_fn(cond_user, i, e) != 0 : e / nrm_b < p.eps; };
+    if (stop(0)) return {true, 0, e};
     int j = 0;
     bool flag = false;
     for (; j < m1; ++j) {
@@ -1162,7 +1317,7 @@ struct Solver {
                          (const double *)inv1->as<double>());
       e = read_scalar(ctx.get(), stat);
       if (record) hist.push_back(e / nrm_b);
-      if (e / nrm_b < p.eps) {
+      if (stop(j + 1)) {
         ++j;
         flag = true;
         break;
@@ -1182,8 +1337,19 @@ struct Solver {
 // ----------------------------------------------------------- set-up ------
 static int ilog2_floor(double v) { return v < 1 ? 0 : (int)std::floor(std::log2(v)); }
 
+// a composed nest (f3r_nest_create): generic FGMRES levels from the
+// outermost inner one down, and the innermost operator
+struct NestSpec {
+  std::vector<f3r_level> levels;
+  int inner = 0;  // Solver::inner
+  f3r_apply_fn fn = nullptr;
+  void *user = nullptr;
+  int op_dtype = F3R_F32;
+};
+
 Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec *Pin, const f3r_params &p,
-                      const Mat *dev = nullptr) {
+                      const Mat *dev = nullptr, const NestSpec *nest = nullptr) {
+  F3R_CHECK(Pin || (nest && nest->inner == 2), F3R_ERR_ARG, "solver: a preconditioner is needed");
   F3R_CHECK(p.variant >= 0 && p.variant <= 2, F3R_ERR_ARG, "solver: variant must be 0 (HALF), 1 (FLOAT) or 2 (DOUBLE)");
   F3R_CHECK(p.m1 >= 1 && p.m1 <= 120, F3R_ERR_ARG, "solver: m1 must be in [1, 120]");
   F3R_CHECK(p.m2 >= 1 && p.m2 <= 16 && p.m3 >= 1 && p.m3 <= 16, F3R_ERR_ARG, "solver: m2, m3 in [1, 16]");
@@ -1266,18 +1432,22 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   }
   bool ident = true;
   for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;
+  F3R_CHECK(!nest || nest->inner != 2 || ident, F3R_ERR_UNSUPPORTED,
+            "nest: a caller's operator needs the matrix's own row order (sigma 1)");
   // innermost preconditioner in the internal order and the innermost precision
-  const int pdt = p.variant == 0 ? F3R_F16 : p.variant == 1 ? F3R_F32 : F3R_F64;
-  auto P = std::make_shared<Prec>();
-  P->ctx = ctx;
-  P->kind = Pin->kind;
-  P->dtype = pdt;
-  P->tol = Pin->tol;
-  P->acc = Pin->acc;
-  P->host = Pin->host;
-  F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
-  prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, pdt == F3R_F16 ? K16 : K, p.index != 0);
-  S->P = P;
+  if (Pin) {
+    const int pdt = p.variant == 0 ? F3R_F16 : p.variant == 1 ? F3R_F32 : F3R_F64;
+    auto P = std::make_shared<Prec>();
+    P->ctx = ctx;
+    P->kind = Pin->kind;
+    P->dtype = pdt;
+    P->tol = Pin->tol;
+    P->acc = Pin->acc;
+    P->host = Pin->host;
+    F3R_CHECK(P->host.f[0].n == A.n, F3R_ERR_ARG, "solver: preconditioner size differs from A");
+    prec_build_device(P.get(), ident ? nullptr : &hs.perm, C, pdt == F3R_F16 ? K16 : K, p.index != 0);
+    S->P = P;
+  }
   S->range_s = p.variant != 0 ? 1.f
                : p.range_scale > 0 ? (float)p.range_scale
                                    : std::ldexp(1.f, ilog2_floor(std::sqrt((double)A.n) / 4.0));
@@ -1322,13 +1492,24 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
     for (int k = 0; k <= m; ++k) L.V[k] = L.Vbuf->template as<E>() + (int64_t)k * ld;
     for (int k = 0; k < m; ++k) L.Z[k] = L.Zbuf->template as<E>() + (int64_t)k * ld;
   };
-  if (p.depth != 0) {
-    // the nests of work/src/depth.cpp:122-167: {m, basis storage, matrix values}
+  if (p.depth != 0 || nest) {
+    // the nests of work/src/depth.cpp:122-167, or the caller's levels:
+    // {m, basis storage, matrix values}
     struct Spec {
       int m, store, mat;
     };
     std::vector<Spec> spec;
-    switch (p.depth) {
+    if (nest) {
+      for (const f3r_level &q : nest->levels) spec.push_back({q.m, q.basis, q.matrix});
+      S->inner = nest->inner;
+      S->op_fn = nest->fn;
+      S->op_user = nest->user;
+      S->op_dtype = nest->op_dtype;
+      if (nest->inner == 2) {
+        S->op_in = dev_alloc(dtype_size(nest->op_dtype) * std::max<int64_t>(1, n));
+        S->op_out = dev_alloc(dtype_size(nest->op_dtype) * std::max<int64_t>(1, n));
+      }
+    } else switch (p.depth) {
       case 1: spec = {{64, F3R_F32, F3R_F32}}; break;                                           // F2
       case 2: spec = {{64, F3R_F16, F3R_F16}}; break;                                           // F2H
       case 3: spec = {{8, F3R_F32, F3R_F32}, {8, F3R_F32, F3R_F16}}; break;                     // F3
@@ -1442,6 +1623,13 @@ using namespace f3r;
 struct f3r_solver_s {
   Solver *s;
 };
+struct f3r_op_s {
+  std::shared_ptr<Context> ctx;
+  int32_t n;
+  int dtype;
+  f3r_apply_fn fn;
+  void *user;
+};
 
 
 extern "C" {
@@ -1484,6 +1672,93 @@ int f3r_solve(f3r_solver S, f3r_vec b, f3r_vec x, f3r_result *res) {
   F3R_CHECK(S && b && x && res, F3R_ERR_ARG, "solve: null argument");
   std::memset(res, 0, sizeof(*res));
   solver_solve(S->s, vec_of(b), vec_of(x), res);
+  F3R_API_END
+}
+
+int f3r_solve_cond(f3r_solver S, f3r_vec b, f3r_vec x, f3r_cond_fn cond, void *user, f3r_result *res) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && b && x && res && cond, F3R_ERR_ARG, "solve_cond: null argument");
+  std::memset(res, 0, sizeof(*res));
+  S->s->cond_fn = cond;
+  S->s->cond_user = user;
+  try {
+    solver_solve(S->s, vec_of(b), vec_of(x), res);
+  } catch (...) {
+    S->s->cond_fn = nullptr;
+    S->s->cond_user = nullptr;
+    throw;
+  }
+  S->s->cond_fn = nullptr;
+  S->s->cond_user = nullptr;
+  F3R_API_END
+}
+
+int f3r_op_create(f3r_ctx ctx, int32_t n, int dtype, f3r_apply_fn fn, void *user, f3r_op *op) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && fn && op && n > 0, F3R_ERR_ARG, "op_create: bad argument");
+  F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16, F3R_ERR_ARG, "op_create: fp64, fp32 or fp16 vectors");
+  *op = new f3r_op_s{ctx_of(ctx), n, dtype, fn, user};
+  F3R_API_END
+}
+
+int f3r_op_destroy(f3r_op op) {
+  delete op;
+  return F3R_OK;
+}
+
+int f3r_nest_create(f3r_ctx ctx, f3r_mat A, const f3r_level *levels, int32_t nlevels, const f3r_inner *inner,
+                    const f3r_params *params, f3r_solver *S) {
+  F3R_API_BEGIN
+  F3R_CHECK(ctx && A && levels && inner && params && S, F3R_ERR_ARG, "nest_create: null argument");
+  *S = nullptr;
+  auto m = mat_of(A);
+  F3R_CHECK(m->host, F3R_ERR_ARG, "nest_create: A must carry its CSR");
+  F3R_CHECK(nlevels >= 1 && nlevels <= 4, F3R_ERR_ARG, "nest_create: 1 to 4 FGMRES levels");
+  for (int32_t l = 0; l < nlevels; ++l) {
+    const f3r_level &q = levels[l];
+    F3R_CHECK(q.m >= 1 && q.m <= 64, F3R_ERR_ARG, "nest_create: a level's m in [1, 64]");
+    F3R_CHECK((q.basis == F3R_F32 || q.basis == F3R_F16) && (q.matrix == F3R_F32 || q.matrix == F3R_F16),
+              F3R_ERR_ARG, "nest_create: a level's basis and matrix are F3R_F32 or F3R_F16");
+  }
+  F3R_CHECK(inner->kind >= F3R_INNER_PREC && inner->kind <= F3R_INNER_OP, F3R_ERR_ARG, "nest_create: inner kind");
+  f3r_params p = *params;
+  p.variant = 0;
+  p.depth = 0;
+  p.static_weight = 0;
+  const Prec *P = nullptr;
+  if (inner->kind != F3R_INNER_OP) {
+    F3R_CHECK(inner->P, F3R_ERR_ARG, "nest_create: the innermost level needs a preconditioner");
+    P = prec_of(inner->P).get();
+  }
+  if (inner->kind == F3R_INNER_RICHARDSON) {
+    F3R_CHECK(inner->m4 >= 1 && inner->m4 <= 16 && inner->c >= 1 && inner->static_weight >= 0, F3R_ERR_ARG,
+              "nest_create: Richardson m4 in [1, 16], c >= 1, static_weight >= 0");
+    F3R_CHECK(levels[nlevels - 1].basis == F3R_F32, F3R_ERR_ARG,
+              "nest_create: the level above the Richardson needs an fp32 basis");
+    p.m4 = inner->m4;
+    p.c = inner->c;
+    p.static_weight = inner->static_weight;
+    // the F3R nest itself: f3r_solver_create's fused solver
+    if (nlevels == 2 && levels[0].basis == F3R_F32 && levels[0].matrix == F3R_F32 && levels[1].basis == F3R_F32 &&
+        levels[1].matrix == F3R_F16 && levels[0].m <= 16 && levels[1].m <= 16) {
+      p.m2 = levels[0].m;
+      p.m3 = levels[1].m;
+      *S = new f3r_solver_s{solver_create(ctx_of(ctx), *m->host, P, p, m.get())};
+      return F3R_OK;
+    }
+  }
+  NestSpec nest;
+  nest.levels.assign(levels, levels + nlevels);
+  nest.inner = inner->kind;
+  if (inner->kind == F3R_INNER_OP) {
+    F3R_CHECK(inner->op && inner->op->n == m->host->n, F3R_ERR_ARG, "nest_create: an operator of the matrix's size");
+    F3R_CHECK(inner->op->ctx.get() == ctx_of(ctx).get(), F3R_ERR_ARG, "nest_create: the operator's context");
+    nest.fn = inner->op->fn;
+    nest.user = inner->op->user;
+    nest.op_dtype = inner->op->dtype;
+  }
+  p.m2 = p.m3 = 1;  // not used by the generic levels
+  *S = new f3r_solver_s{solver_create(ctx_of(ctx), *m->host, P, p, m.get(), &nest)};
   F3R_API_END
 }
 

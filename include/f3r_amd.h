@@ -324,6 +324,61 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
                            double *bytes);
 int f3r_solver_destroy(f3r_solver S);
 
+/* --------------------------------------------------- composed nests ---- */
+/* The reference composes its solvers from objects: FGMRES<T>(A, M, {m,
+ * normalize}) levels whose M is another level, a preconditioner, or an
+ * Operator(shape, lambda) (sparse/models.hpp:46-126, solver/gmres.hpp:56-135),
+ * under Restarted<double>(A, FGMRES<double>(A, ..., {m1, true}), {restart})
+ * (solver/restart.hpp:8-37).  f3r_nest_create builds such a chain.
+ *
+ * A caller's operator: out = Op(in) on non-owning views of n elements of
+ * `dtype` on the context's stream (the callback may call any f3r_* function
+ * of that context on them, e.g. f3r_spmv, f3r_prec_apply, f3r_vec_axpy).  An
+ * F3R_F16 operator receives vectors multiplied by the solver's power-of-two
+ * range factor (f3r_params.range_scale), as the fp16 preconditioner does, and
+ * must be linear; F3R_F32 / F3R_F64 operators see the level's values.
+ * Returns F3R_OK or an error status, which the solve returns.
+ * replaces: Operator<L>(shape, lambda) (sparse/models.hpp:46-126) */
+typedef int (*f3r_apply_fn)(void *user, f3r_vec in, f3r_vec out);
+typedef struct f3r_op_s *f3r_op;
+int f3r_op_create(f3r_ctx ctx, int32_t n, int dtype, f3r_apply_fn fn, void *user, f3r_op *op);
+int f3r_op_destroy(f3r_op op);
+
+/* one FGMRES<T>(A_level, M, {m, false}) level: basis storage F3R_F32
+ * (FGMRES<float>) or F3R_F16 (FGMRES<_Float16>: fp16 storage, fp32
+ * arithmetic), the level's matrix values F3R_F32 or F3R_F16 */
+typedef struct {
+  int32_t m;
+  int32_t basis;
+  int32_t matrix;
+} f3r_level;
+/* the innermost operator of a nest */
+enum { F3R_INNER_PREC = 0,        /* M applied directly (the nests of work/src/depth.cpp) */
+       F3R_INNER_RICHARDSON = 1,  /* the adaptive (or static-weight) fp16 Richardson of
+                                     work/src/f3r-gpu.cpp:229-260 with M, m4 steps, probe
+                                     period c; the level above it must have an fp32 basis */
+       F3R_INNER_OP = 2 };        /* the caller's operator */
+typedef struct {
+  int32_t kind;
+  f3r_prec P;            /* PREC, RICHARDSON */
+  f3r_op op;             /* OP */
+  int32_t m4, c;         /* RICHARDSON */
+  double static_weight;  /* RICHARDSON: > 0 fixed weight, no probes */
+} f3r_inner;
+/* Restarted<double>(A, FGMRES<double>(A, L_0(L_1(... L_{k-1}(inner))), {m1,
+ * true}), {restart}) with k = nlevels in [1, 4] levels of m in [1, 64]; from
+ * params: m1, restart, eps, range_scale and the layout (C, sigma, K, index,
+ * flags).  The composition of f3r_solver_create (levels {m2, F32, F32},
+ * {m3, F32, F16} over the Richardson) runs f3r_solver_create's solver, bit for
+ * bit; any other composition runs the generic levels (no HIP graphs). */
+int f3r_nest_create(f3r_ctx ctx, f3r_mat A, const f3r_level *levels, int32_t nlevels, const f3r_inner *inner,
+                    const f3r_params *params, f3r_solver *S);
+/* Solver::solve(b, x, cond) (sparse/models.hpp:191-267): stop when
+ * cond(user, i, |e_i|) returns nonzero, called at the start of every outer
+ * cycle (i = 0) and after every outer step i, instead of |e| / ||b|| < eps */
+typedef int (*f3r_cond_fn)(void *user, int32_t iter, double res_nrm);
+int f3r_solve_cond(f3r_solver S, f3r_vec b, f3r_vec x, f3r_cond_fn cond, void *user, f3r_result *res);
+
 /* ----------------------------------------------------- batched solves ---- */
 /* Many independent systems (each right-hand side its own segment) solved in
  * lockstep: every launch covers all segments.  Segment s is A[s] with the

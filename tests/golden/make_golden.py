@@ -2,12 +2,13 @@
 
 Runs the binaries that oracle/ref_harness/Makefile builds from the reference
 checkout (oracle/_ref/: ref_driver over the reference's headers, the
-reference's CPU driver work/src/f3r.cpp in three precisions, and its matrix
-generator matrix/hpcg.cpp) on small inputs, single-threaded so that every
+reference's CPU driver work/src/f3r.cpp in three precisions, its study
+drivers work/src/depth.cpp and work/src/static.cpp, and its matrix generator
+matrix/hpcg.cpp) on small inputs, single-threaded so that every
 reduction runs in sequential order, and stores inputs and outputs as .npz
 data.  Run in the container that has the reference checkout:
 
-    make -C oracle/ref_harness && python tests/golden/make_golden.py [units f3r steps mm cpu]
+    make -C oracle/ref_harness && python tests/golden/make_golden.py [units f3r steps mm cpu ablation]
 """
 import os
 import subprocess
@@ -200,8 +201,41 @@ def cpu_driver_fixture(T):
     print("wrote cpu driver fixture")
 
 
+def ablation_fixture(T):
+    """The reference's study drivers on the 32^3 HPCG matrix of the CPU-driver
+    fixture (same seeded right-hand side, BJILU0 on 112 blocks, restart 3):
+    the nesting-depth nests of work/src/depth.cpp (F4, F3, F3H, F2, F2H) and
+    the static-weight Richardson of work/src/static.cpp (weights 0.8, 1.0,
+    1.2; 1.2 does not converge within 3 x 100 outer steps)."""
+    mdir = os.path.join(T, "matrix")
+    wdir = os.path.join(T, "work")
+    os.makedirs(mdir, exist_ok=True)
+    os.makedirs(wdir, exist_ok=True)
+    run("hpcg", 5, 5, 5, cwd=mdir)
+    n = 32 ** 3
+    b = np.random.default_rng(7).uniform(-1, 1, n)
+    with open(os.path.join(mdir, "hpcg_5_5_5_b.mtx"), "w") as f:
+        f.write("%%%%MatrixMarket matrix array real general\n%d 1\n" % n)
+        f.write("\n".join("%.17e" % t for t in b) + "\n")
+    out = {"b_seed": np.array(7)}
+    for nest in ("f4", "f3", "f3h", "f2", "f2h"):
+        line = run(nest, "hpcg_5_5_5.mtx", 1.0, 1, 8, 4, 2, 64, 3, cwd=wdir).strip()
+        f = line.split(",")
+        # Problem,Method,Prec,M2,M3,M4,W,Precond,ACC,Time,Iter,ImplRes,ExplRes (Iter = outer steps x 64)
+        out[f"{nest}_csv"] = np.array([float(f[10]), float(f[11]), float(f[12])])
+        print("depth", nest, line, flush=True)
+    for w in (0.8, 1.0, 1.2):
+        line = run("static", "hpcg_5_5_5.mtx", 1.0, 1, 8, 4, 2, 0, 3, w, cwd=wdir).strip()
+        f = line.split(",")
+        # Iter = outer steps x m2 m3 m4
+        out[f"static{w:.1f}_csv"] = np.array([float(f[10]), float(f[11]), float(f[12])])
+        print("static", w, line, flush=True)
+    np.savez_compressed(os.path.join(HERE, "ablation_hpcg32.npz"), **out)
+    print("wrote ablation fixture")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["units", "f3r", "steps", "mm", "cpu"]
+    which = sys.argv[1:] or ["units", "f3r", "steps", "mm", "cpu", "ablation"]
     with tempfile.TemporaryDirectory() as T:
         if "units" in which:
             unit_fixtures(T)
@@ -213,3 +247,5 @@ if __name__ == "__main__":
             mm_fixtures(T)
         if "cpu" in which:
             cpu_driver_fixture(T)
+        if "ablation" in which:
+            ablation_fixture(T)
