@@ -183,100 +183,19 @@ template <class T> struct EpiResidNorm {
   }
 };
 
-// The finish of the SpMV + dots kernel's block partials inside the CGS
-// update: block k < ncols adds dot k's G partials (dpart[k * G + b]) in the
-// order of k_dots_finish with 1024 threads (thread t: partials t, t + 1024,
-// ... in turn; then the 16 waves' sums in wave order), so the bits are the
-// same, and publishes it; every block waits for the ncols sums before its
-// update.  The producers are the launch's lowest-numbered blocks, which are
-// dispatched first and wait on nothing, so every wait ends; it is bounded
-// all the same (kLookbackSpins polls, then the status becomes NaN).
-// Publication as in grid_sum: an agent-scope atomic store of the value,
-// completed before the flag's atomic store; the waiting block reads both with
-// agent-scope atomic loads.  pub[k * kCounterStride] is dot k's flag, the
-// value sits 8 bytes after it; the last block clears the flags.
-constexpr int kLookbackSpins = 1 << 22;
-template <class T>
-__device__ __forceinline__ T *pub_value(unsigned *pub, int k) {
-  return reinterpret_cast<T *>(pub + k * kCounterStride + 2);
-}
-template <class T, int NV>
-__device__ void dots_lookback(const T *__restrict__ dpart, int G, int ncols, unsigned *pub, T (&h)[NV],
-                              bool &late) {
-  __shared__ T fin[16];
-  __shared__ T hsh[NV];
-  __shared__ int bad;
-  const int lane = threadIdx.x & (kWave - 1), rw = threadIdx.x / kWave;
-  if ((int)blockIdx.x < ncols) {
-    const T *__restrict__ p = dpart + (int64_t)blockIdx.x * G;
-    T vm[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int t = threadIdx.x + kBlock * m;
-      T t16[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) t16[q] = t + q * 1024 < G ? p[t + q * 1024] : (T)0;
-      T a = 0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) a += t16[q];
-      vm[m] = a;
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const T sm = wave_sum(vm[m]);
-      if (lane == 0) fin[rw + 4 * m] = sm;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      T u = fin[0];
-      for (int q = 1; q < 16; ++q) u += fin[q];
-      publish(pub_value<T>(pub, blockIdx.x), u);
-      stores_complete();
-      __hip_atomic_store(&pub[blockIdx.x * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  if (threadIdx.x == 0) {
-    int b = 0;
-    for (int k = 0; k < ncols; ++k) {
-      int it = 0;
-      while (collect(&pub[k * kCounterStride]) == 0u) {
-        if (++it > kLookbackSpins) {
-          b = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-      hsh[k] = collect(pub_value<T>(pub, k));
-    }
-    bad = b;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) h[k] = k < ncols ? hsh[k] : (T)0;
-  late = bad != 0;
-}
-
 // V_{j+1} -= V H(j)[0..j] (MULMV order: V_0 h_0, then fused adds), ||V_{j+1}||,
 // then the last block: H(j, j+1), 1/h, Givens, |e_{j+1}| for the host.
-// dpart != null: H(j)[0..j] are first summed from the SpMV + dots kernel's
-// block partials (dots_lookback), else read from Hcol.
 // VEC: every pointer is 4-element aligned; each lane then takes 4
 // consecutive rows per step with 16-byte (fp32) loads and stores
 template <class T, int NV, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict__ w, Cols<T, NV> cols,
                                                        int ncols, T *Hcol, T *cs, T *sn, T *e, int j,
                                                        int e0_one, T *inv, double *status, T *partial,
-                                                       unsigned *counter, const T *__restrict__ dpart, int G,
-                                                       unsigned *pub) {
+                                                       unsigned *counter) {
   __shared__ T lds[kWavesPerBlock];
   T h[NV];
-  bool late = false;
-  if (dpart) {
-    dots_lookback<T, NV>(dpart, G, ncols, pub, h, late);
-  } else {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
-  }
+  for (int k = 0; k < NV; ++k) h[k] = k < ncols ? Hcol[k] : (T)0;
   T v[1] = {0};
   const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
   int64_t i0 = 0;
@@ -311,18 +230,10 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
     w[i] = wn;
     v[0] = fma_<T>(wn, wn, v[0]);
   }
-  // a late look-back (never expected) poisons the step
-  if (late && threadIdx.x == 0) v[0] = (T)NAN;
   // the grid's sum of ||w||^2 in the last block (one wave of blocks: the
   // arrivals cost less than a one-block finishing launch, 0.12 ms per solve
   // at 128^3)
   if (grid_sum<T, 1>(v, partial, counter, lds) && threadIdx.x == 0) {
-    if (dpart) {  // every block is past its wait: the dots are final, the flags reusable
-      for (int k = 0; k < ncols; ++k) {
-        Hcol[k] = h[k];
-        __hip_atomic_store(&pub[k * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
     T nv = (T)sqrt(v[0]);
     Hcol[j + 1] = nv;
     *inv = (T)1 / nv;
@@ -857,10 +768,6 @@ struct Solver {
   }
   template <class V> SellArgs argsv() const { return sell_args(shv<V>(), false); }
   unsigned *counter(int i) const { return ctx->red.counters->as<unsigned>() + (int64_t)i * kCounterSlot; }
-  // the look-back flags of the fused dots finish, and the partial sums the
-  // SpMV + dots kernel may hold (16 dots of at most kMaxReduceGrid blocks)
-  static constexpr int kLookbackSlot = 16;  // 8..14: the outer dots (k_mdot_strided), 20..: learning
-  static constexpr int64_t kDotsPartials = (int64_t)16 * kMaxReduceGrid;
   template <class T> T *partial() const { return ctx->red.partial->as<T>(); }
   template <class T> InnerLevel<T> &L2() {
     if constexpr (std::is_same<T, float>::value) return L2f; else return L2d;
@@ -883,32 +790,24 @@ struct Solver {
     } else {
       launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
     }
-    if (!fuse_finish())
-      hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(),
-                         dots_partials(), L.Hp() + (int64_t)j * L.ldh());
+    hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(),
+                       dots_partials(), L.Hp() + (int64_t)j * L.ldh());
   }
   // block partials the SpMV + dots kernel wrote per dot (its grid)
   int dots_partials() const {
     const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell uses
     return spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
   }
-  // flags bit 3 keeps the dots' finish in a launch of its own
-  bool fuse_finish() const { return (p.flags & 8) == 0; }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
     for (int k = 0; k <= j; ++k) c.p[k] = L.V[k];
     bool vec = aligned4(L.V[j + 1]);
     for (int k = 0; k <= j; ++k) vec = vec && aligned4(L.V[k]);
     auto kern = vec ? k_cgs_update<T, NV, true> : k_cgs_update<T, NV, false>;
-    // with the fused finish, blocks 0..j sum the dots: at least j + 1 blocks;
-    // the update's own reduction uses the partials after the dots'
-    const bool fuse = fuse_finish();
-    const int grid = fuse ? std::max(dots_grid, j + 1) : dots_grid;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st(), (int64_t)n,
+    hipLaunchKernelGGL(kern, dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
-                       (double *)nullptr, fuse ? partial<T>() + kDotsPartials : partial<T>(), counter(1),
-                       fuse ? (const T *)partial<T>() : (const T *)nullptr, dots_partials(), counter(kLookbackSlot));
+                       (double *)nullptr, partial<T>(), counter(1));
   }
   template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
