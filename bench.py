@@ -335,12 +335,12 @@ def main():
     if args.batch:
         from f3r_amd import batch
         del S
-        bo = batch.run_lockstep(64, 4, seed=0, log=log)
+        bo = batch.run_lockstep(64, 4, seed=0, log=log, tune=True)
         out["batch"] = {"workload": "64 systems (16^3..64^3, 7- and 27-point, diagonal margin U[0.5,2]) x 4 "
                                     "right-hand sides, lockstep F3R (one set of launches for all 256 solves)",
                         **{k: bo[k] for k in ("solves", "padded_rows", "rows_total", "wall_s", "walls_s",
                                               "all_solved", "outer_steps_max", "outer_steps_mean",
-                                              "worst_true_rel_res", "setup_s")}}
+                                              "worst_true_rel_res", "setup_s", "layout")}}
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, full solves) ...")
         out["cpu_baseline"] = cpu_baseline(F, A, b)

@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi
 from ._capi import F16, F32, F64, I32, F3RError, check, lib
 
-__all__ = ["Context", "vector", "CSR", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
+__all__ = ["Context", "vector", "CSR", "CSRDevice", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
            "Operator", "AutoRichardson", "FGMRES", "Restarted",
            "F3RError", "F64", "F32", "F16", "device_count"]
 
@@ -344,6 +344,39 @@ def sell_layout(A, C=32, sigma=1, K=1):
     check(lib.f3r_sell_layout(A.n, _p(A.rptr), _p(A.idx), _p(A.val), C, sigma, K, _p(cptr), _p(cw),
                               _p(sidx), _p(sval), _p(perm)), "sell_layout")
     return cptr, cw[:nch], sidx, sval, perm
+
+
+class CSRDevice:
+    """CSR<T, device>(CSR<double, host>) (sparse/matrix/csr.hpp:12-50): the CSR
+    on a context with values in `dtype`; SpMV with `tpr` lanes per row (a
+    power of two up to 64; 0 = by the mean row length; 1 = each row summed in
+    CSR order, bit-identical to the SELL kernels)."""
+
+    def __init__(self, ctx, A, dtype="f64", tpr=0):
+        self.ctx, self.A, self.dtype, self.tpr = ctx, A, _dtype(dtype), int(tpr)
+        h = ctypes.c_void_p()
+        check(lib.f3r_mat_cast(A.handle(ctx), self.dtype, ctypes.byref(h)), "mat_cast")
+        self.handle = h
+        check(lib.f3r_mat_set_threads_per_row(h, self.tpr), "mat_set_threads_per_row")
+
+    def operate(self, x, y):
+        check(lib.f3r_spmv(self.handle, x.handle, y.handle), "spmv")
+
+    def compute_residual(self, b, x, r):
+        check(lib.f3r_residual(self.handle, b.handle, x.handle, r.handle), "residual")
+
+    def time_spmv(self, x, y, reps=20):
+        ms = ctypes.c_double()
+        check(lib.f3r_spmv_time(self.handle, x.handle, y.handle, reps, ctypes.byref(ms)), "spmv_time")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.f3r_mat_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
 
 
 class SELL:

@@ -92,6 +92,7 @@ SIGNATURES = [
                                         c_vp]),
     ("f3r_mat_cast", ctypes.c_int, [c_vp, ctypes.c_int, ctypes.POINTER(c_vp)]),
     ("f3r_mat_destroy", ctypes.c_int, [c_vp]),
+    ("f3r_mat_set_threads_per_row", ctypes.c_int, [c_vp, c_i32]),
     ("f3r_mat_info", ctypes.c_int, [c_vp, P_i32, P_i64, P_i64, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)]),
     ("f3r_spmv", ctypes.c_int, [c_vp, c_vp, c_vp]),

@@ -19,7 +19,7 @@ import time
 import numpy as np
 
 from . import CSR, F3R, SDAINV, SELL, Context, algorithm, nrm, uniform, vector
-from .tune import autotune
+from .tune import autotune, solver_layout
 
 
 def family(seed=0, count=64):
@@ -51,12 +51,13 @@ class _Worker:
                 t0 = time.time()
                 A = make_system(npts, nx, delta)
                 if self.tune:
-                    best, _ = autotune(self.ctx, A, "f16", **self.cands)
-                    lay = (best["C"], best["sigma"], best["K"])
+                    best, _ = autotune(self.ctx, A, "f16", tprs=(), **self.cands)
+                    L = solver_layout(best)
+                    lay = (L["C"], L["sigma"], L["K"], L["index"])
                 else:
-                    lay = (64, 1, 4)
+                    lay = (64, 1, 4, 1)
                 P = SDAINV(A, 0.1, 1.0, "f16", ctx=self.ctx)
-                S = F3R(self.ctx, A, P, C=lay[0], sigma=lay[1], K=lay[2])
+                S = F3R(self.ctx, A, P, C=lay[0], sigma=lay[1], K=lay[2], index=lay[3])
                 bs = [vector.from_numpy(self.ctx, uniform(A.n, -1.0, 1.0, seed=(self.seed << 20) + 16 * i + r))
                       for r in range(self.nrhs)]
                 xs = [vector(self.ctx, "f64", A.n) for _ in range(self.nrhs)]
@@ -73,7 +74,7 @@ class _Worker:
                     r = S.solve(b, x)
                     outs.append((r.res_iter, r.is_solved, r.solve_s))
                 self.results.append({"system": i, "npts": npts, "nx": nx, "n": A.n, "delta": round(delta, 4),
-                                     "layout": {"C": lay[0], "sigma": lay[1], "K": lay[2]},
+                                     "layout": {"C": lay[0], "sigma": lay[1], "K": lay[2], "index": lay[3]},
                                      "setup_s": round(setup_s, 3),
                                      "outer": [o[0] for o in outs], "solved": all(o[1] for o in outs),
                                      "solve_s": [o[2] for o in outs]})
@@ -96,7 +97,7 @@ def run(count=64, nrhs=4, streams=4, seed=0, tune=True, device=0,
         cands=None, log=None):
     """Solves the batch; returns a summary dict (wall time of the solve phase
     over all streams, per-system results, the worst true residual)."""
-    cands = cands or {"Cs": (64, 128, 256), "sigmas": (1,), "Ks": (1, 2, 4), "reps": 10}
+    cands = cands or {"Cs": (64, 128, 256), "sigmas": (1,), "Ks": (1, 2, 4), "offsets": (False, True), "reps": 10}
     fam = family(seed, count)
     # largest systems first, dealt round-robin, so the streams finish together
     order = sorted(fam, key=lambda t: -(t[2] ** 3) * (27 if t[1] == 27 else 7))
@@ -137,13 +138,30 @@ def run(count=64, nrhs=4, streams=4, seed=0, tune=True, device=0,
     return out
 
 
-def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None):
+def tune_lockstep(ctx, fam, log=None):
+    """Batch layout keywords (C, K, index) autotuned on the family's largest
+    27-point system, the lockstep SpMVs' dominant cost (chunk heights 64 to 256,
+    sigma 1: the lockstep layout keeps each segment's rows in order)"""
+    i, npts, nx, delta = max((t for t in fam if t[1] == 27), key=lambda t: t[2])
+    A = make_system(npts, nx, delta)
+    best, table = autotune(ctx, A, "f16", Cs=(64, 128, 256), sigmas=(1,), Ks=(1, 2, 4, 8), offsets=(False, True),
+                           tprs=(), reps=10)
+    L = solver_layout(best)
+    if log:
+        log(f"lockstep layout tuned on system {i} ({npts}-point {nx}^3): C={L['C']} K={L['K']} index={L['index']} "
+            f"({best['ms'] * 1e3:.1f} us; slowest candidate {table[-1]['ms'] * 1e3:.1f} us)")
+    return {"C": L["C"], "K": L["K"], "index": L["index"]}
+
+
+def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False):
     """The same family solved in lockstep (f3r_batch): one block-diagonal
     layout of all systems x right-hand sides, every launch covering all of
-    them; per-system SDAINV factorised on the device."""
+    them; per-system SDAINV factorised on the device.  tune: the layout
+    (C, K, index format) autotuned first (tune_lockstep)."""
     from . import Batch
     ctx = Context(device)
     fam = family(seed, count)
+    lay = tune_lockstep(ctx, fam, log) if tune else {}
     t0 = time.time()
     As, Ps, bs, meta = [], [], [], []
     for (i, npts, nx, delta) in fam:
@@ -154,7 +172,7 @@ def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None):
             Ps.append(P)
             bs.append(uniform(A.n, -1.0, 1.0, seed=(seed << 20) + 16 * i + r))
             meta.append((i, npts, nx, delta, r))
-    B = Batch(ctx, As, Ps)
+    B = Batch(ctx, As, Ps, **lay)
     rows, segs = B.info()
     setup_s = time.time() - t0
     if log:
@@ -185,6 +203,7 @@ def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None):
            "setup_s": setup_s, "all_solved": all(x.is_solved for x in res),
            "outer_steps_max": int(max(x.res_iter for x in res)), "outer_steps_mean": float(np.mean([x.res_iter for x in res])),
            "outer_steps_per_segment": [int(x.res_iter) for x in res],
+           "layout": lay or {"C": 64, "K": 0, "index": 1},
            "worst_true_rel_res": worst}
     if log:
         log(f"lockstep: {segs} solves in {out['wall_s'] * 1e3:.1f} ms wall (best of 3), all solved "
@@ -200,7 +219,7 @@ def main(argv=None):
     ap.add_argument("--rhs", type=int, default=4)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="no layout autotuning")
     ap.add_argument("--mode", choices=["lockstep", "streams"], default="lockstep",
                     help="lockstep: all solves in shared launches (f3r_batch); streams: one solver per system")
     ap.add_argument("--out", default=None)
@@ -211,7 +230,7 @@ def main(argv=None):
         print(f"[batch {time.time() - t0:6.1f}s] {m}", flush=True)
 
     if a.mode == "lockstep":
-        out = run_lockstep(a.count, a.rhs, a.seed, log=log)
+        out = run_lockstep(a.count, a.rhs, a.seed, log=log, tune=not a.no_tune)
     else:
         out = run(a.count, a.rhs, a.streams, a.seed, not a.no_tune, log=log)
     if a.out:

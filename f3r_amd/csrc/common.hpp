@@ -154,7 +154,9 @@ struct Mat {
   int dtype = F3R_F64;
   // CSR
   std::shared_ptr<HostCSR> host;
-  DevPtr d_rptr, d_idx, d_val;
+  DevPtr d_rptr, d_idx, d_val;  // d_val: fp64 (set-up reads it)
+  DevPtr c_val;                 // the values in `dtype` for SpMV when dtype is not fp64
+  int32_t tpr = 0;              // CSR SpMV lanes per row (0 = by the mean row length)
   // SELL
   std::shared_ptr<SellShared> sh;
   DevPtr s_val;

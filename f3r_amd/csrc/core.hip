@@ -68,17 +68,30 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T1 *__restrict_
   if (grid_sum<A, 1>(v, partial, counter, lds) && threadIdx.x == 0) *out = (double)v[0];
 }
 
-// CSR SpMV, one row per thread, fp64 values (CSR<double>::operate,
-// csr.hpp:32-50).  mode 0: y = A x, 1: y = b - A x.
-template <class X, class Y>
-__global__ void k_csr(int32_t n, const int32_t *__restrict__ rptr, const int32_t *__restrict__ idx,
-                      const double *__restrict__ val, const X *__restrict__ x, Y *__restrict__ y,
-                      const Y *__restrict__ b) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double t = 0;
-  for (int k = rptr[i]; k < rptr[i + 1]; ++k) t = __builtin_fma(val[k], (double)to_acc(x[idx[k]]), t);
-  y[i] = b ? cvt<Y>((double)to_acc(b[i]) - t) : cvt<Y>(t);
+// CSR SpMV (CSR<T>::operate / compute_residual, csr.hpp:32-50): y = A x, or
+// y = b - A x with b.  TPR consecutive lanes take a row: lane q sums entries
+// q, q + TPR, ... in order, and the TPR sums are added by a butterfly over
+// the lanes, so every lane's loads of val and idx are coalesced with its
+// neighbours'.  TPR = 1 sums each row in CSR order (bit-identical to the SELL
+// kernels and the oracle); TPR > 1 reassociates the sum.  Sums in fp64 for
+// fp64 values, else fp32.
+template <class V, class X, class Y, class ACC, int TPR>
+__global__ __launch_bounds__(kBlock) void k_csr(int32_t n, const int32_t *__restrict__ rptr,
+                                                const int32_t *__restrict__ idx, const V *__restrict__ val,
+                                                const X *__restrict__ x, Y *__restrict__ y,
+                                                const Y *__restrict__ b) {
+  constexpr int kRowsPerBlock = kBlock / TPR;
+  const int q = threadIdx.x % TPR;
+  for (int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / TPR; row < n;
+       row += (int64_t)gridDim.x * kRowsPerBlock) {
+    ACC acc = 0;
+    const int32_t k1 = rptr[row + 1];
+    for (int32_t k = rptr[row] + q; k < k1; k += TPR)
+      acc = fma_<ACC>((ACC)to_acc(val[k]), (ACC)to_acc(x[idx[k]]), acc);
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (q == 0) y[row] = b ? cvt<Y>((ACC)to_acc(b[row]) - acc) : cvt<Y>(acc);
+  }
 }
 
 // ILU(0) triangular solves by levels: each launch solves one level of rows
@@ -563,6 +576,16 @@ static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hi
                            EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
 }
 
+// lanes per row of the CSR SpMV: the caller's choice, else the largest power
+// of two not above half the mean row length (1 to 64)
+static int csr_tpr(const Mat &A) {
+  if (A.tpr > 0) return A.tpr;
+  const double mean = A.host->n > 0 ? (double)A.host->nnz() / A.host->n : 1.0;
+  int t = 1;
+  while (t < 64 && 2.0 * t <= 0.5 * mean) t *= 2;
+  return t;
+}
+
 void mat_apply(const Mat *A, Vec *x, Vec *y, Vec *b) {
   F3R_CHECK(x->n == (A->format == 0 ? A->host->ncols : A->sh->n) && y->n == x->n, F3R_ERR_ARG,
             "spmv: vector length does not match the matrix");
@@ -570,19 +593,42 @@ void mat_apply(const Mat *A, Vec *x, Vec *y, Vec *b) {
   hipStream_t st = A->ctx->stream;
   const void *bp = b ? b->ptr() : nullptr;
   if (A->format == 0) {
-    int g = (A->host->n + kBlock - 1) / kBlock;
-    auto go = [&](auto xt, auto yt) {
+    const int32_t n = A->host->n;
+    const int tpr = csr_tpr(*A);
+    const void *val = A->dtype == F3R_F64 ? A->d_val->p : A->c_val->p;
+    auto go = [&](auto vt, auto xt, auto yt) {
+      using V = decltype(vt);
       using X = decltype(xt);
       using Y = decltype(yt);
-      hipLaunchKernelGGL((k_csr<X, Y>), dim3(g), dim3(kBlock), 0, st, A->host->n, A->d_rptr->as<int32_t>(),
-                         A->d_idx->as<int32_t>(), A->d_val->as<double>(), (const X *)x->ptr(), (Y *)y->ptr(),
-                         (const Y *)bp);
+      using ACC = typename std::conditional<std::is_same<V, double>::value, double, float>::type;
+      const int64_t rows_per_block = kBlock / tpr;
+      const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block,
+                                                                 (int64_t)A->ctx->num_cu * 64));
+#define F3R_CSR_LAUNCH(T)                                                                                 \
+  hipLaunchKernelGGL((k_csr<V, X, Y, ACC, T>), dim3(g), dim3(kBlock), 0, st, n, A->d_rptr->as<int32_t>(), \
+                     A->d_idx->as<int32_t>(), (const V *)val, (const X *)x->ptr(), (Y *)y->ptr(), (const Y *)bp)
+      switch (tpr) {
+        case 1: F3R_CSR_LAUNCH(1); break;
+        case 2: F3R_CSR_LAUNCH(2); break;
+        case 4: F3R_CSR_LAUNCH(4); break;
+        case 8: F3R_CSR_LAUNCH(8); break;
+        case 16: F3R_CSR_LAUNCH(16); break;
+        case 32: F3R_CSR_LAUNCH(32); break;
+        default: F3R_CSR_LAUNCH(64); break;
+      }
+#undef F3R_CSR_LAUNCH
     };
-    if (x->dtype == F3R_F64 && y->dtype == F3R_F64) go(double(), double());
-    else if (x->dtype == F3R_F32 && y->dtype == F3R_F32) go(float(), float());
-    else if (x->dtype == F3R_F32 && y->dtype == F3R_F64) go(float(), double());
-    else if (x->dtype == F3R_F64 && y->dtype == F3R_F32) go(double(), float());
-    else throw Error{F3R_ERR_UNSUPPORTED, "CSR spmv: unsupported vector types"};
+    const int v = A->dtype, xd = x->dtype, yd = y->dtype;
+    if (v == F3R_F64 && xd == F3R_F64 && yd == F3R_F64) go(double(), double(), double());
+    else if (v == F3R_F64 && xd == F3R_F32 && yd == F3R_F32) go(double(), float(), float());
+    else if (v == F3R_F64 && xd == F3R_F32 && yd == F3R_F64) go(double(), float(), double());
+    else if (v == F3R_F64 && xd == F3R_F64 && yd == F3R_F32) go(double(), double(), float());
+    else if (v == F3R_F32 && xd == F3R_F32 && yd == F3R_F32) go(float(), float(), float());
+    else if (v == F3R_F32 && xd == F3R_F32 && yd == F3R_F64) go(float(), float(), double());
+    else if (v == F3R_F16 && xd == F3R_F32 && yd == F3R_F32) go(half_t(), float(), float());
+    else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F16) go(half_t(), half_t(), half_t());
+    else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F32) go(half_t(), half_t(), float());
+    else throw Error{F3R_ERR_UNSUPPORTED, "CSR spmv: unsupported (value, x, y) type combination"};
   } else {
     F3R_CHECK(!A->sh->internal, F3R_ERR_STATE, "spmv: internal-order matrix");
     const int v = A->dtype, xd = x->dtype, yd = y->dtype;
@@ -1237,8 +1283,19 @@ int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, c
 
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
   F3R_API_BEGIN
-  F3R_CHECK(sell && out && sell->m->format == 1 && (sell->m->sell_host_val || sell->m->sell_dev_val64), F3R_ERR_ARG,
-            "mat_cast: needs a SELL matrix");
+  F3R_CHECK(sell && out, F3R_ERR_ARG, "mat_cast: null argument");
+  F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16, F3R_ERR_ARG, "mat_cast: fp64, fp32 or fp16");
+  if (sell->m->format == 0) {  // CSR<T>(CSR<double>): the index arrays shared, the values in T
+    F3R_CHECK(sell->m->ctx && sell->m->d_rptr, F3R_ERR_ARG, "mat_cast: a CSR matrix on a context");
+    auto m = std::make_shared<Mat>(*sell->m);
+    m->dtype = dtype;
+    m->c_val = dtype == F3R_F64 ? nullptr : upload_values(m->ctx.get(), m->host->val, dtype);
+    F3R_HIP(hipStreamSynchronize(m->ctx->stream));
+    *out = new f3r_mat_s{m};
+    return F3R_OK;
+  }
+  F3R_CHECK(sell->m->format == 1 && (sell->m->sell_host_val || sell->m->sell_dev_val64), F3R_ERR_ARG,
+            "mat_cast: needs a SELL matrix or a CSR matrix");
   auto m = std::make_shared<Mat>(*sell->m);
   m->dtype = dtype;
   if (m->sell_host_val)
@@ -1267,6 +1324,16 @@ int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *
   if (nslots) *nslots = m.format == 1 ? m.sh->nslots : (m.host ? m.host->nnz() : 0);
   if (dtype) *dtype = m.dtype;
   if (format) *format = m.format;
+  F3R_API_END
+}
+
+int f3r_mat_set_threads_per_row(f3r_mat A, int32_t tpr) {
+  F3R_API_BEGIN
+  F3R_CHECK(A, F3R_ERR_ARG, "null matrix");
+  F3R_CHECK(A->m->format == 0, F3R_ERR_ARG, "set_threads_per_row: a CSR matrix");
+  F3R_CHECK(tpr == 0 || (tpr >= 1 && tpr <= 64 && (tpr & (tpr - 1)) == 0), F3R_ERR_ARG,
+            "set_threads_per_row: 0 (automatic) or a power of two in [1, 64]");
+  A->m->tpr = tpr;
   F3R_API_END
 }
 

@@ -1,38 +1,75 @@
-"""Autotuning of the SELL-C-sigma layout (chunk height C, sorting window
-sigma, K-slot interleave) for one matrix.
+"""Autotuning of the SpMV format for one matrix: SELL-C-sigma (chunk height
+C, sorting window sigma, K-slot interleave, chunk-offset index format) and
+CSR (lanes per row).
 
-Every candidate layout is built on the device and its SpMV timed with HIP
-events on the context's stream (SELL.time_spmv); the fastest wins.  The
-reference fixes C = 32 and sigma = 1 (SELL<32, T, device>,
-work/src/f3r-gpu.cpp:67, :214); the layout is a free parameter of this build,
-passed to F3R(C=, sigma=, K=).  Timing with sigma > 1 includes the row
+Every candidate is built on the device and its SpMV timed with HIP events on
+the context's stream (time_spmv); the fastest wins.  The reference fixes C =
+32 and sigma = 1 (SELL<32, T, device>, work/src/f3r-gpu.cpp:67, :214); the
+layout is a free parameter of this build, passed to F3R(C=, sigma=, K=,
+index=) and Batch(C=, K=, index=).  Timing with sigma > 1 includes the row
 permutation of x and y, which the solver avoids by renumbering the system,
-so the ranking leans slightly towards sigma = 1.
+so the ranking leans slightly towards sigma = 1.  Every SELL layout and the
+one-lane-per-row CSR sum each row in CSR order, so they give the same bits;
+CSR with several lanes per row reassociates its sums.
 """
 import itertools
 
 import numpy as np
 
-from . import SELL, vector
+from . import SELL, CSRDevice, vector
 
 DEFAULT_C = (32, 64, 128, 256)
 DEFAULT_SIGMA = (1, 256)
-DEFAULT_K = (1, 2, 4)
+DEFAULT_K = (1, 2, 4, 8)
+DEFAULT_OFFSETS = (False, True)
+DEFAULT_TPR = (1, 2, 4, 8, 16, 32)
 
 
-def autotune(ctx, A, dtype="f16", Cs=DEFAULT_C, sigmas=DEFAULT_SIGMA, Ks=DEFAULT_K, reps=20):
-    """returns (best, table): best = {"C", "sigma", "K", "ms"}; table lists
-    every candidate as (ms, C, sigma, K), fastest first"""
-    x = vector.from_numpy(ctx, np.ones(A.n), dtype)
-    y = vector(ctx, dtype, A.n)
-    table = []
-    for C, sigma, K in itertools.product(Cs, sigmas, Ks):
+def candidates(Cs=DEFAULT_C, sigmas=DEFAULT_SIGMA, Ks=DEFAULT_K, offsets=DEFAULT_OFFSETS, tprs=DEFAULT_TPR):
+    """every valid candidate as a dict: SELL (sigma = 1 or sigma >= C; the
+    chunk-offset format needs sigma = 1 and C >= 64) and CSR (tpr)"""
+    out = []
+    for C, sigma, K, off in itertools.product(Cs, sigmas, Ks, offsets):
         if sigma > 1 and sigma < C:
             continue
-        S = SELL(ctx, A, dtype, C, sigma, K)
-        ms = S.time_spmv(x, y, reps)
-        del S
-        table.append((ms, C, sigma, K))
-    table.sort()
-    ms, C, sigma, K = table[0]
-    return {"C": C, "sigma": sigma, "K": K, "ms": ms}, table
+        if off and (sigma != 1 or C < 64):
+            continue
+        out.append({"format": "sell", "C": C, "sigma": sigma, "K": K, "offsets": off})
+    for t in tprs:
+        out.append({"format": "csr", "tpr": t})
+    return out
+
+
+def build(ctx, A, dtype, cand):
+    if cand["format"] == "csr":
+        return CSRDevice(ctx, A, dtype, tpr=cand["tpr"])
+    return SELL(ctx, A, dtype, cand["C"], cand["sigma"], cand["K"], offsets=cand["offsets"])
+
+
+def autotune(ctx, A, dtype="f16", reps=20, check=False, **space):
+    """returns (best, table): best = the fastest candidate's dict with "ms";
+    table lists (ms, candidate) fastest first.  `space` narrows the search
+    (Cs, sigmas, Ks, offsets, tprs; tprs=() leaves CSR out).  check: every
+    candidate's y is also kept (table entries get "y")"""
+    x = vector.from_numpy(ctx, np.random.default_rng(0).uniform(-1, 1, A.n), dtype)
+    y = vector(ctx, dtype, A.n)
+    table = []
+    for cand in candidates(**space):
+        M = build(ctx, A, dtype, cand)
+        ms = M.time_spmv(x, y, reps)
+        entry = dict(cand, ms=ms)
+        if check:
+            entry["y"] = y.download()
+        del M
+        table.append(entry)
+    table.sort(key=lambda e: e["ms"])
+    best = {k: v for k, v in table[0].items() if k != "y"}
+    return best, table
+
+
+def solver_layout(best):
+    """F3R / Batch keywords of a SELL winner (CSR winners map to the SELL
+    layout of one column index per slot that sums alike: C 64, K 1)"""
+    if best["format"] == "csr":
+        return {"C": 64, "sigma": 1, "K": 1, "index": 0}
+    return {"C": best["C"], "sigma": best["sigma"], "K": best["K"], "index": 1 if best["offsets"] else 0}

@@ -192,8 +192,15 @@ int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, c
                            int32_t C, int32_t K, int64_t *nslots, int64_t *nidx, int64_t *ncoff,
                            int32_t *cptr, int32_t *cwidth, int32_t *iptr, int32_t *idx,
                            int32_t *coff, double *sval);
-/* cross-precision SELL sharing the index arrays (sell.hpp:51-53) */
+/* cross-precision copy sharing the index arrays: a SELL (sell.hpp:51-53) or a
+ * CSR on a context (CSR<T, device>(CSR<double>), csr.hpp:12-30) */
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out);
+/* lanes per row of a CSR matrix's SpMV: a power of two in [1, 64], or 0 for
+ * the largest power of two not above half the mean row length.  With 1 each row is
+ * summed in CSR order (bit-identical to the SELL kernels); with more, the
+ * lanes' strided partial sums are added by a butterfly (coalesced loads).
+ * No counterpart in the reference (one OpenMP iteration per row, csr.hpp:32-50). */
+int f3r_mat_set_threads_per_row(f3r_mat A, int32_t tpr);
 int f3r_mat_destroy(f3r_mat A);
 int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *dtype,
                  int *format);

@@ -1,6 +1,7 @@
 """Batched solves of the seeded family (independent systems on concurrent
 streams, autotuned SELL layouts): every system and right-hand side reaches a
 true fp64 relative residual of 1e-8."""
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -16,15 +17,44 @@ def test_batch_small():
         assert r["layout"]["C"] in (64, 128) and r["layout"]["K"] in (2, 4)
 
 
-def test_autotune_picks_the_fastest():
+def test_autotune_candidates_agree_and_the_tuned_solve_is_unchanged():
+    # every candidate computes the same SpMV (SELL layouts and one-lane CSR
+    # sum in CSR order: the same bits; several lanes per row within fp32
+    # rounding), the winner is the fastest, and a solve with the tuned layout
+    # gives the default layout's solution bit for bit
     import f3r_amd as F
-    from f3r_amd.tune import autotune
+    from f3r_amd.tune import autotune, solver_layout
     ctx = F.Context(0)
-    A = F.CSR.stencil27(48, beta=0.5)
+    A = F.CSR.stencil27(40, beta=0.5)
     F.algorithm.scaling(A)
-    best, table = autotune(ctx, A, "f16", Cs=(32, 64, 256), sigmas=(1, 256), Ks=(1, 4), reps=10)
-    assert best["ms"] == min(t[0] for t in table) and best["ms"] > 0
-    assert len(table) == 3 * 2 * 2   # every (C, sigma, K) with sigma = 1 or sigma >= C
+    best, table = autotune(ctx, A, "f16", Cs=(32, 64, 256), sigmas=(1, 256), Ks=(1, 4, 8), offsets=(False, True),
+                           tprs=(1, 4, 16), reps=5, check=True)
+    # SELL: 3 C x 2 sigma x 3 K with one column index per slot (sigma 256 >= every C), and the
+    # chunk-offset format for sigma 1 and C >= 64 (2 C x 3 K); CSR: 3 lane counts
+    sells = [e for e in table if e["format"] == "sell"]
+    assert len(sells) == 18 + 6 and len(table) == len(sells) + 3
+    assert best["ms"] == min(e["ms"] for e in table) > 0
+    ref = sells[0]["y"]
+    for e in table:
+        if e["format"] == "sell" or e["tpr"] == 1:
+            assert np.array_equal(e["y"].view(np.uint16), ref.view(np.uint16)), e
+        else:
+            assert np.allclose(e["y"].astype(np.float64), ref.astype(np.float64), rtol=2e-3, atol=1e-4), e
+    b = F.uniform(A.n, -1.0, 1.0, seed=3)
+    bv = F.vector.from_numpy(ctx, b)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    xs, its = [], []
+    for kw in ({}, solver_layout(best)):
+        x = F.vector(ctx, "f64", A.n).fill(0.0)
+        r = F.F3R(ctx, A, P, **kw).solve(bv, x)
+        assert r.is_solved
+        xs.append(x.download())
+        its.append(r.res_iter)
+    print("tuned", best, "outer steps", its, flush=True)
+    if solver_layout(best)["sigma"] == 1:  # the same row order: the same reductions, the same bits
+        assert np.array_equal(xs[0], xs[1])
+    else:  # a sorted row order changes the reductions' order
+        assert abs(its[0] - its[1]) <= 1
     ctx.close()
 
 

@@ -244,3 +244,55 @@ def test_device_sell_build_matches_host(ctx, C, K, offsets):
             Hd.operate(xv, y1)
             Dd.operate(xv, y2)
             assert np.array_equal(y1.download().view(np.uint8), y2.download().view(np.uint8)), (name, dt)
+
+
+# ----------------------------------------------------------- CSR SpMV ----
+CSR_COMBOS = COMBOS + [("f64", "f32", "f32", 0, 1, 1, 0)]
+
+
+@pytest.mark.parametrize("vd,xd,yd,vp,xp,yp,ap", CSR_COMBOS)
+def test_csr_spmv_one_lane_per_row_bitexact(ctx, vd, xd, yd, vp, xp, yp, ap):
+    # one lane per row sums in CSR order: the oracle's bits, residual too
+    import f3r_amd as F
+    for name, A in _mats():
+        x = np.random.default_rng(1).uniform(-1, 1, A.n).astype(NP[xd])
+        b = np.random.default_rng(2).uniform(-1, 1, A.n).astype(NP[yd])
+        print(f"csr spmv {name} {vd}/{xd}/{yd} tpr=1", flush=True)
+        M = F.CSRDevice(ctx, A, vd, tpr=1)
+        xv = F.vector.from_numpy(ctx, x)
+        yv = F.vector(ctx, yd, A.n)
+        M.operate(xv, yv)
+        ref = O.spmv(A.n, A.rptr, A.idx, A.val, x, vp, xp, yp, ap)
+        assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, vd, xd, yd)
+        bv = F.vector.from_numpy(ctx, b)
+        M.compute_residual(bv, xv, yv)
+        ref = O.spmv(A.n, A.rptr, A.idx, A.val, x, vp, xp, yp, ap, b=b)
+        assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, "resid")
+
+
+@pytest.mark.parametrize("tpr", [0, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("vd,xd,yd", [("f64", "f64", "f64"), ("f32", "f32", "f32"), ("f16", "f16", "f16"),
+                                      ("f16", "f32", "f32")])
+def test_csr_spmv_lanes_per_row(ctx, tpr, vd, xd, yd):
+    # several lanes per row reassociate the sum: within the accumulation's
+    # rounding of sum |a_ij x_j| (fp64 sums for fp64 values, else fp32) plus
+    # the output's rounding
+    import f3r_amd as F
+    for name, A in _mats():
+        x = np.random.default_rng(3).uniform(-1, 1, A.n).astype(NP[xd])
+        vals = A.val.astype(NP[vd]).astype(np.float64)
+        xs = x.astype(np.float64)
+        rows = np.repeat(np.arange(A.n), np.diff(A.rptr))
+        exact = np.bincount(rows, vals * xs[A.idx], minlength=A.n)
+        mag = np.bincount(rows, np.abs(vals * xs[A.idx]), minlength=A.n)
+        print(f"csr spmv {name} {vd}/{xd}/{yd} tpr={tpr}", flush=True)
+        M = F.CSRDevice(ctx, A, vd, tpr=tpr)
+        xv = F.vector.from_numpy(ctx, x)
+        yv = F.vector(ctx, yd, A.n)
+        M.operate(xv, yv)
+        got = yv.download().astype(np.float64)
+        nmax = int(np.diff(A.rptr).max())
+        u_acc = 2.0 ** -53 if vd == "f64" else 2.0 ** -24
+        u_out = {"f64": 2.0 ** -53, "f32": 2.0 ** -24, "f16": 2.0 ** -11}[yd]
+        tol = 2 * nmax * u_acc * mag + u_out * np.abs(exact) + (6e-8 if yd == "f16" else 0.0)
+        assert np.all(np.abs(got - exact) <= tol), (name, float(np.max(np.abs(got - exact) - tol)))
