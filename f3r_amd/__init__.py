@@ -382,14 +382,15 @@ class CSRDevice:
 class SELL:
     """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166)."""
 
-    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, device=False, _handle=None):
+    def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, device=False, lds=False,
+                 _handle=None):
         self.ctx = ctx
         self.A = A
         self.dtype = _dtype(dtype)
-        self.C, self.sigma, self.K, self.offsets = C, sigma, K, offsets
+        self.C, self.sigma, self.K, self.offsets, self.lds = C, sigma, K, offsets, lds
         if _handle is None:
             h = ctypes.c_void_p()
-            flags = (1 if offsets else 0) | (2 if device else 0)
+            flags = (1 if offsets else 0) | (2 if device else 0) | (4 if lds else 0)
             check(lib.f3r_mat_to_sell_ex(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, flags,
                                          ctypes.byref(h)), "mat_to_sell")
             self.handle = h
@@ -406,6 +407,12 @@ class SELL:
         ns = ctypes.c_int64()
         check(lib.f3r_mat_info(self.handle, None, None, ctypes.byref(ns), None, None), "mat_info")
         return ns.value
+
+    def lds_blocks(self):
+        """(blocks of 256 rows, blocks whose x window is staged in LDS)"""
+        b, st = ctypes.c_int32(), ctypes.c_int32()
+        check(lib.f3r_mat_lds_blocks(self.handle, ctypes.byref(b), ctypes.byref(st)), "mat_lds_blocks")
+        return b.value, st.value
 
     def chunks(self):
         """(chunks, chunks in the chunk-offset format)"""

@@ -139,6 +139,10 @@ struct SellShared {  // index data shared by all precisions of one matrix
   int32_t offset_chunks = 0;
   int64_t offset_slots = 0, explicit_nnz = 0;
   int32_t off_bias = 0;
+  // LDS-staged x gathers (F3R_SELL_LDS): per block of kBlock sorted rows, the
+  // column window (first column, length) its slots read; null when not used
+  DevPtr bwin;
+  int32_t lds_cap = 0, lds_blocks = 0;  // window capacity in elements of x; blocks that fit
   // algorithmic bytes of one SpMV: every nonzero's value once (no padding),
   // an int32 index only for the nonzeros of int32-index chunks (offset
   // chunks read none), x and y once

@@ -564,10 +564,47 @@ DevPtr upload_sell_values(Context *ctx, const HostSell &hs, int dtype) {
   return upload_values(ctx, hs.val, dtype);
 }
 
+// LDS of a block's x window in the LDS-staged SpMV: 32 KB, so that four
+// blocks of 256 rows still fit a CU's 160 KB
+constexpr size_t kLdsWindowBytes = 32 * 1024;
+
+// per block of kBlock sorted rows, the window of columns its slots read
+static std::vector<int2> block_windows(const HostSell &hs) {
+  const int64_t nb = ((int64_t)hs.n + kBlock - 1) / kBlock;
+  std::vector<int2> w(std::max<int64_t>(nb, 1), int2{0, 0});
+  const int C = hs.C, K = hs.K;
+  for (int32_t s = 0; s < hs.nchunks; ++s) {
+    const int32_t r0 = s * C, tc = std::min(C, hs.n - r0);
+    const int32_t groups = chunk_groups(hs.cwidth[s]);
+    for (int32_t j = 0; j < tc; ++j) {
+      const int64_t b = (r0 + j) / kBlock;
+      int lo = w[b].y > 0 ? w[b].x : INT32_MAX, hi = w[b].y > 0 ? w[b].x + w[b].y : INT32_MIN;
+      for (int32_t t = 0; t < groups * K; ++t) {
+        const int32_t c = hs.idx[chunk_slot_pos(hs.cptr[s], hs.cwidth[s], tc, j, t, K)];
+        lo = std::min(lo, c);
+        hi = std::max(hi, c + 1);
+      }
+      if (hi > lo) w[b] = int2{lo, hi - lo};
+    }
+  }
+  return w;
+}
+
 template <class V, class X, class Y>
 static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hipStream_t st) {
   using ACC = typename std::conditional<std::is_same<V, double>::value, double, float>::type;
   SellArgs a = sell_args(*A->sh, true);
+  if (A->sh->bwin) {  // LDS-staged x gathers
+    const int2 *w = A->sh->bwin->as<int2>();
+    const int cap = (int)(kLdsWindowBytes / sizeof(X));
+    if (b)
+      launch_sell_lds<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x, w, cap,
+                                 EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
+    else
+      launch_sell_lds<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x, w, cap,
+                                 EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
+    return;
+  }
   if (b)
     launch_sell<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
                            EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
@@ -1200,7 +1237,10 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
   F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
   *out = nullptr;
-  F3R_CHECK((flags & ~3) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  F3R_CHECK((flags & ~7) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  const bool lds = (flags & F3R_SELL_LDS) != 0;
+  F3R_CHECK(!lds || (flags & (F3R_SELL_OFFSETS | F3R_SELL_DEVICE)) == 0, F3R_ERR_ARG,
+            "to_sell: LDS-staged gathers need the int32-index layout built on the host");
   if ((flags & 2) && sigma == 1 && csr->m->d_rptr) {  // build on the device
     std::shared_ptr<SellShared> sh;
     DevPtr v64;
@@ -1230,6 +1270,15 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   m->sh = upload_sell_shared(ctx->c.get(), hs, false);
   m->sh->nnz = csr->m->host->nnz();
   m->s_val = upload_sell_values(ctx->c.get(), hs, dtype);
+  if (lds) {
+    const std::vector<int2> w = block_windows(hs);
+    const int32_t cap = (int32_t)(kLdsWindowBytes / dtype_size(dtype));  // x of the matrix's precision
+    m->sh->lds_cap = cap;
+    for (const int2 &q : w) m->sh->lds_blocks += q.y <= cap ? 1 : 0;
+    m->sh->bwin = dev_alloc(sizeof(int2) * w.size());
+    F3R_HIP(hipMemcpyAsync(m->sh->bwin->p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice, ctx->c->stream));
+    F3R_HIP(hipStreamSynchronize(ctx->c->stream));
+  }
   m->sell_host_val = std::make_shared<std::vector<double>>(std::move(hs.val));
   *out = new f3r_mat_s{m};
   F3R_API_END
@@ -1334,6 +1383,15 @@ int f3r_mat_set_threads_per_row(f3r_mat A, int32_t tpr) {
   F3R_CHECK(tpr == 0 || (tpr >= 1 && tpr <= 64 && (tpr & (tpr - 1)) == 0), F3R_ERR_ARG,
             "set_threads_per_row: 0 (automatic) or a power of two in [1, 64]");
   A->m->tpr = tpr;
+  F3R_API_END
+}
+
+int f3r_mat_lds_blocks(f3r_mat A, int32_t *blocks, int32_t *staged) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && blocks && staged, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(A->m->format == 1, F3R_ERR_ARG, "mat_lds_blocks: needs a SELL matrix");
+  *blocks = (int32_t)((A->m->sh->n + kBlock - 1) / kBlock);
+  *staged = A->m->sh->bwin ? A->m->sh->lds_blocks : 0;
   F3R_API_END
 }
 

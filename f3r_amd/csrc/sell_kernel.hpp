@@ -238,6 +238,73 @@ __global__ __launch_bounds__(kBlock) void k_sell(SellArgs a, const V *__restrict
   epi.finish(st);
 }
 
+// Row r of an int32-index SELL with x staged in LDS: xs[c - lo] holds x[c]
+// for the block's column window [lo, lo + len).  The slots and the sum are
+// those of sell_row (same bits); only the source of x differs.
+template <class V, class X, class ACC, int K>
+__device__ __forceinline__ ACC sell_row_lds(const SellArgs &a, const V *__restrict__ val, const X *xs, int lo, int r) {
+  const int s = r >> a.log2C;
+  const int j = r - (s << a.log2C);
+  const int C = 1 << a.log2C;
+  const int rem = a.n - (s << a.log2C);
+  const int tc = rem < C ? rem : C;
+  const int groups = chunk_groups(a.cwidth[s]);
+  int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K;
+  const int64_t step = (int64_t)tc * K;
+  const int32_t *__restrict__ ip = a.idx + pos;
+  ACC acc = 0;
+  for (int g = 0; g < groups; ++g, pos += step, ip += step) {
+    const Pack<V, K> pv = load_pack<V, K>(val + pos);
+    const Pack<int32_t, K> pi = load_pack<int32_t, K>(ip);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = fma_<ACC>((ACC)to_acc(pv.v[k]), (ACC)to_acc(xs[pi.v[k] - lo]), acc);
+  }
+  return acc;
+}
+
+// SELL SpMV with LDS-staged x gathers (int32-index chunks only): one row per
+// thread; a block whose column window fits the LDS (bwin[b] = {lo, len},
+// len <= cap) first copies x[lo, lo + len) into LDS with coalesced loads, then
+// gathers from LDS; a wider block gathers from global memory as k_sell does.
+// Worth it when the blocks' windows are narrow (banded matrices); the
+// autotuner decides.
+template <class V, class X, class ACC, int K, class Epi>
+__global__ __launch_bounds__(kBlock) void k_sell_lds(SellArgs a, const V *__restrict__ val, const X *__restrict__ x,
+                                                     const int2 *__restrict__ bwin, int cap, Epi epi) {
+  extern __shared__ unsigned char sell_lds_raw[];
+  X *xs = reinterpret_cast<X *>(sell_lds_raw);
+  const int b = xcd_block();
+  const int2 win = bwin[b];
+  const bool staged = win.y <= cap;
+  if (staged) {
+    for (int i = threadIdx.x; i < win.y; i += kBlock) xs[i] = x[win.x + i];
+    __syncthreads();
+  }
+  typename Epi::State st{};
+  const int r = b * kBlock + threadIdx.x;
+  if (r < a.n) {
+    const typename Epi::Pre p = epi.pre(r);
+    epi.row(r, staged ? sell_row_lds<V, X, ACC, K>(a, val, xs, win.x, r) : sell_row<V, X, ACC, K>(a, val, x, r), p,
+            st);
+  }
+  epi.finish(st);
+}
+
+template <class V, class X, class ACC, class Epi>
+inline void launch_sell_lds(int K, const SellArgs &a, const V *val, const X *x, const int2 *bwin, int cap, Epi epi,
+                            hipStream_t st) {
+  if (a.n == 0) return;
+  const dim3 g((a.n + kBlock - 1) / kBlock);
+  const size_t lds = (size_t)cap * sizeof(X);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((k_sell_lds<V, X, ACC, 1, Epi>), g, dim3(kBlock), lds, st, a, val, x, bwin, cap, epi); break;
+    case 2: hipLaunchKernelGGL((k_sell_lds<V, X, ACC, 2, Epi>), g, dim3(kBlock), lds, st, a, val, x, bwin, cap, epi); break;
+    case 4: hipLaunchKernelGGL((k_sell_lds<V, X, ACC, 4, Epi>), g, dim3(kBlock), lds, st, a, val, x, bwin, cap, epi); break;
+    case 8: hipLaunchKernelGGL((k_sell_lds<V, X, ACC, 8, Epi>), g, dim3(kBlock), lds, st, a, val, x, bwin, cap, epi); break;
+    default: break;
+  }
+}
+
 // grid: 0 = one row per thread; otherwise the given (capped) block count
 template <class V, class X, class ACC, class Epi>
 inline void launch_sell(int K, const SellArgs &a, const V *val, const X *x, Epi epi,

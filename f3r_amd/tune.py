@@ -1,6 +1,6 @@
 """Autotuning of the SpMV format for one matrix: SELL-C-sigma (chunk height
-C, sorting window sigma, K-slot interleave, chunk-offset index format) and
-CSR (lanes per row).
+C, sorting window sigma, K-slot interleave, chunk-offset index format,
+LDS-staged x gathers) and CSR (lanes per row).
 
 Every candidate is built on the device and its SpMV timed with HIP events on
 the context's stream (time_spmv); the fastest wins.  The reference fixes C =
@@ -22,19 +22,24 @@ DEFAULT_C = (32, 64, 128, 256)
 DEFAULT_SIGMA = (1, 256)
 DEFAULT_K = (1, 2, 4, 8)
 DEFAULT_OFFSETS = (False, True)
+DEFAULT_LDS = (False, True)
 DEFAULT_TPR = (1, 2, 4, 8, 16, 32)
 
 
-def candidates(Cs=DEFAULT_C, sigmas=DEFAULT_SIGMA, Ks=DEFAULT_K, offsets=DEFAULT_OFFSETS, tprs=DEFAULT_TPR):
+def candidates(Cs=DEFAULT_C, sigmas=DEFAULT_SIGMA, Ks=DEFAULT_K, offsets=DEFAULT_OFFSETS, lds=DEFAULT_LDS,
+               tprs=DEFAULT_TPR):
     """every valid candidate as a dict: SELL (sigma = 1 or sigma >= C; the
-    chunk-offset format needs sigma = 1 and C >= 64) and CSR (tpr)"""
+    chunk-offset format needs sigma = 1 and C >= 64; LDS-staged gathers the
+    int32-index format) and CSR (tpr)"""
     out = []
-    for C, sigma, K, off in itertools.product(Cs, sigmas, Ks, offsets):
+    for C, sigma, K, off, st in itertools.product(Cs, sigmas, Ks, offsets, lds):
         if sigma > 1 and sigma < C:
             continue
         if off and (sigma != 1 or C < 64):
             continue
-        out.append({"format": "sell", "C": C, "sigma": sigma, "K": K, "offsets": off})
+        if off and st:
+            continue
+        out.append({"format": "sell", "C": C, "sigma": sigma, "K": K, "offsets": off, "lds": st})
     for t in tprs:
         out.append({"format": "csr", "tpr": t})
     return out
@@ -43,13 +48,13 @@ def candidates(Cs=DEFAULT_C, sigmas=DEFAULT_SIGMA, Ks=DEFAULT_K, offsets=DEFAULT
 def build(ctx, A, dtype, cand):
     if cand["format"] == "csr":
         return CSRDevice(ctx, A, dtype, tpr=cand["tpr"])
-    return SELL(ctx, A, dtype, cand["C"], cand["sigma"], cand["K"], offsets=cand["offsets"])
+    return SELL(ctx, A, dtype, cand["C"], cand["sigma"], cand["K"], offsets=cand["offsets"], lds=cand["lds"])
 
 
 def autotune(ctx, A, dtype="f16", reps=20, check=False, **space):
     """returns (best, table): best = the fastest candidate's dict with "ms";
     table lists (ms, candidate) fastest first.  `space` narrows the search
-    (Cs, sigmas, Ks, offsets, tprs; tprs=() leaves CSR out).  check: every
+    (Cs, sigmas, Ks, offsets, lds, tprs; tprs=() leaves CSR out).  check: every
     candidate's y is also kept (table entries get "y")"""
     x = vector.from_numpy(ctx, np.random.default_rng(0).uniform(-1, 1, A.n), dtype)
     y = vector(ctx, dtype, A.n)
@@ -69,7 +74,52 @@ def autotune(ctx, A, dtype="f16", reps=20, check=False, **space):
 
 def solver_layout(best):
     """F3R / Batch keywords of a SELL winner (CSR winners map to the SELL
-    layout of one column index per slot that sums alike: C 64, K 1)"""
+    layout of one column index per slot that sums alike: C 64, K 1; the
+    solver's fused kernels gather from global memory, so LDS staging is
+    dropped)"""
     if best["format"] == "csr":
         return {"C": 64, "sigma": 1, "K": 1, "index": 0}
     return {"C": best["C"], "sigma": best["sigma"], "K": best["K"], "index": 1 if best["offsets"] else 0}
+
+
+def _problem(spec):
+    """stencil27:<nx>[:<beta>] | stencil7:<nx>,<ny>,<nz> | mtx:<path>"""
+    from . import CSR, algorithm, io
+    kind, _, arg = spec.partition(":")
+    if kind == "stencil27":
+        nx, _, beta = arg.partition(":")
+        A = CSR.stencil27(int(nx), beta=float(beta or 0.0))
+    elif kind == "stencil7":
+        nx, ny, nz = (int(t) for t in arg.split(","))
+        A = CSR.dominant(7, nx, ny, nz, margin=1.0)
+    elif kind == "mtx":
+        A = io.mm.read_matrix(arg)
+    else:
+        raise ValueError(f"unknown problem {spec}")
+    algorithm.scaling(A)
+    return A
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="autotune the SpMV format of one matrix and print the table")
+    ap.add_argument("--problem", default="stencil27:128:0.5",
+                    help="stencil27:<nx>[:<beta>] | stencil7:<nx>,<ny>,<nz> (2-D: nz = 1) | mtx:<path>")
+    ap.add_argument("--dtype", default="f16")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--no-csr", action="store_true")
+    a = ap.parse_args(argv)
+    from . import Context
+    ctx = Context(0)
+    A = _problem(a.problem)
+    best, table = autotune(ctx, A, a.dtype, reps=a.reps, **({"tprs": ()} if a.no_csr else {}))
+    for e in table:
+        print(json.dumps({k: (round(v * 1e3, 2) if k == "ms" else v) for k, v in e.items()}), flush=True)
+    print(json.dumps({"problem": a.problem, "n": A.n, "nnz": A.nnz, "dtype": a.dtype,
+                      "best": {k: (round(v * 1e3, 2) if k == "ms" else v) for k, v in best.items()}}))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

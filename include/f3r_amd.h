@@ -170,6 +170,12 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
 /* flag bit 1: build the layout on the device from the CSR's device copy
  * (sigma = 1 only; identical arrays to the host build) */
 #define F3R_SELL_DEVICE 2
+/* flag bit 2: LDS-staged x gathers (int32-index layout built on the host):
+ * every block of 256 rows whose slots read a column window of at most 32 KB
+ * of x copies that window into LDS with coalesced loads and gathers from
+ * there; wider blocks gather from global memory.  Same sums, same bits.  For
+ * banded matrices; f3r_mat_lds_blocks tells how many blocks are staged. */
+#define F3R_SELL_LDS 4
 int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
                        int32_t K, int32_t flags, f3r_mat *sell);
 /* host-only SELL-C-sigma layout of a CSR matrix (csr_to_sell /
@@ -204,6 +210,8 @@ int f3r_mat_set_threads_per_row(f3r_mat A, int32_t tpr);
 int f3r_mat_destroy(f3r_mat A);
 int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *dtype,
                  int *format);
+/* SELL with F3R_SELL_LDS: blocks of 256 rows, and blocks whose x window fits the LDS */
+int f3r_mat_lds_blocks(f3r_mat A, int32_t *blocks, int32_t *staged);
 /* SELL: chunks in total and chunks stored in the chunk-offset format */
 int f3r_mat_chunks(f3r_mat A, int32_t *chunks, int32_t *offset_chunks);
 /* y = A x  (SELL::operate / CSR::operate, sell.hpp:55-71, csr.hpp:32-40).

@@ -28,11 +28,12 @@ def test_autotune_candidates_agree_and_the_tuned_solve_is_unchanged():
     A = F.CSR.stencil27(40, beta=0.5)
     F.algorithm.scaling(A)
     best, table = autotune(ctx, A, "f16", Cs=(32, 64, 256), sigmas=(1, 256), Ks=(1, 4, 8), offsets=(False, True),
-                           tprs=(1, 4, 16), reps=5, check=True)
-    # SELL: 3 C x 2 sigma x 3 K with one column index per slot (sigma 256 >= every C), and the
-    # chunk-offset format for sigma 1 and C >= 64 (2 C x 3 K); CSR: 3 lane counts
+                           lds=(False, True), tprs=(1, 4, 16), reps=5, check=True)
+    # SELL: 3 C x 2 sigma x 3 K with one column index per slot (sigma 256 >= every C), with and
+    # without LDS-staged gathers, and the chunk-offset format for sigma 1 and C >= 64 (2 C x 3 K);
+    # CSR: 3 lane counts
     sells = [e for e in table if e["format"] == "sell"]
-    assert len(sells) == 18 + 6 and len(table) == len(sells) + 3
+    assert len(sells) == 2 * 18 + 6 and len(table) == len(sells) + 3
     assert best["ms"] == min(e["ms"] for e in table) > 0
     ref = sells[0]["y"]
     for e in table:

@@ -304,3 +304,41 @@ def test_csr_spmv_lanes_per_row(ctx, tpr, vd, xd, yd):
         u_out = {"f64": 2.0 ** -53, "f32": 2.0 ** -24, "f16": 2.0 ** -11}[yd]
         tol = 2 * nmax * u_acc * mag + u_out * np.abs(exact) + (6e-8 if yd == "f16" else 0.0)
         assert np.all(np.abs(got - exact) <= tol), (name, float(np.max(np.abs(got - exact) - tol)))
+
+
+# ----------------------------------------------- LDS-staged x gathers ----
+@pytest.mark.parametrize("C,sigma,K", [(64, 1, 4), (32, 1, 1), (128, 256, 2), (64, 1, 8)])
+def test_sell_lds_gathers_bitexact(ctx, C, sigma, K):
+    # the same slots and sums as the global gathers: the oracle's bits, for
+    # banded matrices (every block staged) and a wide random one (none staged)
+    import f3r_amd as F
+    mats = [("band2d", F.CSR(*PB.stencil7(256, 48, 1))), ("tridiag", F.CSR(*PB.stencil7(5000, 1, 1))),
+            ("wide", F.CSR(*PB.random_sparse(40000, 12, 7)))]
+    for name, A in mats:
+        x = np.random.default_rng(1).uniform(-1, 1, A.n)
+        b = np.random.default_rng(2).uniform(-1, 1, A.n)
+        for vd, xd, yd, vp, xp, yp, ap in COMBOS:
+            print(f"lds spmv {name} {vd}/{xd}/{yd} C={C} sigma={sigma} K={K}", flush=True)
+            S = F.SELL(ctx, A, vd, C=C, sigma=sigma, K=K, lds=True)
+            nb, staged = S.lds_blocks()
+            if name == "wide":
+                assert staged < nb // 2, (nb, staged)
+            elif sigma == 1:
+                assert staged == nb, (name, nb, staged)
+            xv = F.vector.from_numpy(ctx, x.astype(NP[xd]))
+            yv = F.vector(ctx, yd, A.n)
+            S.operate(xv, yv)
+            ref = O.spmv(A.n, A.rptr, A.idx, A.val, x.astype(NP[xd]), vp, xp, yp, ap)
+            assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, vd, xd, yd)
+            if yd == xd:
+                bv = F.vector.from_numpy(ctx, b.astype(NP[yd]))
+                S.compute_residual(bv, xv, yv)
+                ref = O.spmv(A.n, A.rptr, A.idx, A.val, x.astype(NP[xd]), vp, xp, yp, ap, b=b.astype(NP[yd]))
+                assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, "resid")
+
+
+def test_sell_lds_rejects_offsets(ctx):
+    import f3r_amd as F
+    A = F.CSR(*PB.stencil7(16, 16, 1))
+    with pytest.raises(F.F3RError):
+        F.SELL(ctx, A, "f16", C=64, sigma=1, K=4, offsets=True, lds=True)
