@@ -12,14 +12,15 @@ from collections import namedtuple
 import numpy as np
 
 from . import _capi
-from ._capi import F16, F32, F64, I32, F3RError, check, lib
+from ._capi import BF16, F16, F32, F64, I32, F3RError, check, lib
 
-__all__ = ["Context", "vector", "CSR", "CSRDevice", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
+__all__ = ["BF16", "Context", "vector", "CSR", "CSRDevice", "SELL", "SDAINV", "ILU0", "F3R", "Batch", "algorithm", "io", "solve_res_t",
            "Operator", "AutoRichardson", "FGMRES", "Restarted",
            "F3RError", "F64", "F32", "F16", "device_count"]
 
-_NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32}
-_DT = {"f64": F64, "f32": F32, "f16": F16, "i32": I32, "double": F64, "float": F32, "half": F16}
+# bf16 vectors move as their 16-bit patterns (numpy has no bfloat16)
+_NP = {F64: np.float64, F32: np.float32, F16: np.float16, I32: np.int32, BF16: np.uint16}
+_DT = {"f64": F64, "f32": F32, "f16": F16, "i32": I32, "bf16": BF16, "double": F64, "float": F32, "half": F16}
 
 
 def _dtype(d):
@@ -86,6 +87,10 @@ class vector:
         arr = np.asarray(arr)
         dt = _dtype(dtype) if dtype is not None else _dtype(arr.dtype)
         v = cls(ctx, dt, arr.size)
+        if dt == BF16 and arr.dtype != np.uint16:
+            # values (not bit patterns): rounded to bf16 on the device from fp64
+            v.copy(cls.from_numpy(ctx, arr.astype(np.float64)))
+            return v
         v.upload(arr)
         return v
 

@@ -9,7 +9,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libf3r_amd.so")
 
-F64, F32, F16, I32 = 0, 1, 2, 3
+F64, F32, F16, I32, BF16 = 0, 1, 2, 3, 4
 OK = 0
 ERR_ARG = -1
 

@@ -222,6 +222,9 @@ DevPtr cast_values_device(Context *c, const DevPtr &v64, int64_t n, int dtype) {
   if (dtype == F3R_F32)
     hipLaunchKernelGGL(k_vals_cast<float>, dim3(2048), dim3(256), 0, c->stream, n, (const double *)v64->as<double>(),
                        d->as<float>());
+  else if (dtype == F3R_BF16)
+    hipLaunchKernelGGL(k_vals_cast<bf16_t>, dim3(2048), dim3(256), 0, c->stream, n,
+                       (const double *)v64->as<double>(), d->as<bf16_t>());
   else
     hipLaunchKernelGGL(k_vals_cast<half_t>, dim3(2048), dim3(256), 0, c->stream, n,
                        (const double *)v64->as<double>(), d->as<half_t>());

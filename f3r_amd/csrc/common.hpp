@@ -38,9 +38,11 @@ inline size_t dtype_size(int dt) {
     case F3R_F32: return 4;
     case F3R_F16: return 2;
     case F3R_I32: return 4;
+    case F3R_BF16: return 2;
   }
   return 0;
 }
+
 
 // ------------------------------------------------------ device memory -----
 struct DevBuf {
@@ -143,6 +145,7 @@ struct SellShared {  // index data shared by all precisions of one matrix
   // column window (first column, length) its slots read; null when not used
   DevPtr bwin;
   int32_t lds_cap = 0, lds_blocks = 0;  // window capacity in elements of x; blocks that fit
+  int32_t lds_fit[4] = {0, 0, 0, 0};    // by log2(sizeof x) - 1: the widest window that fits
   // algorithmic bytes of one SpMV: every nonzero's value once (no padding),
   // an int32 index only for the nonzeros of int32-index chunks (offset
   // chunks read none), x and y once

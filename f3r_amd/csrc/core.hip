@@ -347,6 +347,11 @@ void vec_fill(Vec *v, double val) {
     case F3R_F64: hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kBlock), 0, st, (double *)v->ptr(), v->n, val); break;
     case F3R_F32: hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kBlock), 0, st, (float *)v->ptr(), v->n, (float)val); break;
     case F3R_F16: hipLaunchKernelGGL(k_fill<half_t>, dim3(g), dim3(kBlock), 0, st, (half_t *)v->ptr(), v->n, (half_t)(float)val); break;
+    case F3R_BF16: {
+      const uint16_t bits = d2bf_bits(val);
+      hipLaunchKernelGGL(k_fill<uint16_t>, dim3(g), dim3(kBlock), 0, st, (uint16_t *)v->ptr(), v->n, bits);
+      break;
+    }
     case F3R_I32: hipLaunchKernelGGL(k_fill<int32_t>, dim3(g), dim3(kBlock), 0, st, (int32_t *)v->ptr(), v->n, (int32_t)val); break;
   }
   F3R_HIP(hipGetLastError());
@@ -357,6 +362,7 @@ template <class S> static void cast_from(const S *s, Vec *d, hipStream_t st, int
     case F3R_F64: hipLaunchKernelGGL((k_cast<S, double>), dim3(g), dim3(kBlock), 0, st, s, (double *)d->ptr(), d->n); break;
     case F3R_F32: hipLaunchKernelGGL((k_cast<S, float>), dim3(g), dim3(kBlock), 0, st, s, (float *)d->ptr(), d->n); break;
     case F3R_F16: hipLaunchKernelGGL((k_cast<S, half_t>), dim3(g), dim3(kBlock), 0, st, s, (half_t *)d->ptr(), d->n); break;
+    case F3R_BF16: hipLaunchKernelGGL((k_cast<S, bf16_t>), dim3(g), dim3(kBlock), 0, st, s, (bf16_t *)d->ptr(), d->n); break;
   }
 }
 
@@ -373,6 +379,7 @@ void vec_copy(Vec *s, Vec *d) {
     case F3R_F64: cast_from((const double *)s->ptr(), d, st, g); break;
     case F3R_F32: cast_from((const float *)s->ptr(), d, st, g); break;
     case F3R_F16: cast_from((const half_t *)s->ptr(), d, st, g); break;
+    case F3R_BF16: cast_from((const bf16_t *)s->ptr(), d, st, g); break;
   }
   F3R_HIP(hipGetLastError());
 }
@@ -385,6 +392,7 @@ void vec_axpy(double a, Vec *x, Vec *y) {
     case F3R_F64: hipLaunchKernelGGL(k_axpy<double>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const double *)x->ptr(), (double *)y->ptr()); break;
     case F3R_F32: hipLaunchKernelGGL(k_axpy<float>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const float *)x->ptr(), (float *)y->ptr()); break;
     case F3R_F16: hipLaunchKernelGGL(k_axpy<half_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const half_t *)x->ptr(), (half_t *)y->ptr()); break;
+    case F3R_BF16: hipLaunchKernelGGL(k_axpy<bf16_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (const bf16_t *)x->ptr(), (bf16_t *)y->ptr()); break;
     default: throw Error{F3R_ERR_UNSUPPORTED, "axpy: int32"};
   }
   F3R_HIP(hipGetLastError());
@@ -397,6 +405,7 @@ void vec_scale(Vec *x, double a) {
     case F3R_F64: hipLaunchKernelGGL(k_scal<double>, dim3(g), dim3(kBlock), 0, st, x->n, a, (double *)x->ptr()); break;
     case F3R_F32: hipLaunchKernelGGL(k_scal<float>, dim3(g), dim3(kBlock), 0, st, x->n, a, (float *)x->ptr()); break;
     case F3R_F16: hipLaunchKernelGGL(k_scal<half_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (half_t *)x->ptr()); break;
+    case F3R_BF16: hipLaunchKernelGGL(k_scal<bf16_t>, dim3(g), dim3(kBlock), 0, st, x->n, a, (bf16_t *)x->ptr()); break;
     default: throw Error{F3R_ERR_UNSUPPORTED, "scale: int32"};
   }
   F3R_HIP(hipGetLastError());
@@ -424,12 +433,14 @@ double dot_raw(Context *c, int da, const void *pa, int db, const void *pb, int64
       case F3R_F64: { F3R_DOT(TA, double) break; }
       case F3R_F32: { F3R_DOT(TA, float) break; }
       case F3R_F16: { F3R_DOT(TA, half_t) break; }
+      case F3R_BF16: { F3R_DOT(TA, bf16_t) break; }
     }
   };
   switch (da) {
     case F3R_F64: pick_b(double()); break;
     case F3R_F32: pick_b(float()); break;
     case F3R_F16: pick_b(half_t()); break;
+    case F3R_BF16: pick_b(bf16_t()); break;
   }
 #undef F3R_DOT
   F3R_HIP(hipGetLastError());
@@ -465,12 +476,14 @@ static void dot_noalloc(Context *c, int da, const void *pa, int db, const void *
       case F3R_F64: { F3R_DOTNA(TA, double) break; }
       case F3R_F32: { F3R_DOTNA(TA, float) break; }
       case F3R_F16: { F3R_DOTNA(TA, half_t) break; }
+      case F3R_BF16: { F3R_DOTNA(TA, bf16_t) break; }
     }
   };
   switch (da) {
     case F3R_F64: pick_b(double()); break;
     case F3R_F32: pick_b(float()); break;
     case F3R_F16: pick_b(half_t()); break;
+    case F3R_BF16: pick_b(bf16_t()); break;
   }
 #undef F3R_DOTNA
   F3R_HIP(hipGetLastError());
@@ -549,13 +562,13 @@ DevPtr upload_values(Context *ctx, const std::vector<double> &v, int dtype) {
     for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
     F3R_HIP(hipMemcpyAsync(d->p, f.data(), 4 * f.size(), hipMemcpyHostToDevice, st));
     F3R_HIP(hipStreamSynchronize(st));
-  } else if (dtype == F3R_F16) {
+  } else if (dtype == F3R_F16 || dtype == F3R_BF16) {
     std::vector<uint16_t> h(v.size());
-    for (size_t i = 0; i < v.size(); ++i) h[i] = d2h_bits(v[i]);
+    for (size_t i = 0; i < v.size(); ++i) h[i] = dtype == F3R_F16 ? d2h_bits(v[i]) : d2bf_bits(v[i]);
     F3R_HIP(hipMemcpyAsync(d->p, h.data(), 2 * h.size(), hipMemcpyHostToDevice, st));
     F3R_HIP(hipStreamSynchronize(st));
   } else {
-    throw Error{F3R_ERR_UNSUPPORTED, "matrix values must be fp64, fp32 or fp16"};
+    throw Error{F3R_ERR_UNSUPPORTED, "matrix values must be fp64, fp32, fp16 or bf16"};
   }
   return d;
 }
@@ -596,7 +609,8 @@ static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hi
   SellArgs a = sell_args(*A->sh, true);
   if (A->sh->bwin) {  // LDS-staged x gathers
     const int2 *w = A->sh->bwin->as<int2>();
-    const int cap = (int)(kLdsWindowBytes / sizeof(X));
+    // LDS sized to the widest window that fits, so narrow bands keep the occupancy
+    const int cap = A->sh->lds_fit[sizeof(X) == 2 ? 0 : sizeof(X) == 4 ? 1 : 2];
     if (b)
       launch_sell_lds<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x, w, cap,
                                  EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
@@ -665,6 +679,8 @@ void mat_apply(const Mat *A, Vec *x, Vec *y, Vec *b) {
     else if (v == F3R_F16 && xd == F3R_F32 && yd == F3R_F32) go(half_t(), float(), float());
     else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F16) go(half_t(), half_t(), half_t());
     else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F32) go(half_t(), half_t(), float());
+    else if (v == F3R_BF16 && xd == F3R_BF16 && yd == F3R_BF16) go(bf16_t(), bf16_t(), bf16_t());
+    else if (v == F3R_BF16 && xd == F3R_F32 && yd == F3R_F32) go(bf16_t(), float(), float());
     else throw Error{F3R_ERR_UNSUPPORTED, "CSR spmv: unsupported (value, x, y) type combination"};
   } else {
     F3R_CHECK(!A->sh->internal, F3R_ERR_STATE, "spmv: internal-order matrix");
@@ -676,6 +692,8 @@ void mat_apply(const Mat *A, Vec *x, Vec *y, Vec *b) {
     else if (v == F3R_F16 && xd == F3R_F32 && yd == F3R_F32) sell_apply_t<half_t, float, float>(A, x->ptr(), y->ptr(), bp, st);
     else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F16) sell_apply_t<half_t, half_t, half_t>(A, x->ptr(), y->ptr(), bp, st);
     else if (v == F3R_F16 && xd == F3R_F16 && yd == F3R_F32) sell_apply_t<half_t, half_t, float>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_BF16 && xd == F3R_BF16 && yd == F3R_BF16) sell_apply_t<bf16_t, bf16_t, bf16_t>(A, x->ptr(), y->ptr(), bp, st);
+    else if (v == F3R_BF16 && xd == F3R_F32 && yd == F3R_F32) sell_apply_t<bf16_t, float, float>(A, x->ptr(), y->ptr(), bp, st);
     else throw Error{F3R_ERR_UNSUPPORTED, "SELL spmv: unsupported (value, x, y) type combination"};
   }
   F3R_HIP(hipGetLastError());
@@ -1275,6 +1293,11 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
     const int32_t cap = (int32_t)(kLdsWindowBytes / dtype_size(dtype));  // x of the matrix's precision
     m->sh->lds_cap = cap;
     for (const int2 &q : w) m->sh->lds_blocks += q.y <= cap ? 1 : 0;
+    for (int e = 0; e < 3; ++e) {
+      const int32_t lim = (int32_t)(kLdsWindowBytes >> (e + 1));  // x of 2, 4, 8 bytes
+      for (const int2 &q : w)
+        if (q.y <= lim) m->sh->lds_fit[e] = std::max(m->sh->lds_fit[e], q.y);
+    }
     m->sh->bwin = dev_alloc(sizeof(int2) * w.size());
     F3R_HIP(hipMemcpyAsync(m->sh->bwin->p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice, ctx->c->stream));
     F3R_HIP(hipStreamSynchronize(ctx->c->stream));
@@ -1333,7 +1356,8 @@ int f3r_sell_offset_layout(int32_t n, const int32_t *rptr, const int32_t *col, c
 int f3r_mat_cast(f3r_mat sell, int dtype, f3r_mat *out) {
   F3R_API_BEGIN
   F3R_CHECK(sell && out, F3R_ERR_ARG, "mat_cast: null argument");
-  F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16, F3R_ERR_ARG, "mat_cast: fp64, fp32 or fp16");
+  F3R_CHECK(dtype == F3R_F64 || dtype == F3R_F32 || dtype == F3R_F16 || dtype == F3R_BF16, F3R_ERR_ARG,
+            "mat_cast: fp64, fp32, fp16 or bf16");
   if (sell->m->format == 0) {  // CSR<T>(CSR<double>): the index arrays shared, the values in T
     F3R_CHECK(sell->m->ctx && sell->m->d_rptr, F3R_ERR_ARG, "mat_cast: a CSR matrix on a context");
     auto m = std::make_shared<Mat>(*sell->m);

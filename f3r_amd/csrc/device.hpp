@@ -10,6 +10,7 @@
 namespace f3r {
 
 using half_t = _Float16;
+using bf16_t = __bf16;
 
 constexpr int kBlock = 256;          // threads per block of the streaming kernels
 constexpr int kWave = 64;            // CDNA wavefront
@@ -20,6 +21,7 @@ template <class T> struct Acc { using type = float; };   // fp16 / fp32 data: fp
 template <> struct Acc<double> { using type = double; };
 
 __device__ __forceinline__ float to_acc(half_t v) { return (float)v; }
+__device__ __forceinline__ float to_acc(bf16_t v) { return (float)v; }
 __device__ __forceinline__ float to_acc(float v) { return v; }
 __device__ __forceinline__ double to_acc(double v) { return v; }
 
@@ -79,6 +81,48 @@ __device__ __forceinline__ half_t d2h(double d) {
   return __builtin_bit_cast(half_t, bits);
 }
 template <> __device__ __forceinline__ half_t cvt<half_t>(double v) { return d2h(v); }
+// fp64 -> bfloat16 bits, correctly rounded (round to nearest even), from the
+// fp64 bit pattern: 8 significant bits for normals (exponent >= -126), fewer
+// for subnormals (down to 2^-133)
+__host__ __device__ inline uint16_t d2bf_bits(double d) {
+  uint64_t u;
+  __builtin_memcpy(&u, &d, 8);
+  const uint16_t sign = (uint16_t)((u >> 48) & 0x8000u);
+  const uint64_t mag = u & 0x7fffffffffffffffULL;
+  if (mag >= 0x7ff0000000000000ULL)  // inf / nan
+    return sign | 0x7f80u | (mag > 0x7ff0000000000000ULL ? 0x0040u : 0u);
+  if ((mag >> 52) == 0) return sign;  // fp64 subnormals: below half the smallest bf16
+  const int e = (int)(mag >> 52) - 1023;
+  if (e > 127) return sign | 0x7f80u;
+  if (e < -134) return sign;
+  const uint64_t m = (mag & 0xfffffffffffffULL) | (1ULL << 52);
+  const int drop = e >= -126 ? 45 : 45 + (-126 - e);
+  uint64_t q = m >> drop;
+  const uint64_t rest = m & ((1ULL << drop) - 1), half = 1ULL << (drop - 1);
+  q += (rest > half || (rest == half && (q & 1))) ? 1 : 0;
+  if (e >= -126) {
+    uint32_t ex = (uint32_t)(e + 127);
+    if (q >> 8) {  // mantissa overflow into the next binade
+      q >>= 1;
+      ++ex;
+    }
+    if (ex >= 255) return sign | 0x7f80u;
+    return (uint16_t)(sign | (ex << 7) | (q & 0x7fu));
+  }
+  return (uint16_t)(sign | q);  // q == 0x80 is the smallest normal
+}
+// bfloat16: fp32 -> bf16 round to nearest even from the fp32 bit pattern;
+// fp64 -> bf16 correctly rounded from the fp64 bits (no double rounding)
+template <> __device__ __forceinline__ bf16_t cvt<bf16_t>(float v) {
+  uint32_t u = __builtin_bit_cast(uint32_t, fp32_rounded(v));
+  uint16_t b;
+  if ((u & 0x7fffffffu) > 0x7f800000u)
+    b = (uint16_t)((u >> 16) | 0x0040u);  // nan stays nan
+  else
+    b = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  return __builtin_bit_cast(bf16_t, b);
+}
+template <> __device__ __forceinline__ bf16_t cvt<bf16_t>(double v) { return __builtin_bit_cast(bf16_t, d2bf_bits(v)); }
 
 // -------------------------------------------------------------- loads ----
 // K consecutive elements starting at an address aligned to K elements.

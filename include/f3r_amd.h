@@ -32,8 +32,9 @@ enum {
 const char *f3r_strerror(int status);
 const char *f3r_last_error(void); /* detail of the last error on this thread */
 
-/* element types: the precisions of kmm::vector<T, device> */
-enum { F3R_F64 = 0, F3R_F32 = 1, F3R_F16 = 2, F3R_I32 = 3 };
+/* element types: the precisions of kmm::vector<T, device>, and bfloat16
+ * (8-bit exponent, 8 significant bits; fp32 arithmetic like fp16) */
+enum { F3R_F64 = 0, F3R_F32 = 1, F3R_F16 = 2, F3R_I32 = 3, F3R_BF16 = 4 };
 
 typedef struct f3r_ctx_s *f3r_ctx;
 typedef struct f3r_vec_s *f3r_vec;

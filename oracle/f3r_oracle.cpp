@@ -74,11 +74,59 @@ uint16_t d2h_bits(double d) {
 }
 inline h16 d2h(double d) { return h16{d2h_bits(d)}; }
 
+// ------------------------------------------------------------ bfloat16 ----
+// 1 sign, 8 exponent, 7 stored mantissa bits: the top half of an fp32.
+struct b16 {
+  uint16_t b;
+};
+inline float b2f(b16 h) {
+  uint32_t u = (uint32_t)h.b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// fp32 -> bf16, round to nearest even: the value lies between the two bf16
+// numbers whose bits are the top 16 of u and that plus one; the discarded
+// low half decides, ties to the even one
+inline b16 f2b(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return b16{(uint16_t)((u >> 16) | 0x40u)};
+  uint32_t hi = u >> 16, lo = u & 0xffffu;
+  if (lo > 0x8000u || (lo == 0x8000u && (hi & 1u))) ++hi;  // carries into the exponent as it should
+  return b16{(uint16_t)hi};
+}
+// fp64 -> bf16, correctly rounded: from the fp64 value's position between
+// consecutive bf16 numbers (exponent e >= -126: spacing 2^(e-7); below:
+// subnormal spacing 2^-133)
+uint16_t d2b_bits(double d) {
+  if (std::isnan(d)) return std::signbit(d) ? 0xffc0u : 0x7fc0u;
+  const uint16_t sign = std::signbit(d) ? 0x8000u : 0u;
+  double a = std::fabs(d);
+  if (a == 0.0) return sign;
+  int e;
+  std::frexp(a, &e);  // a = f * 2^e, f in [0.5, 1): unbiased exponent e - 1
+  const int ue = e - 1;
+  const int q = ue >= -126 ? ue - 7 : -133;  // log2 of the spacing at a
+  const double scaled = std::ldexp(a, -q);    // exact: a power-of-two scaling
+  double k = std::floor(scaled);
+  const double r = scaled - k;                // exact
+  if (r > 0.5 || (r == 0.5 && std::fmod(k, 2.0) != 0.0)) k += 1.0;
+  const double v = std::ldexp(k, q);
+  if (v >= 0x1p128) return sign | 0x7f80u;   // beyond the largest finite bf16
+  float fv = (float)v;                        // exact: v has at most 8 significant bits
+  uint32_t u;
+  std::memcpy(&u, &fv, 4);
+  return (uint16_t)(sign | (u >> 16));
+}
+inline b16 d2b(double d) { return b16{d2b_bits(d)}; }
+
 // ------------------------------------------------------------ traits ------
 // load: value as it enters arithmetic; cvt<T>: rounding to storage type T.
 inline double ld(double x) { return x; }
 inline float ld(float x) { return x; }
 inline float ld(h16 x) { return h2f(x); }
+inline float ld(b16 x) { return b2f(x); }
 
 template <class T> struct cv;
 template <> struct cv<double> {
@@ -92,6 +140,10 @@ template <> struct cv<float> {
 template <> struct cv<h16> {
   static h16 from(double x) { return d2h(x); }
   static h16 from(float x) { return f2h(x); }
+};
+template <> struct cv<b16> {
+  static b16 from(double x) { return d2b(x); }
+  static b16 from(float x) { return f2b(x); }
 };
 template <class T, class S> inline T cvt(S x) { return cv<T>::from(x); }
 
@@ -1124,6 +1176,9 @@ extern "C" {
 uint16_t orc_f2h(float f) { return f2h(f).b; }
 float orc_h2f(uint16_t h) { return h2f(h16{h}); }
 uint16_t orc_d2h(double d) { return d2h_bits(d); }
+uint16_t orc_d2b(double d) { return d2b_bits(d); }
+uint16_t orc_f2b(float f) { return f2b(f).b; }
+float orc_b2f(uint16_t b) { return b2f(b16{b}); }
 void orc_d2h_array(int64_t n, const double *in, uint16_t *out) {
   for (int64_t i = 0; i < n; ++i) out[i] = d2h_bits(in[i]);
 }
@@ -1227,6 +1282,8 @@ void orc_spmv(int32_t n, const int32_t *rptr, const int32_t *idx, const double *
   ORC_CASE(2, 1, 1, 1, h16, float, float, float)
   ORC_CASE(2, 2, 2, 1, h16, h16, h16, float)
   ORC_CASE(2, 2, 1, 1, h16, h16, float, float)
+  ORC_CASE(3, 3, 3, 1, b16, b16, b16, float)
+  ORC_CASE(3, 1, 1, 1, b16, float, float, float)
 #undef ORC_CASE
   if (val_prec == 2 && x_prec == 2 && y_prec == 2 && acc_prec == 2) {
     CSRv<h16> Av(A);

@@ -37,6 +37,11 @@ extern "C" {
 uint16_t orc_f2h(float f);
 float orc_h2f(uint16_t h);
 uint16_t orc_d2h(double d);
+/* bfloat16 (precision code 3 of orc_spmv): fp64 and fp32 to bf16 correctly
+ * rounded (nearest even), bf16 to fp32 exactly */
+uint16_t orc_d2b(double d);
+uint16_t orc_f2b(float f);
+float orc_b2f(uint16_t b);
 void orc_d2h_array(int64_t n, const double *in, uint16_t *out);
 void orc_f2h_array(int64_t n, const float *in, uint16_t *out);
 void orc_h2f_array(int64_t n, const uint16_t *in, float *out);

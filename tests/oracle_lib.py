@@ -12,8 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
-F64, F32, F16 = 0, 1, 2
-_NP = {F64: np.float64, F32: np.float32, F16: np.float16}
+F64, F32, F16, BF16 = 0, 1, 2, 3
+_NP = {F64: np.float64, F32: np.float32, F16: np.float16, BF16: np.uint16}  # bf16 as its bits
 
 
 class Opts(ctypes.Structure):
@@ -51,6 +51,12 @@ def lib():
         L.orc_f2h.restype = ctypes.c_uint16
         L.orc_h2f.restype = ctypes.c_float
         L.orc_d2h.restype = ctypes.c_uint16
+        L.orc_d2b.restype = ctypes.c_uint16
+        L.orc_d2b.argtypes = [ctypes.c_double]
+        L.orc_f2b.restype = ctypes.c_uint16
+        L.orc_f2b.argtypes = [ctypes.c_float]
+        L.orc_b2f.restype = ctypes.c_float
+        L.orc_b2f.argtypes = [ctypes.c_uint16]
         L.orc_sdainv.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_ilu0.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,

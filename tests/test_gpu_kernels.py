@@ -342,3 +342,62 @@ def test_sell_lds_rejects_offsets(ctx):
     A = F.CSR(*PB.stencil7(16, 16, 1))
     with pytest.raises(F.F3RError):
         F.SELL(ctx, A, "f16", C=64, sigma=1, K=4, offsets=True, lds=True)
+
+
+# ----------------------------------------------------------- bfloat16 ----
+def test_bf16_vector_casts_and_blas1(ctx):
+    # device conversions into bf16 (from fp64 correctly rounded, from fp32 and
+    # fp16 nearest even) and back, against the oracle's pinned conversions;
+    # axpy, scale and dot in fp32 arithmetic
+    import f3r_amd as F
+    L = O.lib()
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-1, 1, 5000) * 2.0 ** rng.integers(-135, 127, 5000),
+                        [0.0, -0.0, 3.3961e38, 3.4e38, 2.0 ** -134, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8]])
+    v64 = F.vector.from_numpy(ctx, x)
+    vb = F.vector(ctx, "bf16", x.size).copy(v64)
+    want = np.array([L.orc_d2b(float(t)) for t in x], np.uint16)
+    assert np.array_equal(vb.download(), want)
+    xf = x.astype(np.float32)
+    vb.copy(F.vector.from_numpy(ctx, xf))
+    assert np.array_equal(vb.download(), np.array([L.orc_f2b(float(t)) for t in xf], np.uint16))
+    back = F.vector(ctx, "f32", x.size).copy(vb)
+    assert np.array_equal(back.download(), np.array([L.orc_b2f(int(b)) for b in vb.download()], np.float32))
+    # BLAS-1 on bf16 data: fp32 arithmetic, one rounding to bf16 per element
+    a = rng.uniform(-1, 1, 4096)
+    b = rng.uniform(-1, 1, 4096)
+    va, vbb = F.vector.from_numpy(ctx, a, "bf16"), F.vector.from_numpy(ctx, b, "bf16")
+    af = np.array([L.orc_b2f(int(t)) for t in va.download()], np.float32)
+    bf = np.array([L.orc_b2f(int(t)) for t in vbb.download()], np.float32)
+    d = F.dot(va, vbb)
+    assert abs(d - float(af.astype(np.float64) @ bf.astype(np.float64))) <= 4096 * 2.0 ** -24 * float(np.abs(af) @ np.abs(bf))
+    vbb.axpy(0.5, va)
+    exp = np.array([L.orc_f2b(float(np.float32(np.float64(0.5) * ai + bi))) for ai, bi in zip(af, bf)], np.uint16)
+    # fma(0.5, a, b) in fp32 then bf16: 0.5 * a is exact, so fma = fp32(a/2 + b)
+    assert np.array_equal(vbb.download(), exp)
+
+
+@pytest.mark.parametrize("fmt", ["sell-int32", "sell-offsets", "csr"])
+def test_bf16_spmv_bitexact(ctx, fmt):
+    import f3r_amd as F
+    for name, A in _mats():
+        if fmt == "sell-offsets" and name == "ragged":
+            continue
+        x = np.random.default_rng(4).uniform(-1, 1, A.n)
+        b = np.random.default_rng(5).uniform(-1, 1, A.n)
+        for xd, yd, xp, yp in (("bf16", "bf16", 3, 3), ("f32", "f32", 1, 1)):
+            print(f"bf16 spmv {name} {fmt} x {xd}", flush=True)
+            if fmt == "csr":
+                M = F.CSRDevice(ctx, A, "bf16", tpr=1)
+            else:
+                M = F.SELL(ctx, A, "bf16", C=64, sigma=1, K=4, offsets=fmt == "sell-offsets")
+            xv = F.vector.from_numpy(ctx, x, xd)
+            xh = xv.download()
+            yv = F.vector(ctx, yd, A.n)
+            M.operate(xv, yv)
+            ref = O.spmv(A.n, A.rptr, A.idx, A.val, xh, 3, xp, yp, 1)
+            assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, fmt, xd)
+            bv = F.vector.from_numpy(ctx, b, yd)
+            M.compute_residual(bv, xv, yv)
+            ref = O.spmv(A.n, A.rptr, A.idx, A.val, xh, 3, xp, yp, 1, b=bv.download())
+            assert np.array_equal(yv.download().view(np.uint8), ref.view(np.uint8)), (name, fmt, "resid")

@@ -360,3 +360,70 @@ def test_f3r_against_live_reference_build(tmp_path):
     xr = PB.read_vec_bin(tmp_path / "x.bin", np.float64)
     x, r, _ = O.f3r_solve(n, rp, ix, v, b, variant=0, ref_arith=1, nthreads=1)
     assert np.array_equal(x, xr)
+
+
+# -------------------------------------------------------------- bfloat16 ----
+def _bf16_value(bits):
+    return np.frombuffer((np.asarray(bits, np.uint32) << 16).astype(np.uint32).tobytes(), np.float32).astype(np.float64)
+
+
+def test_bf16_roundtrip_exhaustive():
+    L = O.lib()
+    for b in range(1 << 16):
+        f = L.orc_b2f(b)
+        assert f == _bf16_value(b)[0] or (np.isnan(f) and np.isnan(_bf16_value(b)[0]))
+        if not np.isnan(f):
+            assert L.orc_f2b(f) == b, hex(b)
+
+
+def test_bf16_from_fp32_against_neighbours():
+    # fp32 -> bf16 nearest even, from the two bf16 numbers around each value
+    # (differences of fp32 numbers are exact in fp64)
+    L = O.lib()
+    u = np.random.default_rng(5).integers(0, 1 << 32, 20000, dtype=np.uint64).astype(np.uint32)
+    u = u[(u & 0x7f800000) != 0x7f800000]  # finite
+    u = np.concatenate([u, (u & 0xffff0000) | 0x8000])  # exact ties
+    f = np.frombuffer(u.tobytes(), np.float32)
+    lo = u >> 16
+    vlo, vhi = _bf16_value(lo), _bf16_value(lo + 1)
+    vhi = np.where(((lo + 1) & 0x7fff) == 0x7f80, 2.0 ** 128, vhi)  # rounding up past the largest: inf
+    x = np.abs(f.astype(np.float64))
+    dl, dh = x - np.abs(vlo), np.abs(vhi) - x
+    want = np.where((dh < dl) | ((dh == dl) & (lo % 2 == 1)), lo + 1, lo).astype(np.uint16)
+    got = np.array([L.orc_f2b(float(t)) for t in f], np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_bf16_from_fp64_exact_rounding():
+    # fp64 -> bf16 against exact rational arithmetic: normal and subnormal
+    # ranges, exact ties and their fp64 neighbours, overflow
+    from fractions import Fraction
+    L = O.lib()
+    rng = np.random.default_rng(6)
+    xs = list(rng.uniform(-1, 1, 500) * 2.0 ** rng.integers(-140, 128, 500))
+    for b in rng.integers(1, 0x7f7f, 300):  # ties between bf16 neighbours, and next to them
+        m = (float(_bf16_value(b)[0]) + float(_bf16_value(b + 1)[0])) / 2
+        xs += [m, np.nextafter(m, 0.0), np.nextafter(m, np.inf), -m]
+    xs += [3.4e38, 3.3961e38, 2.0 ** -134, 2.0 ** -133 * 1.5, 2.0 ** -135, 0.0, -0.0]
+    grid = {}  # every finite non-negative bf16 value, by value
+    for b in range(0x7f80):
+        grid[b] = Fraction(float(_bf16_value(b)[0]))
+    vals = sorted((v, b) for b, v in grid.items())
+    import bisect
+    keys = [v for v, _ in vals]
+    for x in xs:
+        a = Fraction(abs(x))
+        i = bisect.bisect_left(keys, a)
+        if i >= len(keys):
+            want = vals[-1][1]  # above the largest finite: it, or inf (below)
+        elif keys[i] == a:
+            want = vals[i][1]
+        else:
+            lo, hi = vals[i - 1], vals[i]
+            dl, dh = a - lo[0], hi[0] - a
+            want = hi[1] if dh < dl or (dh == dl and hi[1] % 2 == 0) else lo[1]
+        if a >= Fraction(2) ** 128 - Fraction(2) ** 119:  # past the midpoint above the largest
+            want = 0x7f80
+        if np.signbit(x):
+            want |= 0x8000
+        assert L.orc_d2b(float(x)) == want, (x, hex(L.orc_d2b(float(x))), hex(want))
