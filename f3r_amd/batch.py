@@ -153,11 +153,12 @@ def tune_lockstep(ctx, fam, log=None):
     return {"C": L["C"], "K": L["K"], "index": L["index"]}
 
 
-def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False):
+def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False, use_graph=True):
     """The same family solved in lockstep (f3r_batch): one block-diagonal
     layout of all systems x right-hand sides, every launch covering all of
     them; per-system SDAINV factorised on the device.  tune: the layout
-    (C, K, index format) autotuned first (tune_lockstep)."""
+    (C, K, index format) autotuned first (tune_lockstep).  use_graph: the
+    level-2 body replayed from HIP graphs."""
     from . import Batch
     ctx = Context(device)
     fam = family(seed, count)
@@ -172,7 +173,7 @@ def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False):
             Ps.append(P)
             bs.append(uniform(A.n, -1.0, 1.0, seed=(seed << 20) + 16 * i + r))
             meta.append((i, npts, nx, delta, r))
-    B = Batch(ctx, As, Ps, **lay)
+    B = Batch(ctx, As, Ps, use_graph=use_graph, **lay)
     rows, segs = B.info()
     setup_s = time.time() - t0
     if log:
@@ -222,6 +223,7 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="no layout autotuning")
     ap.add_argument("--mode", choices=["lockstep", "streams"], default="lockstep",
                     help="lockstep: all solves in shared launches (f3r_batch); streams: one solver per system")
+    ap.add_argument("--no-graph", action="store_true", help="lockstep: launch the level-2 body eagerly")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     t0 = time.time()
@@ -230,7 +232,7 @@ def main(argv=None):
         print(f"[batch {time.time() - t0:6.1f}s] {m}", flush=True)
 
     if a.mode == "lockstep":
-        out = run_lockstep(a.count, a.rhs, a.seed, log=log, tune=not a.no_tune)
+        out = run_lockstep(a.count, a.rhs, a.seed, log=log, tune=not a.no_tune, use_graph=not a.no_graph)
     else:
         out = run(a.count, a.rhs, a.streams, a.seed, not a.no_tune, log=log)
     if a.out:

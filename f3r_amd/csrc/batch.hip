@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <map>
 #include <numeric>
 
 #include "common.hpp"
@@ -721,6 +722,16 @@ struct BatchSolver {
   BLevel<float> L2, L3;
   DevPtr hin, hout, hout2, hr, tmp, tt, t2;
   int64_t cnt = 0, calls = 0;
+  std::map<uint64_t, hipGraphExec_t> graphs;  // level-2 body per probe pattern
+  std::vector<hipGraph_t> graph_src;
+
+  BatchSolver() = default;
+  BatchSolver(const BatchSolver &) = delete;
+  BatchSolver &operator=(const BatchSolver &) = delete;
+  ~BatchSolver() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto &g : graph_src) (void)hipGraphDestroy(g);
+  }
 
   hipStream_t st() const { return ctx->stream; }
   int32_t *bs() const { return bseg->as<int32_t>(); }
@@ -918,14 +929,63 @@ struct BatchSolver {
                          (const float *)nullptr);
   }
 
-  // FGMRES(m2) on the fp32 values, fp32 basis, normalize = false
-  void level2(const double *in, double *out) {
+  // the level-2 Arnoldi loop: every buffer it touches is the solver's own, so
+  // the same launches serve every call with the same probe positions
+  void level2_body() {
     BLevel<float> &L = L2;
-    hipLaunchKernelGGL((k_b_cast<double, float>), dim3(G), dim3(kBlock), 0, st(), in, L.Vc(N, 0));
     for (int j = 0; j < L.m; ++j) {
       level3(L.Vc(N, j), L.Zc(N, j));
       spmv<float, float, float, float>(A32->as<float>(), false, (const float *)L.Zc(N, j), L.Vc(N, j + 1), nullptr);
       arnoldi(L, j, 1, nullptr, (const float *)L.Vc(N, 0), false);
+    }
+  }
+
+  // probe positions of the next level-2 body as a bit mask (bit q: the q-th
+  // level-4 call probes); UINT64_MAX when not representable
+  uint64_t next_pattern() const {
+    const int64_t calls_per = (int64_t)p.m2 * p.m3;
+    if (calls_per > 63) return UINT64_MAX;
+    uint64_t mask = 0;
+    for (int64_t q = 0; q < calls_per; ++q)
+      if (((cnt + q + 1) % p.c) == 0) mask |= 1ULL << q;
+    return mask | (1ULL << 63);
+  }
+
+  // FGMRES(m2) on the fp32 values, fp32 basis, normalize = false.  With
+  // use_graph the body is captured once per probe pattern (at most 33) and
+  // replayed: it holds no host synchronisation, memset or host copy, and the
+  // Richardson weights and probe results stay in device memory between calls
+  void level2(const double *in, double *out) {
+    BLevel<float> &L = L2;
+    hipLaunchKernelGGL((k_b_cast<double, float>), dim3(G), dim3(kBlock), 0, st(), in, L.Vc(N, 0));
+    const uint64_t key = p.use_graph ? next_pattern() : UINT64_MAX;
+    if (key == UINT64_MAX || (graphs.size() > 32 && graphs.find(key) == graphs.end())) {
+      level2_body();
+    } else {
+      auto it = graphs.find(key);
+      if (it == graphs.end()) {
+        const int64_t cnt0 = cnt, calls0 = calls;
+        hipGraph_t gr;
+        F3R_HIP(hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+        try {
+          level2_body();
+        } catch (...) {
+          (void)hipStreamEndCapture(st(), &gr);
+          throw;
+        }
+        F3R_HIP(hipStreamEndCapture(st(), &gr));
+        hipGraphExec_t ex;
+        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) (void)hipGraphDestroy(gr);
+        F3R_HIP(e);
+        graph_src.push_back(gr);
+        it = graphs.emplace(key, ex).first;
+        cnt = cnt0;  // the capture only recorded the calls
+        calls = calls0;
+      }
+      F3R_HIP(hipGraphLaunch(it->second, st()));
+      cnt += (int64_t)p.m2 * p.m3;
+      calls += (int64_t)p.m2 * p.m3;
     }
     hipLaunchKernelGGL((k_b_combine<float, float, double>), dim3(G), dim3(kBlock), 0, st(),
                        (const float *)L.Zc(N, 0), N, (const float *)L.H->as<float>(), L.hs(), L.m + 1,

@@ -233,3 +233,40 @@ def test_lockstep_rejects_bad_parameters():
     assert B.info()[1] == 2
     del B
     ctx.close()
+
+
+@pytest.mark.parametrize("c", [64, 8])
+def test_lockstep_graph_replay_equals_eager(c):
+    """the level-2 body captured per probe pattern and replayed gives the
+    eager launches' bits, over two solves in a row (the second replays the
+    graphs the first captured, with host reads and copies between replays);
+    c = 8 puts probe calls inside the captured body"""
+    import numpy as np
+    import f3r_amd as F
+    ctx = F.Context(0)
+    As, Ps, bs = [], [], []
+    for (i, npts, nx, delta, A) in _family(F, 3, seed=11):
+        P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+        for r in range(2):
+            As.append(A)
+            Ps.append(P)
+            bs.append(F.uniform(A.n, -1.0, 1.0, seed=31 * i + r))
+    bv = [F.vector.from_numpy(ctx, b) for b in bs]
+    out = {}
+    for g in (False, True):
+        B = F.Batch(ctx, As, Ps, c=c, use_graph=g)
+        runs = []
+        for rep in range(2):
+            xv = [F.vector(ctx, "f64", A.n).fill(0.0) for A in As]
+            res = B.solve(bv, xv)
+            runs.append(([r.res_iter for r in res], [r.res_nrm2 for r in res], [x.download() for x in xv]))
+        out[g] = runs
+        print(f"c={c} graph={g}: outer steps {runs[0][0]}", flush=True)
+    for rep in range(2):
+        it0, n0, x0 = out[False][rep]
+        it1, n1, x1 = out[True][rep]
+        assert it0 == it1 and n0 == n1
+        assert all(np.array_equal(a, b) for a, b in zip(x0, x1))
+    for s, (A, b) in enumerate(zip(As, bs)):
+        assert _true_res(A, out[True][1][2][s], b) <= 1e-8
+    ctx.close()
