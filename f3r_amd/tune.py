@@ -114,10 +114,11 @@ def main(argv=None):
     ctx = Context(0)
     A = _problem(a.problem)
     best, table = autotune(ctx, A, a.dtype, reps=a.reps, **({"tprs": ()} if a.no_csr else {}))
+    def us(e):  # the table's times in microseconds
+        return {("us" if k == "ms" else k): (round(v * 1e3, 2) if k == "ms" else v) for k, v in e.items()}
     for e in table:
-        print(json.dumps({k: (round(v * 1e3, 2) if k == "ms" else v) for k, v in e.items()}), flush=True)
-    print(json.dumps({"problem": a.problem, "n": A.n, "nnz": A.nnz, "dtype": a.dtype,
-                      "best": {k: (round(v * 1e3, 2) if k == "ms" else v) for k, v in best.items()}}))
+        print(json.dumps(us(e)), flush=True)
+    print(json.dumps({"problem": a.problem, "n": A.n, "nnz": A.nnz, "dtype": a.dtype, "best": us(best)}))
     return 0
 
 
