@@ -1039,7 +1039,7 @@ HostCSR host_csr_from(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *
   h.idx.assign(col, col + nnz);
   h.val.assign(val, val + nnz);
   for (int32_t i = 0; i < nrows; ++i) {
-    F3R_CHECK(h.rptr[i] <= h.rptr[i + 1], F3R_ERR_ARG, "CSR: rptr is not monotone");
+    F3R_CHECK(h.rptr[i] <= h.rptr[i + 1] && h.rptr[i + 1] <= nnz, F3R_ERR_ARG, "CSR: rptr is not monotone");
     for (int32_t k = h.rptr[i]; k < h.rptr[i + 1]; ++k) {
       F3R_CHECK(h.idx[k] >= 0 && h.idx[k] < ncols, F3R_ERR_ARG, "CSR: column index out of range");
       F3R_CHECK(k == h.rptr[i] || h.idx[k - 1] < h.idx[k], F3R_ERR_ARG,

@@ -1192,10 +1192,13 @@ struct Solver {
     launch_sell<double, double, double>(sh->K, a, A64->as<double>(), x_int->as<double>(), er, st(), spmv_grid);
     hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V, (const double *)inv1->as<double>());
     double e = read_scalar(ctx.get(), stat);
-    if (record) hist.push_back(e / nrm_b);
+    // |e| / ||b||; a zero right-hand side is met only by a zero residual (its
+    // solution from x = 0 is x = 0, with no step)
+    auto rel = [&](double v) { return nrm_b > 0 ? v / nrm_b : (v == 0 ? 0.0 : HUGE_VAL); };
+    if (record) hist.push_back(rel(e));
     // the caller's cond(i, |e|) of Solver::solve (sparse/models.hpp:191-267),
     // called as FGMRES::kernel calls it (i = 0 first, then the step count)
-    auto stop = [&](int i) { return cond_fn ? cond_fn(cond_user, i, e) != 0 : e / nrm_b < p.eps; };
+    auto stop = [&](int i) { return cond_fn ? cond_fn(cond_user, i, e) != 0 : rel(e) < p.eps; };
     if (stop(0)) return {true, 0, e};
     int j = 0;
     bool flag = false;
@@ -1215,7 +1218,7 @@ struct Solver {
       hipLaunchKernelGGL(k_vscale<double>, dim3(grid()), dim3(kBlock), 0, st(), nn, V + (int64_t)(j + 1) * nn,
                          (const double *)inv1->as<double>());
       e = read_scalar(ctx.get(), stat);
-      if (record) hist.push_back(e / nrm_b);
+      if (record) hist.push_back(rel(e));
       if (stop(j + 1)) {
         ++j;
         flag = true;

@@ -51,6 +51,34 @@ def test_error_paths_without_gpu():
     assert lib.f3r_prec_apply(P.handle, None, None) == -1
 
 
+@pytest.mark.parametrize("rp,ix,msg", [
+    ([0, 5, 2], [0, 1], b"monotone"),       # a row pointer beyond nnz
+    ([0, -1, 2], [0, 1], b"monotone"),      # a negative row pointer
+    ([0, 1, 2], [0, 2], b"out of range"),   # a column outside the matrix
+    ([0, 1, 2], [0, -1], b"out of range"),
+    ([0, 2, 2], [1, 1], b"sorted"),         # a repeated column
+    ([0, 1, 3], [0, 1], b"rptr[n]"),        # rptr[n] != nnz
+])
+def test_malformed_csr_rejected(rp, ix, msg):
+    # every malformed CSR is refused before any array is read out of bounds
+    from f3r_amd._capi import lib
+    rp = np.array(rp, np.int32)
+    ix = np.array(ix, np.int32)
+    v = np.ones(len(ix))
+    h = ctypes.c_void_p()
+    st = lib.f3r_mat_create_csr(None, 2, 2, len(ix), rp.ctypes.data, ix.ctypes.data, v.ctypes.data, ctypes.byref(h))
+    assert st == -1 and msg in lib.f3r_last_error(), lib.f3r_last_error()
+
+
+def test_empty_csr_on_the_host():
+    # a 0 x 0 matrix is a valid CSR; its host set-up is empty as well
+    import f3r_amd as F
+    A = F.CSR(0, np.array([0], np.int32), np.array([], np.int32), np.array([]))
+    assert A.n == 0 and A.nnz == 0
+    F.algorithm.scaling(A)
+    F.SDAINV(A, 0.1, 1.0)
+
+
 def test_generator_matches_reference_hpcg(golden):
     import f3r_amd as F
     import hashlib

@@ -247,6 +247,11 @@ def test_solve_edge_cases(ctx):
     xv.fill(0.0)
     r1 = S1.solve(bv, xv)
     assert not r1.is_solved and r1.res_iter == 1
+    # a zero right-hand side from x = 0: solved with no step, x stays 0
+    zv = F.vector(ctx, "f64", A.n).fill(0.0)
+    xv.fill(0.0)
+    rz = S.solve(zv, xv)
+    assert rz.is_solved and rz.res_iter == 0 and rz.nrm_b == 0.0 and not np.any(xv.download())
 
 
 def test_c2_shape_64cube(ctx):

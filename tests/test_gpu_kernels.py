@@ -139,6 +139,23 @@ def test_blas1(ctx):
     assert np.array_equal(w.download(), np.arange(3.0, 7.0))
 
 
+def test_empty_vectors_and_matrix(ctx):
+    # zero-length vectors: every BLAS-1 call is a no-op and a dot is 0; the
+    # solver refuses an empty matrix with an error status
+    import f3r_amd as F
+    for dt in ("f64", "f32", "f16", "bf16"):
+        print(f"empty vectors {dt}", flush=True)
+        a = F.vector(ctx, dt, 0).fill(1.0)
+        b = F.vector(ctx, dt, 0).fill(2.0)
+        a.axpy(3.0, b)
+        assert F.dot(a, b) == 0.0
+        assert a.download().size == 0
+    A = F.CSR(0, np.array([0], np.int32), np.array([], np.int32), np.array([]))
+    P = F.SDAINV(A, 0.1, 1.0)
+    with pytest.raises(F.F3RError):
+        F.F3R(ctx, A, P)
+
+
 def test_sdainv_apply_bitexact(ctx):
     import f3r_amd as F
     A = F.CSR.convdiff7(12, sigma=2.0, corr=2.0, peclet=0.0, seed=3)
