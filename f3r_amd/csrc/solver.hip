@@ -297,8 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update_strided(int64_t n, T *__r
   }
 }
 
-// <w, V_k> for k < ncols, V contiguous; blockIdx.y selects 16 columns
-template <class T>
+// <w, V_k> for k < ncols, V contiguous; blockIdx.y selects 16 columns.
+// VEC: every column 4-element aligned (n % 4 == 0); each lane then takes 4
+// consecutive rows per step with 32-byte (fp64) loads
+template <class T, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_mdot_strided(int64_t n, const T *__restrict__ w,
                                                          const T *__restrict__ V, int ncols, T *Hcol,
                                                          T *partial, unsigned *counters) {
@@ -308,11 +310,25 @@ __global__ __launch_bounds__(kBlock) void k_mdot_strided(int64_t n, const T *__r
   T v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const T wi = w[i];
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x, gstride = (int64_t)gridDim.x * kBlock;
+  if constexpr (VEC) {
+    for (int64_t q = gid; q < n / 4; q += gstride) {
+      const V4<T> wq = ld4(w + 4 * q);
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      if (k0 + k < ncols) v[k] = fma_<T>(wi, V[(int64_t)(k0 + k) * n + i], v[k]);
+      for (int k = 0; k < NV; ++k)
+        if (k0 + k < ncols) {
+          const V4<T> vq = ld4(V + (int64_t)(k0 + k) * n + 4 * q);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[k] = fma_<T>(wq.v[c], vq.v[c], v[k]);
+        }
+    }
+  } else {
+    for (int64_t i = gid; i < n; i += gstride) {
+      const T wi = w[i];
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        if (k0 + k < ncols) v[k] = fma_<T>(wi, V[(int64_t)(k0 + k) * n + i], v[k]);
+    }
   }
   const int ng = (gridDim.x + kGroup - 1) / kGroup;
   if (grid_sum<T, NV>(v, partial + (int64_t)blockIdx.y * NV * (gridDim.x + ng), counters + (int64_t)blockIdx.y * kCounterSlot, lds) &&
@@ -1208,9 +1224,9 @@ struct Solver {
                                           EpiStore<double, double>{V + (int64_t)(j + 1) * nn, nullptr}, st());
       const int ncols = j + 1;
       dim3 gm(red_grid, (ncols + 15) / 16);
-      hipLaunchKernelGGL(k_mdot_strided<double>, gm, dim3(kBlock), 0, st(), nn,
-                         (const double *)(V + (int64_t)(j + 1) * nn), (const double *)V, ncols,
-                         H + (int64_t)j * (m1 + 1), partial<double>(), counter(8));
+      auto mdot = nn % 4 == 0 && aligned4(V) ? k_mdot_strided<double, true> : k_mdot_strided<double, false>;
+      hipLaunchKernelGGL(mdot, gm, dim3(kBlock), 0, st(), nn, (const double *)(V + (int64_t)(j + 1) * nn),
+                         (const double *)V, ncols, H + (int64_t)j * (m1 + 1), partial<double>(), counter(8));
       hipLaunchKernelGGL(k_cgs_update_strided<double>, dim3(red_grid), dim3(kBlock), 0, st(), nn,
                          V + (int64_t)(j + 1) * nn, (const double *)V, ncols, H + (int64_t)j * (m1 + 1),
                          c1->as<double>(), s1->as<double>(), e1->as<double>(), j, inv1->as<double>(), stat,
