@@ -10,13 +10,16 @@
 //     kept (io.hpp:168-171);
 //   * an off-diagonal entry of a symmetric file is mirrored (io.hpp:172-177);
 //   * entries are placed by row in file order and each row is sorted by column
-//     (io.hpp:49-62).  Repeated (row, column) entries are kept, not summed; they
-//     stay in file order, where the reference's quicksort may swap them.
+//     (io.hpp:49-62).  A repeated (row, column) entry, which the reference
+//     keeps as two entries of one row, is refused (F3R_ERR_UNSUPPORTED): this
+//     library's CSR has one entry per position.
 // Where the reference calls exit(1) (unreadable header, a right-hand side that
 // is not "array ... general" with one column or not of the matrix's size),
 // these return F3R_ERR_ARG.  Additionally, entries outside the declared shape
-// and truncated files are errors, and "pattern" / "complex" files, which the
-// reference's real reader misparses, are F3R_ERR_UNSUPPORTED.
+// and truncated files are errors (so is a header that declares more entries
+// than the file has bytes for, before anything is allocated), and
+// "pattern" / "complex" files, which the reference's real reader misparses,
+// are F3R_ERR_UNSUPPORTED.
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
@@ -96,6 +99,8 @@ MMMatrix read_matrix(const char *path) {
   const bool sym = tok[4] == "symmetric";
   Cursor c{s.data() + skip_comments(s), s.data() + s.size(), path};
   const int64_t nr = c.index(0, INT32_MAX - 1), nc = c.index(0, INT32_MAX), nl = c.index(0, INT64_MAX / 4);
+  // an entry takes at least 6 bytes ("1 1 1\n")
+  F3R_CHECK(nl <= (int64_t)s.size() / 6, F3R_ERR_ARG, std::string("mm: truncated file ") + path);
   std::vector<int32_t> r, cl;
   std::vector<double> v;
   r.reserve(sym ? 2 * nl : nl);
@@ -137,6 +142,9 @@ MMMatrix read_matrix(const char *path) {
     for (int32_t k = M.rptr[i]; k < M.rptr[i + 1]; ++k) row.push_back({M.col[k], M.val[k]});
     std::stable_sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
     for (size_t q = 0; q < row.size(); ++q) {
+      F3R_CHECK(q == 0 || row[q].first != row[q - 1].first, F3R_ERR_UNSUPPORTED,
+                std::string("mm: a repeated entry (row ") + std::to_string(i + 1) + ", column " +
+                    std::to_string(row[q].first + 1) + ") in " + path);
       M.col[M.rptr[i] + q] = row[q].first;
       M.val[M.rptr[i] + q] = row[q].second;
     }
@@ -152,6 +160,7 @@ std::vector<double> read_vector(const char *path) {
   Cursor c{s.data() + skip_comments(s), s.data() + s.size(), path};
   const int64_t nr = c.index(0, INT32_MAX), nc = c.index(0, INT32_MAX);
   F3R_CHECK(nc == 1, F3R_ERR_ARG, std::string("mm: a right-hand side has one column: ") + path);
+  F3R_CHECK(nr <= (int64_t)s.size() / 2, F3R_ERR_ARG, std::string("mm: truncated file ") + path);  // "1\n" each
   std::vector<double> b(nr);
   for (auto &x : b) x = c.next();
   return b;

@@ -60,6 +60,24 @@ def test_reader_errors():
     assert lib.f3r_mm_read(None, None, ctypes.byref(h)) == -1
 
 
+def test_reader_refuses_repeats_and_overlong_headers(tmp_path):
+    # a repeated (row, column) entry, and headers that declare more entries or
+    # rows than the file holds, are errors before anything large is allocated
+    cases = {
+        "repeat.mtx": ("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1.0\n2 2 2.0\n1 1 3.0\n",
+                       "read_matrix", -4),
+        "long.mtx": ("%%MatrixMarket matrix coordinate real general\n2 2 1000000000000\n1 1 1.0\n",
+                     "read_matrix", -1),
+        "long_b.mtx": ("%%MatrixMarket matrix array real general\n2000000000 1\n1.0\n", "read_vector", -1),
+    }
+    for name, (text, fn, st) in cases.items():
+        path = tmp_path / name
+        path.write_text(text)
+        with pytest.raises(F.F3RError) as e:
+            getattr(F.io.mm, fn)(str(path))
+        assert e.value.status == st, (name, e.value.status)
+
+
 def test_rhs_path_and_fallback():
     buf = ctypes.create_string_buffer(64)
     assert lib.f3r_mm_rhs_path(b"../matrix/hpcg_5_5_5.mtx", buf, 64) == 0
