@@ -709,8 +709,19 @@ class Restarted:
         if cond is None:
             st = lib.f3r_solve(self.handle, b.handle, x.handle, ctypes.byref(r))
         else:
-            cb = _capi.COND_FN(lambda user, i, e: 1 if cond(int(i), float(e)) else 0)
+            raised = []
+
+            def call(user, i, e):
+                # an exception must not cross the C ABI: keep it, stop the solve
+                try:
+                    return 1 if cond(int(i), float(e)) else 0
+                except BaseException as exc:
+                    raised.append(exc)
+                    return 1
+            cb = _capi.COND_FN(call)
             st = lib.f3r_solve_cond(self.handle, b.handle, x.handle, cb, None, ctypes.byref(r))
+            if raised:
+                raise raised[0]
         if st != 0 and self._op is not None and self._op.error is not None:
             err, self._op.error = self._op.error, None
             raise err

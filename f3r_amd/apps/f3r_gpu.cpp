@@ -100,7 +100,11 @@ int main(int argc, char **argv) {
     std::vector<char> bpath(src.size() + 8);
     CK(f3r_mm_rhs_path(src.c_str(), bpath.data(), (int64_t)bpath.size()));
     int64_t nb = 0;
-    if (f3r_mm_read_vector(bpath.data(), 0, nullptr, &nb) == F3R_OK) {
+    // a _b.mtx that exists must be read (io.hpp:108-126); only a missing one
+    // falls back to the generated right-hand side
+    if (std::FILE *bf = std::fopen(bpath.data(), "rb")) {
+      std::fclose(bf);
+      CK(f3r_mm_read_vector(bpath.data(), 0, nullptr, &nb));
       b.resize(nb);
       CK(f3r_mm_read_vector(bpath.data(), nb, b.data(), &nb));
       if ((int32_t)b.size() != A.n) {

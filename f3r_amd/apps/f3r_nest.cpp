@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -196,6 +197,19 @@ int main(int argc, char **argv) {
       auto two = S4.solve(b, x, [](int i, double) { return i >= 2; });
       report("the caller's stopping rule", two.is_solved && two.res_iter == 2,
              " (" + std::to_string(two.res_iter) + " outer steps)");
+      // 5. an exception thrown by the stopping rule stops the solve and
+      //    reaches the caller unchanged
+      bool caught = false;
+      try {
+        x.fill(0.0);
+        S4.solve(b, x, [](int i, double) -> bool {
+          if (i >= 1) throw std::runtime_error("stop rule failed");
+          return false;
+        });
+      } catch (const std::runtime_error &e) {
+        caught = std::string(e.what()) == "stop rule failed";
+      }
+      report("an exception in the stopping rule reaches the caller", caught, "");
     }
     return failed;
   } catch (const f3r::error &e) {

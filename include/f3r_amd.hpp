@@ -28,6 +28,8 @@
 #define F3R_AMD_HPP
 
 #include <cstdint>
+#include <cstdio>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -130,8 +132,11 @@ template <class T> class vector : public view {
     keep_ = std::shared_ptr<f3r_vec_s>(v, [](f3r_vec p) { f3r_vec_destroy(p); });
     v_ = v;
   }
-  // host data of T's precision (double or float)
-  void upload(const std::vector<T> &h) const { check(f3r_vec_upload(v_, h.data()), "vec_upload"); }
+  // host data of T's precision (double or float), exactly size() entries
+  void upload(const std::vector<T> &h) const {
+    if ((int64_t)h.size() != size()) throw error(F3R_ERR_ARG, "vector::upload: host data of another size");
+    check(f3r_vec_upload(v_, h.data()), "vec_upload");
+  }
   std::vector<T> download() const {
     std::vector<T> h(size());
     check(f3r_vec_download(v_, h.data()), "vec_download");
@@ -196,7 +201,11 @@ struct mm {  // sparse/matrix/io.hpp:104-217
     check(f3r_mm_rhs_path(path.c_str(), bp.data(), (int64_t)bp.size()), "mm_rhs_path");
     std::vector<double> b;
     int64_t nb = 0;
-    if (f3r_mm_read_vector(bp.data(), 0, nullptr, &nb) == F3R_OK) {
+    // an existing _b.mtx must be read; only a missing one falls back
+    std::FILE *bf = std::fopen(bp.data(), "rb");
+    if (bf) {
+      std::fclose(bf);
+      check(f3r_mm_read_vector(bp.data(), 0, nullptr, &nb), "mm_read_vector");
       b.resize(nb);
       check(f3r_mm_read_vector(bp.data(), nb, b.data(), &nb), "mm_read_vector");
       if (nb != A.n) throw error(F3R_ERR_ARG, "mm::read_system: the right-hand side's size differs from the matrix's");
@@ -443,10 +452,14 @@ template <class T> class Restarted {
           "nest_create");
     st_->s = std::shared_ptr<f3r_solver_s>(h, [](f3r_solver q) { f3r_solver_destroy(q); });
   }
-  // Solver::solve(b, x, cond) (sparse/models.hpp:191-267): cond(i, |e|)
+  // Solver::solve(b, x, cond) (sparse/models.hpp:191-267): cond(i, |e|).
+  // An exception from cond stops the solve and is rethrown here.
   solve_res_t solve(const view &b, const view &x, std::function<bool(int, double)> cond) const {
     f3r_result r{};
-    check(f3r_solve_cond(get(), b.get(), x.get(), &Restarted::trampoline, &cond, &r), "solve_cond");
+    Cond c{std::move(cond), nullptr};
+    const int st = f3r_solve_cond(get(), b.get(), x.get(), &Restarted::trampoline, &c, &r);
+    if (c.err) std::rethrow_exception(c.err);
+    check(st, "solve_cond");
     last_ = r;
     return {r.is_solved != 0, r.res_iter, r.res_nrm2};
   }
@@ -460,8 +473,19 @@ template <class T> class Restarted {
     std::vector<std::shared_ptr<void>> keep;
     std::shared_ptr<CSR> csr;
   };
+  struct Cond {
+    std::function<bool(int, double)> f;
+    std::exception_ptr err;
+  };
+  // no exception crosses the C ABI: it is kept, and the solve is stopped
   static int trampoline(void *user, int32_t i, double e) {
-    return (*static_cast<std::function<bool(int, double)> *>(user))(i, e) ? 1 : 0;
+    Cond *c = static_cast<Cond *>(user);
+    try {
+      return c->f(i, e) ? 1 : 0;
+    } catch (...) {
+      c->err = std::current_exception();
+      return 1;
+    }
   }
   std::shared_ptr<State> st_;
   mutable f3r_result last_{};

@@ -41,7 +41,7 @@ def test_cpp_layer_driver_checks():
     # Problem,Method,Prec,M2,M3,M4,W,Precond,ACC,Time,Iter,ImplRes,ExplRes
     assert f[0] == "stencil27_24" and float(f[11]) < 1e-8 and float(f[12]) < 1e-8
     checks = [ln for ln in lines if ln.startswith("check ")]
-    assert len(checks) == 5 and all(": ok" in ln for ln in checks), checks
+    assert len(checks) == 6 and all(": ok" in ln for ln in checks), checks
     assert out.returncode == 0
 
 
@@ -113,6 +113,17 @@ def test_python_cond_and_operator_error(ctx):
     seen = []
     r = S.solve(bv, x, cond=lambda i, e: seen.append(i) or i >= 1)
     assert r.res_iter == 1 and r.is_solved and seen == [0, 1]
+
+    # an exception in the stopping rule stops the solve and reaches the caller
+    class Stop(Exception):
+        pass
+
+    def stop_rule(i, e):
+        if i >= 1:
+            raise Stop("stop rule failed")
+        return False
+    with pytest.raises(Stop):
+        S.solve(bv, F.vector(ctx, "f64", A.n).fill(0.0), cond=stop_rule)
 
     class Boom(Exception):
         pass
