@@ -297,14 +297,14 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update_strided(int64_t n, T *__r
   }
 }
 
-// <w, V_k> for k < ncols, V contiguous; blockIdx.y selects 16 columns.
+// <w, V_k> for k < ncols, V contiguous; blockIdx.y selects NV columns (NV
+// sums reduced per block: 4 or 8 for the first columns, 16 beyond).
 // VEC: every column 4-element aligned (n % 4 == 0); each lane then takes 4
 // consecutive rows per step with 32-byte (fp64) loads
-template <class T, bool VEC = false>
+template <class T, bool VEC = false, int NV = 16>
 __global__ __launch_bounds__(kBlock) void k_mdot_strided(int64_t n, const T *__restrict__ w,
                                                          const T *__restrict__ V, int ncols, T *Hcol,
                                                          T *partial, unsigned *counters) {
-  constexpr int NV = 16;
   __shared__ T lds[NV * kWavesPerBlock];
   const int k0 = blockIdx.y * NV;
   T v[NV];
@@ -1223,8 +1223,13 @@ struct Solver {
       launch_sell<double, double, double>(sh->K, a, A64->as<double>(), Z + (int64_t)j * nn,
                                           EpiStore<double, double>{V + (int64_t)(j + 1) * nn, nullptr}, st());
       const int ncols = j + 1;
-      dim3 gm(red_grid, (ncols + 15) / 16);
-      auto mdot = nn % 4 == 0 && aligned4(V) ? k_mdot_strided<double, true> : k_mdot_strided<double, false>;
+      // one grid row of NV columns up to 8 columns (counter slot 8), then
+      // rows of 16 (slots 8.. 8 + (m1 + 15) / 16 - 1)
+      const bool v4 = nn % 4 == 0 && aligned4(V);
+      auto mdot = ncols <= 4   ? (v4 ? k_mdot_strided<double, true, 4> : k_mdot_strided<double, false, 4>)
+                  : ncols <= 8 ? (v4 ? k_mdot_strided<double, true, 8> : k_mdot_strided<double, false, 8>)
+                               : (v4 ? k_mdot_strided<double, true, 16> : k_mdot_strided<double, false, 16>);
+      dim3 gm(red_grid, ncols <= 8 ? 1 : (ncols + 15) / 16);
       hipLaunchKernelGGL(mdot, gm, dim3(kBlock), 0, st(), nn, (const double *)(V + (int64_t)(j + 1) * nn),
                          (const double *)V, ncols, H + (int64_t)j * (m1 + 1), partial<double>(), counter(8));
       hipLaunchKernelGGL(k_cgs_update_strided<double>, dim3(red_grid), dim3(kBlock), 0, st(), nn,
