@@ -186,12 +186,14 @@ template <class T> struct EpiResidNorm {
 // V_{j+1} -= V H(j)[0..j] (MULMV order: V_0 h_0, then fused adds), ||V_{j+1}||,
 // then the last block: H(j, j+1), 1/h, Givens, |e_{j+1}| for the host.
 // VEC: every pointer is 4-element aligned; each lane then takes 4
-// consecutive rows per step with 16-byte (fp32) loads and stores
+// consecutive rows per step with 16-byte (fp32) loads and stores.
+// store = 0: the updated vector is not written back (the last basis vector
+// of an inner level, which nothing reads), only its norm is taken
 template <class T, int NV, bool VEC = false>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict__ w, Cols<T, NV> cols,
                                                        int ncols, T *Hcol, T *cs, T *sn, T *e, int j,
                                                        int e0_one, T *inv, double *status, T *partial,
-                                                       unsigned *counter) {
+                                                       unsigned *counter, int store) {
   __shared__ T lds[kWavesPerBlock];
   T h[NV];
 #pragma unroll
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
           if (k < ncols) r = fma_<T>(vk[k].v[c], h[k], r);
         wv.v[c] = wv.v[c] - r;
       }
-      st4(w + 4 * q, wv);
+      if (store) st4(w + 4 * q, wv);
 #pragma unroll
       for (int c = 0; c < 4; ++c) v[0] = fma_<T>(wv.v[c], wv.v[c], v[0]);
     }
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, T *__restrict_
     for (int k = 1; k < NV; ++k)
       if (k < ncols) r = fma_<T>(cols.p[k][i], h[k], r);
     T wn = w[i] - r;
-    w[i] = wn;
+    if (store) w[i] = wn;
     v[0] = fma_<T>(wn, wn, v[0]);
   }
   // the grid's sum of ||w||^2 in the last block (one wave of blocks: the
@@ -823,7 +825,7 @@ struct Solver {
     hipLaunchKernelGGL(kern, dim3(dots_grid), dim3(kBlock), 0, st(), (int64_t)n,
                        L.V[j + 1], c, j + 1, L.Hp() + (int64_t)j * L.ldh(), L.cs->template as<T>(),
                        L.sn->template as<T>(), L.e->template as<T>(), j, 1, L.inv->template as<T>(),
-                       (double *)nullptr, partial<T>(), counter(1));
+                       (double *)nullptr, partial<T>(), counter(1), j + 1 < L.m ? 1 : 0);
   }
   template <class T, class V> void arnoldi(InnerLevel<T> &L, const V *val, int j) {
     if (j + 1 <= 4) {
@@ -1823,9 +1825,9 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
         s->template spmv_dots<float, half_t, 4>(s->L3f, s->A16->as<half_t>(), s->L3f.m - 1);
         by = s->sh16->algo_bytes(2, s->L3f.Zh.empty() ? 4 : 2, 4) + 4.0 * s->n * s->L3f.m;
         break;
-      case 5:  // level-3 CGS update + norm + Givens of that step
+      case 5:  // level-3 CGS update + norm + Givens of that step (the last: w is not stored)
         s->template cgs_update<float, 4>(s->L3f, s->L3f.m - 1);
-        by = 4.0 * s->n * (s->L3f.m + 2);
+        by = 4.0 * s->n * (s->L3f.m + 1);
         break;
       case 6: {  // the solve's mix of level-3 SpMV + CGS dots launches, steps j = 0 .. m3 - 1 (no finish)
         const int m = s->L3f.m;
@@ -1854,7 +1856,8 @@ int f3r_solver_time_kernel(f3r_solver S, int32_t which, int32_t reps, double *ms
         by = 0;
         for (int j = 0; j < m; ++j) {
           s->template cgs_update<float, 4>(s->L3f, j);
-          by += 4.0 * s->n * (j + 3);  // V_0 .. V_j read, w read and written
+          // V_0 .. V_j read, w read, and written except in the last step
+          by += 4.0 * s->n * (j + (j + 1 < m ? 3 : 2));
         }
         by /= m;
         break;
