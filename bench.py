@@ -201,7 +201,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the solve is not sharded across GPUs "
-                                   "(see DESIGN.md for how it would shard); run with --gpus 1"}))
+                                   "(row blocks with halo exchange would shard it); run with --gpus 1"}))
         sys.exit(2)
 
     import numpy as np
