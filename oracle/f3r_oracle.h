@@ -23,6 +23,28 @@
  *   depth ablations        work/src/depth.cpp:122-167, solver/gmres2.hpp:56-150
  *
  * Precision codes used throughout: 0 = fp64, 1 = fp32, 2 = fp16.
+ *
+ * What pins each part (tests/test_oracle*.py, fixtures in tests/golden/ written
+ * by the reference's own code, built from its sources by oracle/ref_harness):
+ *   fp16 conversions      every fp16 value against numpy's; round to nearest
+ *                         even at ties; fp64 -> fp16 against exact rounding
+ *   bf16 conversions      exact rational rounding of random and boundary values
+ *   scaling, SELL layout  the reference's arrays, bit for bit
+ *   SpMV (all precisions) the reference's results, bit for bit; a brute-force
+ *                         dense product; long-double sums within 1e-14
+ *   SDAINV, ILU(0), BJ    the reference's factors, bit for bit; exact inverse
+ *                         and LU of tridiagonal matrices (closed forms)
+ *   FGMRES / Richardson / the reference's nests (DOUBLE, FLOAT, HALF) run on the
+ *   Restarted / driver    host: solution vectors and residual histories bit for
+ *                         bit (reference arithmetic); its CPU driver's CSV line
+ *   static weights        the reference's static.cpp CSV lines (+-1 outer step)
+ *   depth ablations       the reference's depth.cpp CSV lines (F4, F3, F2 within
+ *                         one step; F3H, F2H no more steps), F3 = F3R with one
+ *                         static step bit for bit, fp16 levels against their
+ *                         fp32 twins
+ *   MLP forward/backward  numpy's fp64 matrix products (forward) and finite
+ *                         differences of the forward (backward)
+ * Nothing here is left unpinned.
  */
 #ifndef F3R_ORACLE_H
 #define F3R_ORACLE_H

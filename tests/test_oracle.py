@@ -321,6 +321,23 @@ def test_build_arithmetic_close_to_reference_arithmetic():
     assert r_bld.expl_res < 1e-8
 
 
+def test_mlp_forward_against_numpy():
+    # the oracle's forward pass: ReLU(W1 x + b1) -> ReLU(W2 . + b2) -> w3 . + b3,
+    # against numpy's matrix products in fp64 (a dropped bias, a missing ReLU
+    # or a transposed weight fails by O(1))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((9, 12))
+    w1, b1 = rng.standard_normal((6, 12)), rng.standard_normal(6)
+    w2, b2 = rng.standard_normal((5, 6)), rng.standard_normal(5)
+    w3, b3 = rng.standard_normal((1, 5)), rng.standard_normal(1)
+    a1 = np.maximum(x @ w1.T + b1, 0.0)
+    a2 = np.maximum(a1 @ w2.T + b2, 0.0)
+    ref = (a2 @ w3.T + b3).reshape(-1)
+    y = O.mlp_forward(x, w1, b1, w2, b2, w3, b3)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * (1 + np.max(np.abs(ref)))
+    assert np.any(a1 == 0) and np.any(a2 == 0)  # both ReLUs clip something here
+
+
 def test_mlp_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
     x = rng.standard_normal((7, 12))
