@@ -8,8 +8,9 @@
 // fp16 values; the SDAINV factors likewise).  Every reduction is per
 // segment: each block writes its partial sums, and one block per segment
 // adds its blocks' partials in a fixed order and does that segment's
-// Hessenberg / Givens / Richardson-weight work.  The host reads one array of
-// residual norms per outer step for all segments.  A segment that has
+// Hessenberg / Givens / Richardson-weight work.  Each outer step of all
+// segments is replayed from one HIP graph, after which the host reads one
+// array of residual norms for all segments.  A segment that has
 // converged stops counting steps and keeps its solution; its vectors may run
 // on with the others, and the breakdown guards below (a zero norm, a zero
 // rotation, a zero pivot) keep them finite.
@@ -723,6 +724,9 @@ struct BatchSolver {
   DevPtr hin, hout, hout2, hr, tmp, tt, t2;
   int64_t cnt = 0, calls = 0;
   std::map<uint64_t, hipGraphExec_t> graphs;  // level-2 body per probe pattern
+  // a whole outer step per (probe pattern, step index j)
+  std::map<std::pair<uint64_t, int>, hipGraphExec_t> step_graphs;
+  static constexpr size_t kMaxStepGraphs = 64;
   std::vector<hipGraph_t> graph_src;
 
   BatchSolver() = default;
@@ -730,6 +734,7 @@ struct BatchSolver {
   BatchSolver &operator=(const BatchSolver &) = delete;
   ~BatchSolver() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto &g : step_graphs) (void)hipGraphExecDestroy(g.second);
     for (auto &g : graph_src) (void)hipGraphDestroy(g);
   }
 
@@ -955,10 +960,12 @@ struct BatchSolver {
   // use_graph the body is captured once per probe pattern (at most 33) and
   // replayed: it holds no host synchronisation, memset or host copy, and the
   // Richardson weights and probe results stay in device memory between calls
-  void level2(const double *in, double *out) {
+  // body_graph false: the body is launched eagerly (inside a capture of the
+  // whole outer step)
+  void level2(const double *in, double *out, bool body_graph) {
     BLevel<float> &L = L2;
     hipLaunchKernelGGL((k_b_cast<double, float>), dim3(G), dim3(kBlock), 0, st(), in, L.Vc(N, 0));
-    const uint64_t key = p.use_graph ? next_pattern() : UINT64_MAX;
+    const uint64_t key = p.use_graph && body_graph ? next_pattern() : UINT64_MAX;
     if (key == UINT64_MAX || (graphs.size() > 32 && graphs.find(key) == graphs.end())) {
       level2_body();
     } else {
@@ -991,6 +998,52 @@ struct BatchSolver {
                        (const float *)L.Zc(N, 0), N, (const float *)L.H->as<float>(), L.hs(), L.m + 1,
                        (const float *)L.e->as<float>(), (int64_t)(L.m + 1), (const int32_t *)nullptr, L.m,
                        (const int32_t *)bs(), out, 0, (const float *)nullptr);
+  }
+
+  // outer Arnoldi step j of every segment: level 2 on V_j into Z_j, the fp64
+  // SpMV into V_{j+1} and the outer Gram-Schmidt / Givens step, whose |e_{j+1}|
+  // per segment lands in `status`.  With use_graph the whole step is
+  // captured once per (probe pattern, j) and replayed as one graph launch
+  // (beyond kMaxStepGraphs of them, only the level-2 bodies are replayed).
+  void outer_step(int j) {
+    BLevel<double> &L = L1;
+    double *V = L.V->as<double>();
+    auto body = [&](bool body_graph) {
+      level2(V + (int64_t)j * N, L.Zc(N, j), body_graph);
+      spmv<double, double, double, double>(A64->as<double>(), false, (const double *)L.Zc(N, j),
+                                           V + (int64_t)(j + 1) * N, nullptr);
+      arnoldi(L, j, 0, status->as<double>(), (const double *)L.Vc(N, 0), false);
+    };
+    const uint64_t pat = p.use_graph ? next_pattern() : UINT64_MAX;
+    const auto key = std::make_pair(pat, j);
+    auto it = step_graphs.find(key);
+    if (pat == UINT64_MAX || (it == step_graphs.end() && step_graphs.size() >= kMaxStepGraphs)) {
+      body(true);
+      return;
+    }
+    if (it == step_graphs.end()) {
+      const int64_t cnt0 = cnt, calls0 = calls;
+      hipGraph_t gr;
+      F3R_HIP(hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+      try {
+        body(false);
+      } catch (...) {
+        (void)hipStreamEndCapture(st(), &gr);
+        throw;
+      }
+      F3R_HIP(hipStreamEndCapture(st(), &gr));
+      hipGraphExec_t ex;
+      const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      if (e != hipSuccess) (void)hipGraphDestroy(gr);
+      F3R_HIP(e);
+      graph_src.push_back(gr);
+      it = step_graphs.emplace(key, ex).first;
+      cnt = cnt0;  // the capture only recorded the calls
+      calls = calls0;
+    }
+    F3R_HIP(hipGraphLaunch(it->second, st()));
+    cnt += (int64_t)p.m2 * p.m3;
+    calls += (int64_t)p.m2 * p.m3;
   }
 
   std::vector<double> read_status() {
@@ -1046,10 +1099,7 @@ struct BatchSolver {
       if (!any) break;
       int j = 0;
       for (; j < m1; ++j) {
-        level2(V + (int64_t)j * N, L.Zc(N, j));
-        spmv<double, double, double, double>(A64->as<double>(), false, (const double *)L.Zc(N, j),
-                                             V + (int64_t)(j + 1) * N, nullptr);
-        arnoldi(L, j, 0, status->as<double>(), (const double *)L.Vc(N, 0), false);
+        outer_step(j);
         std::vector<double> st_h = read_status();
         bool left = false;
         for (int s = 0; s < nseg; ++s) {

@@ -402,8 +402,9 @@ int f3r_solve_cond(f3r_solver S, f3r_vec b, f3r_vec x, f3r_cond_fn cond, void *u
  * read); a system solved for k right-hand sides appears k times.  The
  * segments are laid out as one block-diagonal matrix with each segment
  * padded to whole 2048-row tiles; all reductions, Hessenberg/Givens updates
- * and Richardson weights are per segment, and the host reads one array of
- * residual norms per outer step.  The nest and its per-segment semantics
+ * and Richardson weights are per segment.  With use_graph each outer step
+ * of all segments is one HIP graph launch, after which the host reads one
+ * array of residual norms.  The nest and its per-segment semantics
  * are f3r_solver_create's HALF build (params: m1..m4, c, restart, eps, C, K,
  * index, range_scale; 0 = the per-segment automatic range factor).  The
  * reference solves one system per run (work/src/f3r-gpu.cpp:44-288); this
