@@ -312,7 +312,7 @@ def main():
                                f"m=(100;8,4,2), c=64, restart 20, eps 1e-8",
                    "n": A.n, "nnz": A.nnz, "outer_iters": res.res_iter, "solved": bool(res.is_solved),
                    "true_rel_res": float(true_rel), "precond_calls": res.precond_calls,
-                   "setup_s": round(setup_s, 2), "layout_build": "host" if args.host_build else "device",
+                   "setup_s": round(setup_s, 2), "layout_build": S.layouts_where(),
                    "sdainv_factors": P.where, "sdainv_s": round(factor_s, 3),
                    "graph": not args.no_graph},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -419,6 +419,12 @@ class SELL:
         check(lib.f3r_mat_lds_blocks(self.handle, ctypes.byref(b), ctypes.byref(st)), "mat_lds_blocks")
         return b.value, st.value
 
+    def where(self):
+        """where the layout was built, "host" or "device" (a string)"""
+        w = ctypes.c_int()
+        check(lib.f3r_mat_setup_where(self.handle, ctypes.byref(w)), "mat_setup_where")
+        return "device" if w.value == 1 else "host"
+
     def chunks(self):
         """(chunks, chunks in the chunk-offset format)"""
         c, o = ctypes.c_int32(), ctypes.c_int32()
@@ -555,6 +561,12 @@ class F3R:
         check(lib.f3r_solve(self.handle, b.handle, x.handle, ctypes.byref(r)), "solve")
         return solve_res_t(bool(r.is_solved), r.res_iter, r.res_nrm2, r.nrm_b, r.solve_s, r.precond_calls,
                            r.outer_cycles)
+
+    def layouts_where(self):
+        """where the matrix layouts were built, "host" or "device" (a string)"""
+        w = ctypes.c_int()
+        check(lib.f3r_solver_setup_where(self.handle, ctypes.byref(w)), "solver_setup_where")
+        return "device" if w.value == 1 else "host"
 
     def history(self):
         ln = ctypes.c_int32()

@@ -129,6 +129,8 @@ SIGNATURES = [
     ("f3r_mat_to_sell_ex", ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i32, c_i32, c_i32, c_i32,
                                           ctypes.POINTER(c_vp)]),
     ("f3r_mat_chunks", ctypes.c_int, [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
+    ("f3r_mat_setup_where", ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int)]),
+    ("f3r_solver_setup_where", ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int)]),
     ("f3r_mat_lds_blocks", ctypes.c_int, [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
     ("f3r_sell_offset_layout", ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, P_i64, P_i64, P_i64,
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

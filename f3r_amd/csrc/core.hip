@@ -1259,11 +1259,11 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   const bool lds = (flags & F3R_SELL_LDS) != 0;
   F3R_CHECK(!lds || (flags & (F3R_SELL_OFFSETS | F3R_SELL_DEVICE)) == 0, F3R_ERR_ARG,
             "to_sell: LDS-staged gathers need the int32-index layout built on the host");
-  if ((flags & 2) && sigma == 1 && csr->m->d_rptr) {  // build on the device
+  if ((flags & 2) && csr->m->d_rptr) {  // build on the device
     std::shared_ptr<SellShared> sh;
     DevPtr v64;
     if (build_sell_device(ctx->c.get(), *csr->m->host, csr->m->d_rptr->as<int32_t>(), csr->m->d_idx->as<int32_t>(),
-                          csr->m->d_val->as<double>(), C, K, (flags & 1) != 0, sh, v64)) {
+                          csr->m->d_val->as<double>(), C, sigma, K, (flags & 1) != 0, false, sh, v64)) {
       auto m = std::make_shared<Mat>();
       m->ctx = ctx->c;
       m->format = 1;
@@ -1425,6 +1425,14 @@ int f3r_mat_chunks(f3r_mat A, int32_t *chunks, int32_t *offset_chunks) {
   F3R_CHECK(A->m->format == 1, F3R_ERR_ARG, "mat_chunks: needs a SELL matrix");
   *chunks = A->m->sh->nchunks;
   *offset_chunks = A->m->sh->offset_chunks;
+  F3R_API_END
+}
+
+int f3r_mat_setup_where(f3r_mat A, int *where) {
+  F3R_API_BEGIN
+  F3R_CHECK(A && where, F3R_ERR_ARG, "null argument");
+  F3R_CHECK(A->m->format == 1, F3R_ERR_ARG, "mat_setup_where: needs a SELL matrix");
+  *where = A->m->sell_dev_val64 ? 1 : 0;
   F3R_API_END
 }
 

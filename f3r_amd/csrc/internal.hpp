@@ -83,11 +83,12 @@ std::shared_ptr<Prec> prec_of(f3r_prec p);
 f3r_prec wrap_prec(std::shared_ptr<Prec> p);  // new handle owning a reference
 f3r_vec wrap_vec(const Vec &v);                // new handle of a vector (view)
 void release_vec(f3r_vec v);
-// device build of a sigma = 1 SELL layout from a device CSR (build.hip);
-// false (nothing built) when a chunk is too long for the on-chip sort
+// device build of a SELL-C-sigma layout from a device CSR (build.hip), the
+// columns renumbered into the sorted order when `renumber`; false (nothing
+// built) when a chunk or a sigma window is too long for the on-chip sorts
 bool build_sell_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col,
-                       const double *d_val, int32_t C, int32_t K, bool offsets, std::shared_ptr<SellShared> &sh,
-                       DevPtr &sval);
+                       const double *d_val, int32_t C, int32_t sigma, int32_t K, bool offsets, bool renumber,
+                       std::shared_ptr<SellShared> &sh, DevPtr &sval);
 // device factorisations (factor.hip), bit-identical to sdainv_host /
 // ilu0_host; false (F untouched) when a row outgrows the device capacity
 bool sdainv_device(Context *c, const HostCSR &A, const int32_t *d_rptr, const int32_t *d_col, const double *d_val,

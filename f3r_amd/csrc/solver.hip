@@ -760,6 +760,7 @@ struct Solver {
   // 1/||w|| of a basis vector left unnormalised by the last Arnoldi step;
   // the next innermost entry kernel (its first reader) applies it
   const float *pending_inv = nullptr;
+  bool layouts_on_device = false;  // the matrix layouts were built on the device
   int red_grid = 1;   // capped grid of the outer level's strided reductions
   int dots_grid = 1;  // blocks of the streaming kernels with a grid reduction
   int spmv_grid = 0;  // blocks of the SpMV kernels with a fused grid reduction (0 = one row per thread)
@@ -1299,22 +1300,21 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
   F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "solver: K must be 1, 2, 4 or 8");
   // flags bit 2: build the layouts on the device from the CSR's device copy
-  // (sigma = 1; the same arrays as the host build)
-  const bool on_dev = (p.flags & 4) && sigma == 1 && dev && dev->d_rptr && dev->ctx == ctx;
+  // (the same arrays as the host build)
+  const bool on_dev = (p.flags & 4) && dev && dev->d_rptr && dev->ctx == ctx;
   HostSell hs;
   bool built = false;
   if (on_dev) {
     std::shared_ptr<SellShared> sh, sh16;
     DevPtr v64, v64_16;
     built = build_sell_device(ctx.get(), A, dev->d_rptr->as<int32_t>(), dev->d_idx->as<int32_t>(),
-                              dev->d_val->as<double>(), C, K, p.index != 0, sh, v64);
+                              dev->d_val->as<double>(), C, sigma, K, p.index != 0, true, sh, v64);
     if (built && p.variant == 0 && K16 != K)
       built = build_sell_device(ctx.get(), A, dev->d_rptr->as<int32_t>(), dev->d_idx->as<int32_t>(),
-                                dev->d_val->as<double>(), C, K16, p.index != 0, sh16, v64_16);
+                                dev->d_val->as<double>(), C, sigma, K16, p.index != 0, true, sh16, v64_16);
     if (built) {
       S->sh = sh;
       S->sh->nnz = A.nnz();
-      S->sh->sigma = 1;
       S->sh16 = S->sh;
       S->A64 = cast_values_device(ctx.get(), v64, sh->nslots, F3R_F64);
       if (p.variant != 2) S->A32 = cast_values_device(ctx.get(), v64, sh->nslots, F3R_F32);
@@ -1324,13 +1324,12 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
         } else {
           S->sh16 = sh16;
           S->sh16->nnz = A.nnz();
-          S->sh16->sigma = 1;
           S->A16 = cast_values_device(ctx.get(), v64_16, sh16->nslots, F3R_F16);
         }
       }
       F3R_HIP(hipStreamSynchronize(ctx->stream));
-      hs.perm.resize(A.n);
-      std::iota(hs.perm.begin(), hs.perm.end(), 0);
+      hs.perm = sh->h_perm;
+      S->layouts_on_device = true;
     }
   }
   if (!built) {
@@ -1684,6 +1683,13 @@ int f3r_nest_create(f3r_ctx ctx, f3r_mat A, const f3r_level *levels, int32_t nle
   }
   p.m2 = p.m3 = 1;  // not used by the generic levels
   *S = new f3r_solver_s{solver_create(ctx_of(ctx), *m->host, P, p, m.get(), &nest)};
+  F3R_API_END
+}
+
+int f3r_solver_setup_where(f3r_solver S, int *where) {
+  F3R_API_BEGIN
+  F3R_CHECK(S && where, F3R_ERR_ARG, "setup_where: null argument");
+  *where = S->s->layouts_on_device ? 1 : 0;
   F3R_API_END
 }
 

@@ -169,7 +169,9 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
  * CSR order of every row.  No counterpart in the reference. */
 #define F3R_SELL_OFFSETS 1
 /* flag bit 1: build the layout on the device from the CSR's device copy
- * (sigma = 1 only; identical arrays to the host build) */
+ * (identical arrays to the host build; sigma up to 4096, wider windows and
+ * chunks of more than 4096 entries in the offset format are built on the
+ * host) */
 #define F3R_SELL_DEVICE 2
 /* flag bit 2: LDS-staged x gathers (int32-index layout built on the host):
  * every block of 256 rows whose slots read a column window of at most 32 KB
@@ -215,6 +217,10 @@ int f3r_mat_info(f3r_mat A, int32_t *nrows, int64_t *nnz, int64_t *nslots, int *
 int f3r_mat_lds_blocks(f3r_mat A, int32_t *blocks, int32_t *staged);
 /* SELL: chunks in total and chunks stored in the chunk-offset format */
 int f3r_mat_chunks(f3r_mat A, int32_t *chunks, int32_t *offset_chunks);
+/* SELL: *where = 0: layout built on the host; 1: on the device
+ * (F3R_SELL_DEVICE, when it could run there).  No counterpart in the
+ * reference. */
+int f3r_mat_setup_where(f3r_mat A, int *where);
 /* y = A x  (SELL::operate / CSR::operate, sell.hpp:55-71, csr.hpp:32-40).
  * Accumulates in fp64 for fp64 data, else in fp32 (fp16 x fp16 too). */
 int f3r_spmv(f3r_mat A, f3r_vec x, f3r_vec y);
@@ -275,8 +281,8 @@ typedef struct {
   int32_t use_graph;   /* capture the inner levels in HIP graphs */
   int32_t flags;       /* bit 0: record the outer residual history; bit 1: run the SpMV
                           kernels with fused reductions on the capped grid too; bit 2:
-                          build the matrix layouts on the device (sigma 1; needs A created
-                          with a context) */
+                          build the matrix layouts on the device (as F3R_SELL_DEVICE; needs
+                          A created with a context) */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */
@@ -320,6 +326,10 @@ int f3r_solver_create(f3r_ctx ctx, f3r_mat A, f3r_prec P, const f3r_params *para
 int f3r_solve(f3r_solver S, f3r_vec b, f3r_vec x, f3r_result *res);
 /* |e_j| / ||b|| of each outer step of the last solve (flags bit 0) */
 int f3r_solver_history(f3r_solver S, double *hist, int32_t cap, int32_t *len);
+/* *where = 0: the solver's matrix layouts were built on the host; 1: on the
+ * device (flags bit 2, when it could run there).  No counterpart in the
+ * reference. */
+int f3r_solver_setup_where(f3r_solver S, int *where);
 /* one application of level 2, 3 or 4 as an operator on fp64 vectors, for the
  * per-level error study */
 int f3r_solver_level_apply(f3r_solver S, int32_t level, f3r_vec in, f3r_vec out);

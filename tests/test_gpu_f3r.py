@@ -197,14 +197,18 @@ def test_layouts_give_identical_solves(ctx, kind):
         assert it == outs[0][0] and np.array_equal(x, outs[0][1])
 
 
-def test_device_built_layouts_give_identical_solves(ctx):
+@pytest.mark.parametrize("sigma", [1, 64])
+def test_device_built_layouts_give_identical_solves(ctx, sigma):
+    # sigma 64: the boundary rows (fewer entries) move inside their windows,
+    # the matrix is renumbered, b and x are permuted on the way in and out
     import f3r_amd as F
     A = _problem("lognormal", 18)
     P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
     bv = F.vector.from_numpy(ctx, F.uniform(A.n, -1.0, 1.0, seed=8))
     outs = []
     for dev in (False, True):
-        S = F.F3R(ctx, A, P, device_build=dev)
+        S = F.F3R(ctx, A, P, device_build=dev, sigma=sigma)
+        assert S.layouts_where() == ("device" if dev else "host")
         xv = F.vector(ctx, "f64", A.n).fill(0.0)
         r = S.solve(bv, xv)
         outs.append((r.res_iter, xv.download()))

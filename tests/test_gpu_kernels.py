@@ -243,15 +243,34 @@ def test_bjilu0_block_per_workgroup_matches_oracle(ctx, blocks, dtype):
     assert np.array_equal(got, out)
 
 
-@pytest.mark.parametrize("C,K,offsets", [(64, 4, True), (128, 8, True), (64, 4, False), (256, 2, True)])
-def test_device_sell_build_matches_host(ctx, C, K, offsets):
+def _device_fits(A, C, sigma, offsets):
+    # the device build's on-chip sorts: sigma windows of at most 4096 rows and,
+    # in the offset format, chunks of at most 4096 entries (else the host builds)
+    if sigma > 4096:
+        return False
+    if not offsets or C < 64:
+        return True
+    lens = np.diff(np.asarray(A.rptr))
+    perm = np.concatenate([w0 + np.argsort(-lens[w0:w0 + sigma], kind="stable")
+                           for w0 in range(0, A.n, sigma)])
+    sl = lens[perm]
+    return max(int(sl[c0:c0 + C].sum()) for c0 in range(0, A.n, C)) <= 4096
+
+
+@pytest.mark.parametrize("C,sigma,K,offsets", [(64, 1, 4, True), (128, 1, 8, True), (64, 1, 4, False),
+                                               (256, 1, 2, True), (64, 16, 4, False), (32, 256, 2, False),
+                                               (128, 64, 8, True), (64, 4096, 1, False)])
+def test_device_sell_build_matches_host(ctx, C, sigma, K, offsets):
     # the device-built layout equals the host-built one: same slot count and
-    # chunk split, and bit-identical SpMV results in fp64 and fp16
+    # chunk split, and bit-identical SpMV results in fp64 and fp16 (the rows
+    # sorted inside sigma windows on the device as on the host: y comes back
+    # through the permutation)
     import f3r_amd as F
     for name, A in _mats():
-        print(f"device build {name} C={C} K={K} offsets={offsets}", flush=True)
-        Hs = F.SELL(ctx, A, "f64", C=C, sigma=1, K=K, offsets=offsets)
-        Ds = F.SELL(ctx, A, "f64", C=C, sigma=1, K=K, offsets=offsets, device=True)
+        print(f"device build {name} C={C} sigma={sigma} K={K} offsets={offsets}", flush=True)
+        Hs = F.SELL(ctx, A, "f64", C=C, sigma=sigma, K=K, offsets=offsets)
+        Ds = F.SELL(ctx, A, "f64", C=C, sigma=sigma, K=K, offsets=offsets, device=True)
+        assert Hs.where() == "host" and Ds.where() == ("device" if _device_fits(A, C, sigma, offsets) else "host"), name
         assert Ds.nslots() == Hs.nslots() and Ds.chunks() == Hs.chunks(), name
         x = np.random.default_rng(3).uniform(-1, 1, A.n)
         for dt, np_t in (("f64", np.float64), ("f16", np.float16)):
