@@ -385,17 +385,20 @@ class CSRDevice:
 
 
 class SELL:
-    """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166)."""
+    """SELL<w, T, device> / SELLsig (sparse/matrix/sell.hpp:35-166).  lanes:
+    threads per row of the SpMV, 1 (CSR order) or K (F3R_SELL_TPR)"""
 
     def __init__(self, ctx, A, dtype="f64", C=64, sigma=1, K=1, offsets=False, device=False, lds=False,
-                 _handle=None):
+                 lanes=1, _handle=None):
         self.ctx = ctx
         self.A = A
         self.dtype = _dtype(dtype)
-        self.C, self.sigma, self.K, self.offsets, self.lds = C, sigma, K, offsets, lds
+        self.C, self.sigma, self.K, self.offsets, self.lds, self.lanes = C, sigma, K, offsets, lds, lanes
+        if lanes not in (1, K):
+            raise ValueError("SELL: lanes per row must be 1 or K")
         if _handle is None:
             h = ctypes.c_void_p()
-            flags = (1 if offsets else 0) | (2 if device else 0) | (4 if lds else 0)
+            flags = (1 if offsets else 0) | (2 if device else 0) | (4 if lds else 0) | (8 if lanes > 1 else 0)
             check(lib.f3r_mat_to_sell_ex(ctx.handle, A.handle(ctx), self.dtype, C, sigma, K, flags,
                                          ctypes.byref(h)), "mat_to_sell")
             self.handle = h
@@ -406,7 +409,7 @@ class SELL:
         """cross-precision copy sharing the index arrays (sell.hpp:51-53)"""
         h = ctypes.c_void_p()
         check(lib.f3r_mat_cast(self.handle, _dtype(dtype), ctypes.byref(h)), "mat_cast")
-        return SELL(self.ctx, self.A, dtype, self.C, self.sigma, self.K, self.offsets, _handle=h)
+        return SELL(self.ctx, self.A, dtype, self.C, self.sigma, self.K, self.offsets, lanes=self.lanes, _handle=h)
 
     def nslots(self):
         ns = ctypes.c_int64()

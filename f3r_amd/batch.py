@@ -51,7 +51,8 @@ class _Worker:
                 t0 = time.time()
                 A = make_system(npts, nx, delta)
                 if self.tune:
-                    best, _ = autotune(self.ctx, A, "f16", tprs=(), lds=(False,), **self.cands)
+                    # the solver's fused kernels take one thread per row
+                    best, _ = autotune(self.ctx, A, "f16", tprs=(), lds=(False,), sell_lanes=(False,), **self.cands)
                     L = solver_layout(best)
                     lay = (L["C"], L["sigma"], L["K"], L["index"])
                 else:
@@ -145,7 +146,7 @@ def tune_lockstep(ctx, fam, log=None):
     i, npts, nx, delta = max((t for t in fam if t[1] == 27), key=lambda t: t[2])
     A = make_system(npts, nx, delta)
     best, table = autotune(ctx, A, "f16", Cs=(64, 128, 256), sigmas=(1,), Ks=(1, 2, 4, 8), offsets=(False, True),
-                           lds=(False,), tprs=(), reps=10)
+                           lds=(False,), tprs=(), sell_lanes=(False,), reps=10)
     L = solver_layout(best)
     if log:
         log(f"lockstep layout tuned on system {i} ({npts}-point {nx}^3): C={L['C']} K={L['K']} index={L['index']} "

@@ -134,6 +134,10 @@ __host__ __device__ inline int64_t chunk_slot_pos(int64_t c0, int32_t cw, int32_
 
 struct SellShared {  // index data shared by all precisions of one matrix
   int32_t n = 0, C = 64, sigma = 1, K = 1, nchunks = 0;
+  // lanes per row of the standalone SpMV: 1 (every slot of a row on one
+  // lane, CSR order) or K (F3R_SELL_TPR: lane k of a row takes slot k of
+  // every group, then the K lanes add their sums)
+  int32_t lanes = 1;
   int64_t nslots = 0, nnz = 0;
   bool internal = false;
   DevPtr cptr, cwidth, idx, perm;  // perm null when identity

@@ -619,6 +619,15 @@ static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hi
                                  EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
     return;
   }
+  if (A->sh->lanes > 1) {  // K lanes per row
+    if (b)
+      launch_sell_lanes<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
+                                   EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
+    else
+      launch_sell_lanes<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
+                                   EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
+    return;
+  }
   if (b)
     launch_sell<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
                            EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
@@ -1255,8 +1264,11 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   F3R_CHECK(sigma >= 1, F3R_ERR_ARG, "to_sell: sigma >= 1");
   F3R_CHECK(csr->m->host->n == csr->m->host->ncols, F3R_ERR_ARG, "to_sell: square matrices only");
   *out = nullptr;
-  F3R_CHECK((flags & ~7) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
+  F3R_CHECK((flags & ~15) == 0, F3R_ERR_ARG, "to_sell: unknown flags");
   const bool lds = (flags & F3R_SELL_LDS) != 0;
+  const bool lanes = (flags & F3R_SELL_TPR) != 0;
+  F3R_CHECK(!lanes || ((K == 2 || K == 4 || K == 8) && (flags & (F3R_SELL_OFFSETS | F3R_SELL_LDS)) == 0),
+            F3R_ERR_ARG, "to_sell: K lanes per row need K in {2, 4, 8} and the int32-index layout without LDS staging");
   F3R_CHECK(!lds || (flags & (F3R_SELL_OFFSETS | F3R_SELL_DEVICE)) == 0, F3R_ERR_ARG,
             "to_sell: LDS-staged gathers need the int32-index layout built on the host");
   if ((flags & 2) && csr->m->d_rptr) {  // build on the device
@@ -1271,6 +1283,7 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
       m->host = csr->m->host;
       sh->internal = false;
       sh->nnz = csr->m->host->nnz();
+      sh->lanes = lanes ? K : 1;
       m->sh = sh;
       m->s_val = cast_values_device(ctx->c.get(), v64, sh->nslots, dtype);
       m->sell_dev_val64 = v64;
@@ -1287,6 +1300,7 @@ int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t s
   m->host = csr->m->host;
   m->sh = upload_sell_shared(ctx->c.get(), hs, false);
   m->sh->nnz = csr->m->host->nnz();
+  m->sh->lanes = lanes ? K : 1;
   m->s_val = upload_sell_values(ctx->c.get(), hs, dtype);
   if (lds) {
     const std::vector<int2> w = block_windows(hs);

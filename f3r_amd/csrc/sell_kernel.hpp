@@ -238,6 +238,55 @@ __global__ __launch_bounds__(kBlock) void k_sell(SellArgs a, const V *__restrict
   epi.finish(st);
 }
 
+// SELL SpMV with K lanes per row (int32-index chunks only): lane k of row r
+// sums slot k of each of the row's groups, K contiguous values and indices
+// per group of the K lanes (coalesced, as the one-lane-per-row kernel reads
+// them), then the K lanes add their sums by butterfly shuffles.  The sum is
+// reassociated: not the CSR order.  For rows much longer than K slots.
+template <class V, class X, class ACC, int K, class Epi>
+__global__ __launch_bounds__(kBlock) void k_sell_lanes(SellArgs a, const V *__restrict__ val,
+                                                       const X *__restrict__ x, Epi epi) {
+  static_assert(K >= 2 && kWave % K == 0, "K lanes of a row lie in one wavefront");
+  constexpr int kRowsPerBlock = kBlock / K;
+  const int k = threadIdx.x % K;
+  typename Epi::State st{};
+  for (int r0 = xcd_block() * kRowsPerBlock; r0 < a.n; r0 += gridDim.x * kRowsPerBlock) {
+    const int r = r0 + threadIdx.x / K;
+    ACC acc = 0;
+    if (r < a.n) {
+      const int s = r >> a.log2C;
+      const int j = r - (s << a.log2C);
+      const int C = 1 << a.log2C;
+      const int rem = a.n - (s << a.log2C);
+      const int tc = rem < C ? rem : C;
+      const int groups = chunk_groups(a.cwidth[s]);
+      int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K + k;
+      const int64_t step = (int64_t)tc * K;
+      for (int g = 0; g < groups; ++g, pos += step)
+        acc = fma_<ACC>((ACC)to_acc(val[pos]), (ACC)to_acc(x[a.idx[pos]]), acc);
+    }
+#pragma unroll
+    for (int off = K / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (r < a.n && k == 0) {
+      const typename Epi::Pre p = epi.pre(r);
+      epi.row(r, acc, p, st);
+    }
+  }
+  epi.finish(st);
+}
+
+template <class V, class X, class ACC, class Epi>
+inline void launch_sell_lanes(int K, const SellArgs &a, const V *val, const X *x, Epi epi, hipStream_t st) {
+  if (a.n == 0) return;
+  auto g = [&](int k) { return dim3((unsigned)(((int64_t)a.n * k + kBlock - 1) / kBlock)); };
+  switch (K) {
+    case 2: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 2, Epi>), g(2), dim3(kBlock), 0, st, a, val, x, epi); break;
+    case 4: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 4, Epi>), g(4), dim3(kBlock), 0, st, a, val, x, epi); break;
+    case 8: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 8, Epi>), g(8), dim3(kBlock), 0, st, a, val, x, epi); break;
+    default: break;
+  }
+}
+
 // Row r of an int32-index SELL with x staged in LDS: xs[c - lo] holds x[c]
 // for the block's column window [lo, lo + len).  The slots and the sum are
 // those of sell_row (same bits); only the source of x differs.

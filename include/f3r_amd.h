@@ -179,6 +179,12 @@ int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigm
  * there; wider blocks gather from global memory.  Same sums, same bits.  For
  * banded matrices; f3r_mat_lds_blocks tells how many blocks are staged. */
 #define F3R_SELL_LDS 4
+/* flag bit 3: K threads per row (K in {2, 4, 8}; int32-index layout, no LDS
+ * staging): lane k of a row sums slot k of each of its groups and the K
+ * lanes add their sums.  The same layout; the sum is reassociated (not the
+ * CSR order).  For long rows: the threads-per-row dimension of the SELL
+ * SpMV, chosen by the autotuner with C and sigma. */
+#define F3R_SELL_TPR 8
 int f3r_mat_to_sell_ex(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
                        int32_t K, int32_t flags, f3r_mat *sell);
 /* host-only SELL-C-sigma layout of a CSR matrix (csr_to_sell /

@@ -286,3 +286,21 @@ def test_offset_layout_reproduces_csr_rows(kind, C, K):
     # slots, so the layout never grows by more than that
     plain = F.sell_layout(A, C, 1, K)[3]
     assert L["val"].size <= 1.25 * plain.size + K * ((A.n + C - 1) // C) * C
+
+
+def test_tuner_random_rows_problem():
+    # the autotuner's long-row problem: seeded, rows of 1..maxlen entries with
+    # the diagonal, ascending distinct columns, strictly dominant diagonal
+    from f3r_amd.tune import random_rows, candidates
+    A = random_rows(500, 40, seed=3)
+    B = random_rows(500, 40, seed=3)
+    assert np.array_equal(A.rptr, B.rptr) and np.array_equal(A.idx, B.idx) and np.array_equal(A.val, B.val)
+    lens = np.diff(A.rptr)
+    assert lens.min() >= 1 and lens.max() <= 40
+    for i in range(A.n):
+        c, v = A.idx[A.rptr[i]:A.rptr[i + 1]], A.val[A.rptr[i]:A.rptr[i + 1]]
+        assert np.all(np.diff(c) > 0) and i in c
+        assert v[c == i][0] > np.abs(v[c != i]).sum()
+    # the SELL threads-per-row candidates: K lanes for K >= 2, int32 indices, no staging
+    sells = [e for e in candidates(tprs=()) if e["lanes"] > 1]
+    assert sells and all(e["lanes"] == e["K"] >= 2 and not e["offsets"] and not e["lds"] for e in sells)

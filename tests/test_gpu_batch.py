@@ -30,14 +30,16 @@ def test_autotune_candidates_agree_and_the_tuned_solve_is_unchanged():
     best, table = autotune(ctx, A, "f16", Cs=(32, 64, 256), sigmas=(1, 256), Ks=(1, 4, 8), offsets=(False, True),
                            lds=(False, True), tprs=(1, 4, 16), reps=5, check=True)
     # SELL: 3 C x 2 sigma x 3 K with one column index per slot (sigma 256 >= every C), with and
-    # without LDS-staged gathers, and the chunk-offset format for sigma 1 and C >= 64 (2 C x 3 K);
+    # without LDS-staged gathers, K threads per row for those of them with K >= 2 and no staging
+    # (3 C x 2 sigma x 2 K), and the chunk-offset format for sigma 1 and C >= 64 (2 C x 3 K);
     # CSR: 3 lane counts
     sells = [e for e in table if e["format"] == "sell"]
-    assert len(sells) == 2 * 18 + 6 and len(table) == len(sells) + 3
+    assert len(sells) == 2 * 18 + 12 + 6 and len(table) == len(sells) + 3
+    assert sum(e["lanes"] > 1 for e in sells) == 12
     assert best["ms"] == min(e["ms"] for e in table) > 0
-    ref = sells[0]["y"]
+    ref = next(e for e in sells if e["lanes"] == 1)["y"]
     for e in table:
-        if e["format"] == "sell" or e["tpr"] == 1:
+        if (e["format"] == "sell" and e["lanes"] == 1) or (e["format"] == "csr" and e["tpr"] == 1):
             assert np.array_equal(e["y"].view(np.uint16), ref.view(np.uint16)), e
         else:
             assert np.allclose(e["y"].astype(np.float64), ref.astype(np.float64), rtol=2e-3, atol=1e-4), e

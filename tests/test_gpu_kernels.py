@@ -342,6 +342,49 @@ def test_csr_spmv_lanes_per_row(ctx, tpr, vd, xd, yd):
         assert np.all(np.abs(got - exact) <= tol), (name, float(np.max(np.abs(got - exact) - tol)))
 
 
+@pytest.mark.parametrize("C,sigma,K", [(64, 1, 2), (32, 256, 4), (128, 1, 8)])
+@pytest.mark.parametrize("vd,xd,yd", [("f64", "f64", "f64"), ("f32", "f32", "f32"), ("f16", "f16", "f16"),
+                                      ("f16", "f32", "f32")])
+def test_sell_threads_per_row(ctx, C, sigma, K, vd, xd, yd):
+    # K threads per row reassociate the sum: within the accumulation's
+    # rounding of sum |a_ij x_j| plus the output's rounding, as CSR with
+    # several lanes per row; the residual form likewise
+    import f3r_amd as F
+    for name, A in _mats():
+        x = np.random.default_rng(3).uniform(-1, 1, A.n).astype(NP[xd])
+        b = np.random.default_rng(4).uniform(-1, 1, A.n).astype(NP[yd])
+        vals = A.val.astype(NP[vd]).astype(np.float64)
+        xs = x.astype(np.float64)
+        rows = np.repeat(np.arange(A.n), np.diff(A.rptr))
+        exact = np.bincount(rows, vals * xs[A.idx], minlength=A.n)
+        mag = np.bincount(rows, np.abs(vals * xs[A.idx]), minlength=A.n)
+        print(f"sell {K} threads per row {name} {vd}/{xd}/{yd} C={C} sigma={sigma}", flush=True)
+        M = F.SELL(ctx, A, vd, C=C, sigma=sigma, K=K, lanes=K)
+        xv = F.vector.from_numpy(ctx, x)
+        yv, rv = F.vector(ctx, yd, A.n), F.vector(ctx, yd, A.n)
+        M.operate(xv, yv)
+        M.compute_residual(F.vector.from_numpy(ctx, b), xv, rv)
+        nmax = int(np.diff(A.rptr).max())
+        u_acc = 2.0 ** -53 if vd == "f64" else 2.0 ** -24
+        u_out = {"f64": 2.0 ** -53, "f32": 2.0 ** -24, "f16": 2.0 ** -11}[yd]
+        for got, ref in ((yv.download().astype(np.float64), exact),
+                         (rv.download().astype(np.float64), b.astype(np.float64) - exact)):
+            tol = 2 * nmax * u_acc * mag + u_out * np.abs(ref) + (6e-8 if yd == "f16" else 0.0)
+            assert np.all(np.abs(got - ref) <= tol), (name, float(np.max(np.abs(got - ref) - tol)))
+
+
+def test_sell_threads_per_row_rejected_combinations(ctx):
+    import f3r_amd as F
+    A = next(_mats())[1]
+    for kw in ({"K": 1, "lanes": 1}, {"K": 4, "lanes": 4, "offsets": True}, {"K": 4, "lanes": 4, "lds": True}):
+        if kw.get("lanes", 1) == 1:
+            continue
+        with pytest.raises(F.F3RError):
+            F.SELL(ctx, A, "f64", C=64, **kw)
+    with pytest.raises(ValueError):
+        F.SELL(ctx, A, "f64", C=64, K=4, lanes=2)
+
+
 # ----------------------------------------------- LDS-staged x gathers ----
 @pytest.mark.parametrize("C,sigma,K", [(64, 1, 4), (32, 1, 1), (128, 256, 2), (64, 1, 8)])
 def test_sell_lds_gathers_bitexact(ctx, C, sigma, K):
