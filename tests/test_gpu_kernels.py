@@ -373,6 +373,35 @@ def test_sell_threads_per_row(ctx, C, sigma, K, vd, xd, yd):
             assert np.all(np.abs(got - ref) <= tol), (name, float(np.max(np.abs(got - ref) - tol)))
 
 
+@pytest.mark.parametrize("n", [1, 5, 63, 65, 4097])
+def test_device_build_and_threads_per_row_edge_sizes(ctx, n):
+    # sizes around one chunk, one wavefront and one sigma window (4096 rows):
+    # the device build equals the host build (sigma 4096 and 16, K 4 and 2),
+    # and K threads per row stay within the rounding bound of the fp64 sum
+    import f3r_amd as F
+    A = F.CSR(*PB.random_sparse(n, 12, 100 + n))
+    x = np.random.default_rng(n).uniform(-1, 1, n)
+    xv = F.vector.from_numpy(ctx, x)
+    rows = np.repeat(np.arange(n), np.diff(A.rptr))
+    exact = np.bincount(rows, A.val * x[A.idx], minlength=n)
+    mag = np.bincount(rows, np.abs(A.val * x[A.idx]), minlength=n)
+    nmax = int(np.diff(A.rptr).max())
+    for C, sigma, K in ((64, 4096, 4), (32, 16, 2)):
+        print(f"edge n={n} C={C} sigma={sigma} K={K}", flush=True)
+        Hs = F.SELL(ctx, A, "f64", C=C, sigma=sigma, K=K)
+        Ds = F.SELL(ctx, A, "f64", C=C, sigma=sigma, K=K, device=True)
+        assert Ds.where() == "device" and Ds.nslots() == Hs.nslots()
+        y1, y2 = F.vector(ctx, "f64", n), F.vector(ctx, "f64", n)
+        Hs.operate(xv, y1)
+        Ds.operate(xv, y2)
+        assert np.array_equal(y1.download(), y2.download())
+        Ls = F.SELL(ctx, A, "f64", C=C, sigma=sigma, K=K, lanes=K, device=True)
+        y3 = F.vector(ctx, "f64", n)
+        Ls.operate(xv, y3)
+        tol = 2 * nmax * 2.0 ** -53 * mag + 2.0 ** -53 * np.abs(exact)
+        assert np.all(np.abs(y3.download() - exact) <= tol)
+
+
 def test_sell_threads_per_row_rejected_combinations(ctx):
     import f3r_amd as F
     A = next(_mats())[1]
