@@ -300,3 +300,30 @@ def test_lockstep_many_outer_steps_graph_cap():
     for x, b in zip(out[True][2], bs):
         assert _true_res(A, x, b) <= 1e-9
     ctx.close()
+
+
+def test_lockstep_restart_cycles_graph_equals_eager():
+    """outer cycles of m1 = 5 steps: the whole-step graphs of steps 0..4 are
+    replayed in every restart cycle; the bits match the eager launches"""
+    import numpy as np
+    import f3r_amd as F
+    ctx = F.Context(0)
+    A = F.CSR.dominant(7, 16, margin=0.05)
+    F.algorithm.scaling(A)
+    P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+    bs = [F.uniform(A.n, -1.0, 1.0, seed=21 + r) for r in range(2)]
+    bv = [F.vector.from_numpy(ctx, b) for b in bs]
+    out = {}
+    for g in (False, True):
+        B = F.Batch(ctx, [A, A], [P, P], m=(5, 1, 1, 1), restart=20, use_graph=g)
+        xv = [F.vector(ctx, "f64", A.n).fill(0.0) for _ in bs]
+        res = B.solve(bv, xv)
+        out[g] = ([r.res_iter for r in res], [r.outer_cycles for r in res], [x.download() for x in xv],
+                  [r.is_solved for r in res])
+        print(f"graph={g}: outer steps {out[g][0]} cycles {out[g][1]}", flush=True)
+    assert out[False][0] == out[True][0] and out[False][1] == out[True][1]
+    assert all(np.array_equal(a, b) for a, b in zip(out[False][2], out[True][2]))
+    assert all(out[True][3]) and min(out[True][1]) >= 3
+    for x, b in zip(out[True][2], bs):
+        assert _true_res(A, x, b) <= 1e-8
+    ctx.close()
