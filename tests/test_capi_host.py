@@ -304,3 +304,21 @@ def test_tuner_random_rows_problem():
     # the SELL threads-per-row candidates: K lanes for K >= 2, int32 indices, no staging
     sells = [e for e in candidates(tprs=()) if e["lanes"] > 1]
     assert sells and all(e["lanes"] == e["K"] >= 2 and not e["offsets"] and not e["lds"] for e in sells)
+
+
+def test_header_compiles_as_c99(tmp_path):
+    # the boundary header is plain C: a C caller (cgo-, ctypes- or JNI-style
+    # bindings) includes it without a C++ compiler
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use.c"
+    src.write_text('#include "f3r_amd.h"\n'
+                   "int main(void) { f3r_params p; f3r_result r; (void)r;\n"
+                   "  return f3r_params_default(&p) == F3R_OK && p.m1 > 0 ? 0 : 1; }\n")
+    res = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                          os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "use.o")],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
