@@ -322,3 +322,20 @@ def test_header_compiles_as_c99(tmp_path):
                           os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "use.o")],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_setup_where_argument_checks():
+    # the build-location queries refuse null handles and a CSR matrix (they
+    # report SELL layouts), without touching a GPU
+    from f3r_amd._capi import lib
+    w = ctypes.c_int(-7)
+    assert lib.f3r_mat_setup_where(None, ctypes.byref(w)) == -1 and w.value == -7
+    assert lib.f3r_solver_setup_where(None, ctypes.byref(w)) == -1 and w.value == -7
+    rp, ix, v = np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.ones(2)
+    h = ctypes.c_void_p()
+    assert lib.f3r_mat_create_csr(None, 2, 2, 2, rp.ctypes.data, ix.ctypes.data, v.ctypes.data, ctypes.byref(h)) == 0
+    try:
+        assert lib.f3r_mat_setup_where(h, None) == -1
+        assert lib.f3r_mat_setup_where(h, ctypes.byref(w)) == -1 and b"SELL" in lib.f3r_last_error()
+    finally:
+        lib.f3r_mat_destroy(h)
