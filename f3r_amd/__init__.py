@@ -760,10 +760,11 @@ class Batch:
     Hessenberg/Givens updates and Richardson weights are per segment.  Ps are
     SDAINV preconditioners (fp16).  Solves the HALF nest of F3R.  use_graph:
     the level-2 body is captured in a HIP graph per probe pattern and
-    replayed (the same launches, the same bits)."""
+    replayed (the same launches, the same bits).  lanes: the SpMVs with K
+    threads per row (needs index = 0 and K >= 2; the sums are reassociated)."""
 
     def __init__(self, ctx, As, Ps, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, C=0, K=0, index=1,
-                 range_scale=0.0, use_graph=True):
+                 range_scale=0.0, use_graph=True, lanes=False):
         self.ctx = ctx
         self.n = [A.n for A in As]
         p = _capi.Params()
@@ -772,6 +773,7 @@ class Batch:
         p.c, p.restart, p.eps = c, restart, eps
         p.C, p.K, p.index, p.range_scale = C, K, index, range_scale
         p.use_graph = 1 if use_graph else 0
+        p.flags = (p.flags & ~16) | (16 if lanes else 0)
         ns = len(As)
         ah = (ctypes.c_void_p * ns)(*[A.handle(ctx) for A in As])
         ph = (ctypes.c_void_p * ns)(*[P.handle for P in Ps])

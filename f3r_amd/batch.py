@@ -140,30 +140,34 @@ def run(count=64, nrhs=4, streams=4, seed=0, tune=True, device=0,
 
 
 def tune_lockstep(ctx, fam, log=None):
-    """Batch layout keywords (C, K, index) autotuned on the family's largest
-    27-point system, the lockstep SpMVs' dominant cost (chunk heights 64 to 256,
-    sigma 1: the lockstep layout keeps each segment's rows in order)"""
+    """Batch layout keywords (C, K, index, lanes) autotuned on the family's
+    largest 27-point system, the lockstep SpMVs' dominant cost: chunk heights
+    64 to 256, K, the index format and K threads per row (sigma 1: the
+    lockstep layout keeps each segment's rows in order)"""
     i, npts, nx, delta = max((t for t in fam if t[1] == 27), key=lambda t: t[2])
     A = make_system(npts, nx, delta)
     best, table = autotune(ctx, A, "f16", Cs=(64, 128, 256), sigmas=(1,), Ks=(1, 2, 4, 8), offsets=(False, True),
-                           lds=(False,), tprs=(), sell_lanes=(False,), reps=10)
+                           lds=(False,), tprs=(), sell_lanes=(False, True), reps=10)
     L = solver_layout(best)
+    lanes = best.get("lanes", 1) > 1
     if log:
         log(f"lockstep layout tuned on system {i} ({npts}-point {nx}^3): C={L['C']} K={L['K']} index={L['index']} "
-            f"({best['ms'] * 1e3:.1f} us; slowest candidate {table[-1]['ms'] * 1e3:.1f} us)")
-    return {"C": L["C"], "K": L["K"], "index": L["index"]}
+            f"threads per row {best.get('lanes', 1)} ({best['ms'] * 1e3:.1f} us; slowest candidate "
+            f"{table[-1]['ms'] * 1e3:.1f} us)")
+    return {"C": L["C"], "K": L["K"], "index": L["index"], "lanes": lanes}
 
 
-def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False, use_graph=True):
+def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False, use_graph=True, layout=None):
     """The same family solved in lockstep (f3r_batch): one block-diagonal
     layout of all systems x right-hand sides, every launch covering all of
     them; per-system SDAINV factorised on the device.  tune: the layout
-    (C, K, index format) autotuned first (tune_lockstep).  use_graph: the
-    level-2 body replayed from HIP graphs."""
+    (C, K, index format, threads per row) autotuned first (tune_lockstep);
+    layout: Batch keywords given instead.  use_graph: each outer step
+    replayed from a HIP graph."""
     from . import Batch
     ctx = Context(device)
     fam = family(seed, count)
-    lay = tune_lockstep(ctx, fam, log) if tune else {}
+    lay = tune_lockstep(ctx, fam, log) if tune else dict(layout or {})
     t0 = time.time()
     As, Ps, bs, meta = [], [], [], []
     for (i, npts, nx, delta) in fam:
@@ -205,7 +209,7 @@ def run_lockstep(count=64, nrhs=4, seed=0, device=0, log=None, tune=False, use_g
            "setup_s": setup_s, "all_solved": all(x.is_solved for x in res),
            "outer_steps_max": int(max(x.res_iter for x in res)), "outer_steps_mean": float(np.mean([x.res_iter for x in res])),
            "outer_steps_per_segment": [int(x.res_iter) for x in res],
-           "layout": lay or {"C": 64, "K": 0, "index": 1},
+           "layout": lay or {"C": 64, "K": 0, "index": 1, "lanes": False},
            "worst_true_rel_res": worst}
     if log:
         log(f"lockstep: {segs} solves in {out['wall_s'] * 1e3:.1f} ms wall (best of 3), all solved "
@@ -224,7 +228,9 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="no layout autotuning")
     ap.add_argument("--mode", choices=["lockstep", "streams"], default="lockstep",
                     help="lockstep: all solves in shared launches (f3r_batch); streams: one solver per system")
-    ap.add_argument("--no-graph", action="store_true", help="lockstep: launch the level-2 body eagerly")
+    ap.add_argument("--no-graph", action="store_true", help="lockstep: every kernel launched eagerly (no graphs)")
+    ap.add_argument("--layout", default=None, metavar="C,K,INDEX[,lanes]",
+                    help="lockstep: this layout instead of the tuned one, e.g. 64,4,0,lanes (K threads per row)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     t0 = time.time()
@@ -233,7 +239,12 @@ def main(argv=None):
         print(f"[batch {time.time() - t0:6.1f}s] {m}", flush=True)
 
     if a.mode == "lockstep":
-        out = run_lockstep(a.count, a.rhs, a.seed, log=log, tune=not a.no_tune, use_graph=not a.no_graph)
+        lay = None
+        if a.layout:
+            f = a.layout.split(",")
+            lay = {"C": int(f[0]), "K": int(f[1]), "index": int(f[2]), "lanes": len(f) > 3 and f[3] == "lanes"}
+        out = run_lockstep(a.count, a.rhs, a.seed, log=log, tune=not a.no_tune and lay is None,
+                           use_graph=not a.no_graph, layout=lay)
     else:
         out = run(a.count, a.rhs, a.streams, a.seed, not a.no_tune, log=log)
     if a.out:

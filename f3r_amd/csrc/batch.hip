@@ -479,6 +479,59 @@ __global__ __launch_bounds__(kBlock) void k_sell_mr(SellArgs a, const V *__restr
   }
 }
 
+// The multi-copy SpMV with K threads per row (int32-index layout): lane k of
+// a matrix row sums slot k of each of the row's groups for all NR copies,
+// then the K lanes add their sums by shuffles (the sums are reassociated).
+template <class V, class X, class ACC, int K, int NR, class Y>
+__global__ __launch_bounds__(kBlock) void k_sell_mr_lanes(SellArgs a, const V *__restrict__ val,
+                                                          const X *__restrict__ x, const int32_t *__restrict__ cshift,
+                                                          const int32_t *__restrict__ cstride, Y *__restrict__ y,
+                                                          const Y *__restrict__ b, const int32_t *__restrict__ cseg,
+                                                          const float *__restrict__ ysc) {
+  static_assert(K >= 2 && kWave % K == 0, "K lanes of a row lie in one wavefront");
+  constexpr int kRowsPerBlock = kBlock / K;
+  const int k = threadIdx.x % K;
+  for (int r0 = xcd_block() * kRowsPerBlock; r0 < a.n; r0 += gridDim.x * kRowsPerBlock) {
+    const int r = r0 + threadIdx.x / K;
+    ACC acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0;
+    int s = 0;
+    int64_t vr = 0, stride = 0;
+    if (r < a.n) {
+      s = r >> a.log2C;
+      const int j = r - (s << a.log2C);
+      const int C = 1 << a.log2C;
+      const int rem = a.n - (s << a.log2C);
+      const int tc = rem < C ? rem : C;
+      const int groups = chunk_groups(a.cwidth[s]);
+      const int64_t shift = cshift[s];
+      stride = cstride[s];
+      vr = (int64_t)r + shift;
+      int64_t pos = (int64_t)a.cptr[s] + (int64_t)j * K + k;
+      const int64_t step = (int64_t)tc * K;
+      for (int g = 0; g < groups; ++g, pos += step) {
+        const ACC v = (ACC)to_acc(val[pos]);
+        const int64_t col = (int64_t)a.idx[pos] + shift;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = fma_<ACC>(v, (ACC)to_acc(x[col + (int64_t)c * stride]), acc[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+#pragma unroll
+      for (int off = K / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, kWave);
+    if (r < a.n && k == 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const int64_t o = vr + (int64_t)c * stride;
+        const ACC sum = ysc ? acc[c] * (ACC)ysc[cseg[s] + c] : acc[c];
+        y[o] = b ? cvt<Y>((ACC)to_acc(b[o]) - sum) : cvt<Y>(sum);
+      }
+    }
+  }
+}
+
 // An SDAINV factor split by row kind: a row whose only entry is its diagonal
 // is an elementwise product (dval), any other row is a CSR row summed in
 // column order with fp32 accumulation, as the SELL SpMV sums it; dval holds
@@ -707,6 +760,8 @@ struct BatchSolver {
   // NR > 1: each system's matrix is stored once for its NR right-hand sides
   // (chunk -> vector-row shift and copy stride, per layout)
   int NR = 1;
+  // flags bit 4: the SpMVs with K threads per row (int32-index layouts)
+  bool lanes = false;
   DevPtr csh, cst, csh16, cst16;
   Factor16 W, Z;
   DevPtr dsh;  // per segment: vector row -> row of the factor diagonals
@@ -750,16 +805,30 @@ struct BatchSolver {
     const SellArgs a = sell_args(L, false);
     if (NR == 1) {
       F3R_CHECK(ysc == nullptr, F3R_ERR_STATE, "batch: scaled SpMV needs shared matrices");
-      if (b)
+      if (lanes) {
+        if (b)
+          launch_sell_lanes<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
+        else
+          launch_sell_lanes<V, X, ACC>(L.K, a, val, x, EpiStore<Y, ACC>{y, nullptr}, st());
+      } else if (b) {
         launch_sell<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
-      else
+      } else {
         launch_sell<V, X, ACC>(L.K, a, val, x, EpiStore<Y, ACC>{y, nullptr}, st());
+      }
       return;
     }
     const int32_t *shf = (half_layout ? csh16 : csh)->as<int32_t>(), *str = (half_layout ? cst16 : cst)->as<int32_t>();
     const dim3 g((a.n + kBlock - 1) / kBlock);
     auto go = [&](auto kt, auto nt) {
       constexpr int K = decltype(kt)::value, R = decltype(nt)::value;
+      if constexpr (K >= 2) {
+        if (lanes) {  // K threads per matrix row
+          const dim3 gl((unsigned)(((int64_t)a.n * K + kBlock - 1) / kBlock));
+          hipLaunchKernelGGL((k_sell_mr_lanes<V, X, ACC, K, R, Y>), gl, dim3(kBlock), 0, st(), a, val, x, shf, str,
+                             y, b, (const int32_t *)cseg->as<int32_t>(), ysc);
+          return;
+        }
+      }
       hipLaunchKernelGGL((k_sell_mr<V, X, ACC, K, R, Y>), g, dim3(kBlock), 0, st(), a, val, x, (int64_t)N, shf, str, y,
                          b, (const int32_t *)cseg->as<int32_t>(), ysc);
     };
@@ -1153,9 +1222,12 @@ BatchSolver *batch_create(std::shared_ptr<Context> ctx, const std::vector<std::s
             "batch: C must be a power of two in [64, 2048] (0 = 64)");
   F3R_CHECK(p.K == 0 || p.K == 1 || p.K == 2 || p.K == 4 || p.K == 8, F3R_ERR_ARG,
             "batch: K must be 1, 2, 4 or 8 (0 = 4)");
+  F3R_CHECK(!(p.flags & 16) || (p.index == 0 && p.K != 1), F3R_ERR_ARG,
+            "batch: K threads per row (flags bit 4) need the int32-index layout (index 0) and K >= 2");
   auto B = std::make_unique<BatchSolver>();
   B->ctx = ctx;
   B->p = p;
+  B->lanes = (p.flags & 16) != 0;
   B->nseg = (int)A.size();
   const int nseg = B->nseg;
   B->off.assign(nseg + 1, 0);

@@ -327,3 +327,41 @@ def test_lockstep_restart_cycles_graph_equals_eager():
     for x, b in zip(out[True][2], bs):
         assert _true_res(A, x, b) <= 1e-8
     ctx.close()
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+def test_lockstep_threads_per_row(nrhs):
+    """the lockstep SpMVs with K threads per row (int32-index layout): every
+    segment solved to 1e-8 within one outer step of the one-thread-per-row
+    batch, and graph replay bit-identical to eager launches; nrhs = 1 runs
+    the single-copy kernels, 2 and 4 the multi-copy ones"""
+    import numpy as np
+    import f3r_amd as F
+    ctx = F.Context(0)
+    As, Ps, bs = [], [], []
+    for (i, npts, nx, delta, A) in _family(F, 4, seed=5):
+        P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+        for r in range(nrhs):
+            As.append(A)
+            Ps.append(P)
+            bs.append(F.uniform(A.n, -1.0, 1.0, seed=41 * i + r))
+    bv = [F.vector.from_numpy(ctx, b) for b in bs]
+    out = {}
+    for name, kw in (("one", dict(index=0)), ("lanes", dict(index=0, lanes=True)),
+                     ("lanes eager", dict(index=0, lanes=True, use_graph=False))):
+        B = F.Batch(ctx, As, Ps, **kw)
+        xv = [F.vector(ctx, "f64", A.n).fill(0.0) for A in As]
+        res = B.solve(bv, xv)
+        out[name] = ([r.res_iter for r in res], [x.download() for x in xv], [r.is_solved for r in res])
+        print(f"nrhs={nrhs} {name}: outer steps {out[name][0]}", flush=True)
+    assert all(out["lanes"][2])
+    assert all(abs(a - b) <= 1 for a, b in zip(out["one"][0], out["lanes"][0]))
+    assert out["lanes"][0] == out["lanes eager"][0]
+    assert all(np.array_equal(a, b) for a, b in zip(out["lanes"][1], out["lanes eager"][1]))
+    for s, (A, b) in enumerate(zip(As, bs)):
+        assert _true_res(A, out["lanes"][1][s], b) <= 1e-8
+    # K threads per row need the int32-index layout and K >= 2
+    for kw in (dict(index=1, lanes=True), dict(index=0, K=1, lanes=True)):
+        with pytest.raises(F.F3RError):
+            F.Batch(ctx, As, Ps, **kw)
+    ctx.close()
