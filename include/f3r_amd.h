@@ -422,7 +422,8 @@ int f3r_solve_cond(f3r_solver S, f3r_vec b, f3r_vec x, f3r_cond_fn cond, void *u
  * of all segments is one HIP graph launch, after which the host reads one
  * array of residual norms.  The nest and its per-segment semantics
  * are f3r_solver_create's HALF build (params: m1..m4, c, restart, eps, C, K,
- * index, range_scale; 0 = the per-segment automatic range factor).  The
+ * index, range_scale; 0 = the per-segment automatic range factor; flags bit
+ * 4: the SpMVs with K threads per row, which needs index 0 and K >= 2).  The
  * reference solves one system per run (work/src/f3r-gpu.cpp:44-288); this
  * has no counterpart there. */
 typedef struct f3r_batch_s *f3r_batch;
