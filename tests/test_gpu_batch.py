@@ -365,3 +365,35 @@ def test_lockstep_threads_per_row(nrhs):
         with pytest.raises(F.F3RError):
             F.Batch(ctx, As, Ps, **kw)
     ctx.close()
+
+
+def test_lockstep_many_small_segments():
+    """1024 segments (256 systems of 8^3 to 12^3 rows, 4 right-hand sides
+    each), each padded to a 2048-row tile: every segment's reductions,
+    rotations and weights stay its own, and every solve reaches 1e-8"""
+    import numpy as np
+    import f3r_amd as F
+    from f3r_amd import batch
+    ctx = F.Context(0)
+    rng = np.random.default_rng(17)
+    As, Ps, bs, uniq = [], [], [], []
+    for i in range(256):
+        npts = 7 if i % 2 == 0 else 27
+        A = batch.make_system(npts, int(rng.integers(8, 13)), float(rng.uniform(0.5, 2.0)))
+        P = F.SDAINV(A, 0.1, 1.0, "f16", ctx=ctx)
+        uniq.append(A)
+        for r in range(4):
+            As.append(A)
+            Ps.append(P)
+            bs.append(F.uniform(A.n, -1.0, 1.0, seed=1000 * i + r))
+    B = F.Batch(ctx, As, Ps)
+    rows, segs = B.info()
+    assert segs == 1024 and rows == 1024 * 2048
+    bv = [F.vector.from_numpy(ctx, b) for b in bs]
+    xv = [F.vector(ctx, "f64", A.n).fill(0.0) for A in As]
+    res = B.solve(bv, xv)
+    print(f"1024 segments: outer steps {min(r.res_iter for r in res)}..{max(r.res_iter for r in res)}", flush=True)
+    assert all(r.is_solved for r in res)
+    worst = max(_true_res(A, x.download(), b) for A, x, b in zip(As, xv, bs))
+    assert worst <= 1e-8, worst
+    ctx.close()
