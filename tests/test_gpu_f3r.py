@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+import problems as PB
 
 pytestmark = pytest.mark.gpu
 
@@ -155,6 +156,29 @@ def test_solve_matches_oracle(ctx, kind, m, prec):
     assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter))), (res.res_iter, ro.res_iter)
     h = S.history()
     assert len(h) >= res.res_iter and h[-1] < 1e-8
+
+
+@pytest.mark.parametrize("sigma", [1, 256])
+def test_solve_irregular_matrix(ctx, sigma):
+    # random columns and row lengths (1..24 entries, weak diagonal dominance):
+    # int32-index chunks, general SDAINV factors (CSR applies, no diagonal
+    # fast path), rows sorted inside sigma windows and the matrix renumbered
+    import f3r_amd as F
+    A = F.CSR(*PB.random_sparse(12000, 24, 29, diag_dom=1.02))
+    F.algorithm.scaling(A)
+    P = _prec(F, A, ctx, "sdainv")
+    s = _scale(A.n)
+    b = F.uniform(A.n, -1.0, 1.0, seed=6)
+    S = F.F3R(ctx, A, P, range_scale=s, sigma=sigma, C=64)
+    bv = F.vector.from_numpy(ctx, b)
+    xv = F.vector(ctx, "f64", A.n).fill(0.0)
+    print(f"solve irregular sigma={sigma}", flush=True)
+    res = S.solve(bv, xv)
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=s)
+    print(f"  gpu {res.res_iter} steps, oracle {ro.res_iter}", flush=True)
+    assert res.is_solved and ro.is_solved
+    assert _true_res(A, xv.download(), b) <= 1e-8
+    assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter))), (res.res_iter, ro.res_iter)
 
 
 def test_graph_replay_identical_to_eager(ctx):
