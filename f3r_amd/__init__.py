@@ -532,11 +532,12 @@ class F3R:
     static_weight > 0: the Richardson level with that fixed weight and no probes
     (work/src/static.cpp:99-119).  depth = "F2", "F2H", "F3", "F3H" or "F4": the
     nesting-depth ablations of work/src/depth.cpp:122-167 instead of the F3R
-    nest (M applied directly under fp32 / fp16-basis FGMRES levels)."""
+    nest (M applied directly under fp32 / fp16-basis FGMRES levels).  lanes:
+    every SpMV with K threads per row (index = 0; the sums are reassociated)."""
 
     def __init__(self, ctx, A, M, m=(100, 8, 4, 2), c=64, restart=20, eps=1e-8, variant="half",
                  range_scale=0.0, C=0, sigma=0, K=0, use_graph=True, history=True, grid=0, index=1,
-                 cap_spmv=False, device_build=False, depth=0, static_weight=0.0):
+                 cap_spmv=False, device_build=False, depth=0, static_weight=0.0, lanes=False):
         self.ctx = ctx
         self.A = A
         self.M = M
@@ -548,7 +549,8 @@ class F3R:
         p.range_scale = range_scale
         p.C, p.sigma, p.K = C, sigma, K
         p.use_graph = 1 if use_graph else 0
-        p.flags = (1 if history else 0) | (2 if cap_spmv else 0) | (4 if device_build else 0)
+        p.flags = (1 if history else 0) | (2 if cap_spmv else 0) | (4 if device_build else 0) | \
+            (32 if lanes else 0)
         p.grid = grid
         p.index = index
         p.depth = DEPTHS[depth] if isinstance(depth, str) else int(depth)

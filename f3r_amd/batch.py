@@ -54,7 +54,7 @@ class _Worker:
                     # the solver's fused kernels take one thread per row
                     best, _ = autotune(self.ctx, A, "f16", tprs=(), lds=(False,), sell_lanes=(False,), **self.cands)
                     L = solver_layout(best)
-                    lay = (L["C"], L["sigma"], L["K"], L["index"])
+                    lay = (L["C"], L["sigma"], L["K"], L["index"])  # one thread per row (tuned so)
                 else:
                     lay = (64, 1, 4, 1)
                 P = SDAINV(A, 0.1, 1.0, "f16", ctx=self.ctx)

@@ -802,19 +802,14 @@ struct BatchSolver {
   // ysc (NR > 1 only): per-segment factor on the sums (x stored scaled)
   void spmv(const V *val, bool half_layout, const X *x, Y *y, const Y *b, const float *ysc = nullptr) {
     const SellShared &L = half_layout ? *sh16 : *sh;
-    const SellArgs a = sell_args(L, false);
+    SellArgs a = sell_args(L, false);
     if (NR == 1) {
       F3R_CHECK(ysc == nullptr, F3R_ERR_STATE, "batch: scaled SpMV needs shared matrices");
-      if (lanes) {
-        if (b)
-          launch_sell_lanes<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
-        else
-          launch_sell_lanes<V, X, ACC>(L.K, a, val, x, EpiStore<Y, ACC>{y, nullptr}, st());
-      } else if (b) {
+      a.lanes = lanes ? L.K : 1;  // K threads per row
+      if (b)
         launch_sell<V, X, ACC>(L.K, a, val, x, EpiResid<Y, Y, ACC>{y, b, nullptr}, st());
-      } else {
+      else
         launch_sell<V, X, ACC>(L.K, a, val, x, EpiStore<Y, ACC>{y, nullptr}, st());
-      }
       return;
     }
     const int32_t *shf = (half_layout ? csh16 : csh)->as<int32_t>(), *str = (half_layout ? cst16 : cst)->as<int32_t>();

@@ -508,6 +508,7 @@ SellArgs sell_args(const SellShared &sh, bool with_perm) {
   a.iptr = sh.iptr ? sh.iptr->as<int32_t>() : nullptr;
   a.coff = sh.coff ? sh.coff->as<int32_t>() : nullptr;
   a.off_bias = sh.off_bias;
+  a.lanes = sh.lanes;
   return a;
 }
 
@@ -619,16 +620,7 @@ static void sell_apply_t(const Mat *A, const void *x, void *y, const void *b, hi
                                  EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
     return;
   }
-  if (A->sh->lanes > 1) {  // K lanes per row
-    if (b)
-      launch_sell_lanes<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
-                                   EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
-    else
-      launch_sell_lanes<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
-                                   EpiStore<Y, ACC>{(Y *)y, a.perm}, st);
-    return;
-  }
-  if (b)
+  if (b)  // K threads per row when the layout has them (a.lanes)
     launch_sell<V, X, ACC>(A->sh->K, a, (const V *)A->s_val->p, (const X *)x,
                            EpiResid<Y, Y, ACC>{(Y *)y, (const Y *)b, a.perm}, st);
   else

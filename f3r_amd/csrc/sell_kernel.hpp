@@ -42,6 +42,9 @@ struct SellArgs {
   const int32_t *__restrict__ iptr;
   const int32_t *__restrict__ coff;
   int32_t off_bias;  // -(smallest column offset), >= 0: makes every scalar offset non-negative
+  // threads per row: 1, or K (int32-index chunks only; launch_sell then runs
+  // k_sell_lanes, whose sums are reassociated)
+  int32_t lanes = 1;
 };
 
 // word 3 of a raw buffer resource on gfx9-family GPUs (no format, no swizzle)
@@ -275,18 +278,6 @@ __global__ __launch_bounds__(kBlock) void k_sell_lanes(SellArgs a, const V *__re
   epi.finish(st);
 }
 
-template <class V, class X, class ACC, class Epi>
-inline void launch_sell_lanes(int K, const SellArgs &a, const V *val, const X *x, Epi epi, hipStream_t st) {
-  if (a.n == 0) return;
-  auto g = [&](int k) { return dim3((unsigned)(((int64_t)a.n * k + kBlock - 1) / kBlock)); };
-  switch (K) {
-    case 2: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 2, Epi>), g(2), dim3(kBlock), 0, st, a, val, x, epi); break;
-    case 4: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 4, Epi>), g(4), dim3(kBlock), 0, st, a, val, x, epi); break;
-    case 8: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 8, Epi>), g(8), dim3(kBlock), 0, st, a, val, x, epi); break;
-    default: break;
-  }
-}
-
 // Row r of an int32-index SELL with x staged in LDS: xs[c - lo] holds x[c]
 // for the block's column window [lo, lo + len).  The slots and the sum are
 // those of sell_row (same bits); only the source of x differs.
@@ -354,13 +345,28 @@ inline void launch_sell_lds(int K, const SellArgs &a, const V *val, const X *x, 
   }
 }
 
+// blocks launch_sell runs for `grid` (0 = one row per thread, or per K
+// threads with K threads per row; otherwise that block count when smaller)
+inline int sell_blocks(const SellArgs &a, int grid) {
+  const int64_t full = ((int64_t)a.n * (a.lanes > 1 ? a.lanes : 1) + kBlock - 1) / kBlock;
+  return (int)(grid > 0 && grid < full ? grid : full);
+}
+
 // grid: 0 = one row per thread; otherwise the given (capped) block count
 template <class V, class X, class ACC, class Epi>
 inline void launch_sell(int K, const SellArgs &a, const V *val, const X *x, Epi epi,
                         hipStream_t st, int grid = 0) {
   if (a.n == 0) return;
-  const int full = (a.n + kBlock - 1) / kBlock;
-  dim3 g(grid > 0 && grid < full ? grid : full);
+  const dim3 g(sell_blocks(a, grid));
+  if (a.lanes > 1) {
+    switch (K) {
+      case 2: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 2, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+      case 4: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 4, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+      case 8: hipLaunchKernelGGL((k_sell_lanes<V, X, ACC, 8, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
+      default: break;
+    }
+    return;
+  }
   switch (K) {
     case 1: hipLaunchKernelGGL((k_sell<V, X, ACC, 1, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;
     case 2: hipLaunchKernelGGL((k_sell<V, X, ACC, 2, Epi>), g, dim3(kBlock), 0, st, a, val, x, epi); break;

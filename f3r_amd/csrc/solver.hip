@@ -809,13 +809,9 @@ struct Solver {
     } else {
       launch_sell<V, T, T>(shv<V>().K, argsv<V>(), val, L.Z[j], epi, st(), spmv_grid);
     }
+    // block partials the SpMV + dots kernel wrote per dot: its grid
     hipLaunchKernelGGL(k_dots_finish<T>, dim3(j + 1), dim3(1024), 0, st(), (const T *)partial<T>(),
-                       dots_partials(), L.Hp() + (int64_t)j * L.ldh());
-  }
-  // block partials the SpMV + dots kernel wrote per dot (its grid)
-  int dots_partials() const {
-    const int full = (int)((n + kBlock - 1) / kBlock);  // the grid launch_sell uses
-    return spmv_grid > 0 && spmv_grid < full ? spmv_grid : full;
+                       sell_blocks(argsv<V>(), spmv_grid), L.Hp() + (int64_t)j * L.ldh());
   }
   template <class T, int NV> void cgs_update(InnerLevel<T> &L, int j) {
     Cols<T, NV> c{};
@@ -1299,6 +1295,8 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
   const int K16 = p.K > 0 ? p.K : (A.n > 0 && A.nnz() >= (int64_t)16 * A.n ? 8 : 4);
   F3R_CHECK(C >= 16 && (C & (C - 1)) == 0, F3R_ERR_ARG, "solver: C must be a power of two >= 16");
   F3R_CHECK(K == 1 || K == 2 || K == 4 || K == 8, F3R_ERR_ARG, "solver: K must be 1, 2, 4 or 8");
+  F3R_CHECK(!(p.flags & 32) || (p.index == 0 && K >= 2 && K16 >= 2), F3R_ERR_ARG,
+            "solver: K threads per row (flags bit 5) need the int32-index layout (index 0) and K >= 2");
   // flags bit 2: build the layouts on the device from the CSR's device copy
   // (the same arrays as the host build)
   const bool on_dev = (p.flags & 4) && dev && dev->d_rptr && dev->ctx == ctx;
@@ -1353,6 +1351,11 @@ Solver *solver_create(std::shared_ptr<Context> ctx, const HostCSR &A, const Prec
         S->A16 = upload_sell_values(ctx.get(), h16, F3R_F16);
       }
     }
+  }
+  // flags bit 5: every SpMV with K threads per row (the sums reassociated)
+  if (p.flags & 32) {
+    S->sh->lanes = S->sh->K;
+    S->sh16->lanes = S->sh16->K;
   }
   bool ident = true;
   for (int32_t r = 0; r < A.n && ident; ++r) ident = hs.perm[r] == r;

@@ -11,8 +11,8 @@ permutation of x and y, which the solver avoids by renumbering the system,
 so the ranking leans slightly towards sigma = 1.  Every SELL layout with one
 thread per row and the one-lane-per-row CSR sum each row in CSR order, so
 they give the same bits; K threads per row (SELL) and several lanes per row
-(CSR) reassociate the sums.  The solver's fused kernels take one thread per
-row, so a K-threads winner enters the solver as its layout (C, sigma, K).
+(CSR) reassociate the sums.  The solver runs either SELL form
+(F3R(..., lanes=True) for K threads per row).
 """
 import itertools
 
@@ -80,13 +80,16 @@ def autotune(ctx, A, dtype="f16", reps=20, check=False, **space):
 
 
 def solver_layout(best):
-    """F3R / Batch keywords of a SELL winner (CSR winners map to the SELL
-    layout of one column index per slot that sums alike: C 64, K 1; the
-    solver's fused kernels gather from global memory with one thread per
-    row, so LDS staging and K threads per row are dropped)"""
+    """F3R keywords of a SELL winner (CSR winners map to the SELL layout of
+    one column index per slot that sums alike: C 64, K 1; the solver's fused
+    kernels gather from global memory, so LDS staging is dropped; a K
+    threads per row winner keeps them: lanes=True)"""
     if best["format"] == "csr":
         return {"C": 64, "sigma": 1, "K": 1, "index": 0}
-    return {"C": best["C"], "sigma": best["sigma"], "K": best["K"], "index": 1 if best["offsets"] else 0}
+    kw = {"C": best["C"], "sigma": best["sigma"], "K": best["K"], "index": 1 if best["offsets"] else 0}
+    if best.get("lanes", 1) > 1:
+        kw["lanes"] = True
+    return kw
 
 
 def random_rows(n, maxlen, seed=0):

@@ -288,7 +288,9 @@ typedef struct {
   int32_t flags;       /* bit 0: record the outer residual history; bit 1: run the SpMV
                           kernels with fused reductions on the capped grid too; bit 2:
                           build the matrix layouts on the device (as F3R_SELL_DEVICE; needs
-                          A created with a context) */
+                          A created with a context); bit 5: every SpMV with K threads per
+                          row (F3R_SELL_TPR; needs index 0 and K >= 2; the sums are
+                          reassociated, so level 4 is no longer bit-exact to CSR order) */
   int32_t grid;        /* blocks of the fused SpMV + reduction kernels; 0 = 8 per CU */
   int32_t index;       /* 0: a column index per slot (SELL); 1: chunks whose rows share their
                           column offsets store the offsets once (default; needs C >= 64) */

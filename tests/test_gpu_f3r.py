@@ -181,6 +181,37 @@ def test_solve_irregular_matrix(ctx, sigma):
     assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter))), (res.res_iter, ro.res_iter)
 
 
+@pytest.mark.parametrize("kind", ["irregular", "27pt", "lognormal"])
+def test_solve_threads_per_row(ctx, kind):
+    # every SpMV of the nest with K threads per row (int32-index layouts, the
+    # fused epilogues on the row's first lane): within +-10 % (at least one)
+    # of the oracle's outer steps, to 1e-8, and graph replay equal to eager
+    import f3r_amd as F
+    if kind == "irregular":
+        A = F.CSR(*PB.random_sparse(12000, 24, 29, diag_dom=1.02))
+        F.algorithm.scaling(A)
+    else:
+        A = _problem(kind, 20 if kind == "27pt" else 16)
+    P = _prec(F, A, ctx, "sdainv")
+    s = _scale(A.n)
+    b = F.uniform(A.n, -1.0, 1.0, seed=9)
+    bv = F.vector.from_numpy(ctx, b)
+    _, ro, _ = O.f3r_solve(A.n, A.rptr, A.idx, A.val, b, variant=0, ref_arith=0, range_scale=s)
+    xs, its = [], []
+    for g in (True, False):
+        S = F.F3R(ctx, A, P, range_scale=s, index=0, lanes=True, use_graph=g)
+        xv = F.vector(ctx, "f64", A.n).fill(0.0)
+        res = S.solve(bv, xv)
+        print(f"{kind} threads per row graph={g}: {res.res_iter} steps, oracle {ro.res_iter}", flush=True)
+        assert res.is_solved and _true_res(A, xv.download(), b) <= 1e-8
+        assert abs(res.res_iter - ro.res_iter) <= max(1, int(np.ceil(0.1 * ro.res_iter)))
+        xs.append(xv.download())
+        its.append(res.res_iter)
+    assert its[0] == its[1] and np.array_equal(xs[0], xs[1])
+    with pytest.raises(F.F3RError):
+        F.F3R(ctx, A, P, index=1, lanes=True)
+
+
 def test_graph_replay_identical_to_eager(ctx):
     import f3r_amd as F
     A = _problem("27pt", 20)
