@@ -158,7 +158,8 @@ int f3r_gen_uniform_mt(int64_t n, double lo, double hi, uint64_t seed, double *o
 
 /* SELL<w, T, device>(CSR) (sell.hpp:35-97) and SELLsig (sell.hpp:99-166):
  * C rows per chunk (multiple of 16), sigma = sort window (1 = unsorted),
- * K = slots interleaved per row (1, 2, 4 or 8; K = 1 is the reference layout).
+ * K = slots interleaved per row (1, 2, 4 or 8; K = 1 is the reference layout),
+ * and with F3R_SELL_TPR (f3r_mat_to_sell_ex) also the threads per row.
  * Rows of the result are in sorted order; y of f3r_spmv is written back to
  * the original row order. */
 int f3r_mat_to_sell(f3r_ctx ctx, f3r_mat csr, int dtype, int32_t C, int32_t sigma,
