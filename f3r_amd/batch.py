@@ -7,8 +7,8 @@ system is scaled as the reference driver scales (algorithm.hpp:45-59) and
 gets `nrhs` right-hand sides uniform in [-1, 1].
 
 Every system has its own F3R solver (SDAINV, m = (100; 8, 4, 2)) whose SELL
-layout (C, sigma, K) is autotuned on that system's fp16 matrix
-(tune.autotune).  The systems are spread over `streams` contexts, one HIP
+layout (C, sigma, K, index format, threads per row) is autotuned on that
+system's fp16 matrix (tune.autotune).  The systems are spread over `streams` contexts, one HIP
 stream each, each driven by its own host thread; the HIP library releases the
 GIL, so the streams run concurrently and small systems overlap on the GPU.
 The reference solves one system per run (work/src/f3r-gpu.cpp:44-288).
@@ -51,14 +51,13 @@ class _Worker:
                 t0 = time.time()
                 A = make_system(npts, nx, delta)
                 if self.tune:
-                    # the solver's fused kernels take one thread per row
-                    best, _ = autotune(self.ctx, A, "f16", tprs=(), lds=(False,), sell_lanes=(False,), **self.cands)
+                    best, _ = autotune(self.ctx, A, "f16", tprs=(), lds=(False,), **self.cands)
                     L = solver_layout(best)
-                    lay = (L["C"], L["sigma"], L["K"], L["index"])  # one thread per row (tuned so)
+                    lay = (L["C"], L["sigma"], L["K"], L["index"], L.get("lanes", False))
                 else:
-                    lay = (64, 1, 4, 1)
+                    lay = (64, 1, 4, 1, False)
                 P = SDAINV(A, 0.1, 1.0, "f16", ctx=self.ctx)
-                S = F3R(self.ctx, A, P, C=lay[0], sigma=lay[1], K=lay[2], index=lay[3])
+                S = F3R(self.ctx, A, P, C=lay[0], sigma=lay[1], K=lay[2], index=lay[3], lanes=lay[4])
                 bs = [vector.from_numpy(self.ctx, uniform(A.n, -1.0, 1.0, seed=(self.seed << 20) + 16 * i + r))
                       for r in range(self.nrhs)]
                 xs = [vector(self.ctx, "f64", A.n) for _ in range(self.nrhs)]
@@ -75,7 +74,8 @@ class _Worker:
                     r = S.solve(b, x)
                     outs.append((r.res_iter, r.is_solved, r.solve_s))
                 self.results.append({"system": i, "npts": npts, "nx": nx, "n": A.n, "delta": round(delta, 4),
-                                     "layout": {"C": lay[0], "sigma": lay[1], "K": lay[2], "index": lay[3]},
+                                     "layout": {"C": lay[0], "sigma": lay[1], "K": lay[2], "index": lay[3],
+                                                "lanes": lay[4]},
                                      "setup_s": round(setup_s, 3),
                                      "outer": [o[0] for o in outs], "solved": all(o[1] for o in outs),
                                      "solve_s": [o[2] for o in outs]})
